@@ -1,0 +1,163 @@
+/*
+ * dolfin_amd -- MI355X (gfx950) cell-integral assembly engine, C ABI.
+ *
+ * Drop-in for the cell-integral hot path of DOLFIN 2019.1:
+ *   dolfin::Assembler::assemble(GenericTensor&, const Form&)   (dolfin/fem/Assembler.h:66)
+ *   dolfin::SystemAssembler::assemble(GenericMatrix&, GenericVector&) (SystemAssembler.h:63-84)
+ * The host keeps mesh, dofmap and form objects; it hands this library plain arrays
+ * (cell -> vertex connectivity, cell -> dof maps, device pointers to coordinates,
+ * coefficient vectors and tensor storage) and gets back a device-resident CSR tensor whose
+ * structure equals EigenMatrix::init(diagonal_pattern(sorted)) bit for bit
+ * (dolfin/la/EigenMatrix.cpp:384-410, dolfin/la/SparsityPattern.cpp:511-542).
+ *
+ * Conventions
+ *   - All indices are int32 (dolfin::la_index, dolfin/common/types.h:32-34); values FP64.
+ *   - "h_" arguments are host pointers, "d_" arguments are device pointers.
+ *   - stream is a hipStream_t passed as void* (NULL = the null stream).
+ *   - Every function returns 0 on success and a negative code on error; the message is
+ *     available from dfa_last_error() (thread-local). No exception crosses this boundary
+ *     (the reference raises dolfin_error -> std::runtime_error, dolfin/log/Logger.cpp:142-185;
+ *     the C++ adapter rethrows).
+ *   - Calls on one handle are serialized by the caller; different handles are independent.
+ */
+#ifndef DOLFIN_AMD_H
+#define DOLFIN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes */
+#define DFA_OK 0
+#define DFA_ERR_ARG -1      /* invalid argument / unsupported combination */
+#define DFA_ERR_HIP -2      /* HIP runtime failure */
+#define DFA_ERR_LIMIT -3    /* a size limit of a kernel variant was exceeded */
+#define DFA_ERR_STATE -4    /* handle in the wrong state */
+
+/* Elements (the FunctionSpace of both form arguments). */
+#define DFA_ELEMENT_P1 1          /* scalar Lagrange degree 1 on intervals/triangles/tets */
+#define DFA_ELEMENT_VECTOR_P2 2   /* vector Lagrange degree 2 on tets, 30 dofs per cell   */
+
+/* Form kinds -- the registered counterparts of FFC-generated ufc::cell_integral objects
+   (the tabulate_tensor ABI called at dolfin/fem/Assembler.cpp:181-183). */
+#define DFA_FORM_P1_POISSON 0       /* inner(kappa*grad(u), grad(v))*dx, kappa P1        */
+#define DFA_FORM_P1_LOAD 1          /* f*v*dx, f P1                                       */
+#define DFA_FORM_P1_MASS 2          /* u*v*dx                                             */
+#define DFA_FORM_VP2_ELASTICITY 3   /* inner(2 mu eps(u) + lmbda tr(eps(u)) I, eps(v))*dx */
+#define DFA_FORM_VP2_BODY_FORCE 4   /* inner(f, v)*dx, f a constant 3-vector              */
+
+/* Scatter strategies (chosen per call, or by dfa_autotune). */
+#define DFA_SCATTER_AUTO 0
+#define DFA_SCATTER_ATOMIC 1   /* one thread per cell, FP64 atomics into CSR           */
+#define DFA_SCATTER_BLOCKED 2  /* row blocks owned by workgroups, LDS accumulation,
+                                  each CSR value written once, no global atomics       */
+
+/* Flags of AssemblerBase (dolfin/fem/AssemblerBase.h:46-67). */
+#define DFA_ADD_VALUES 1       /* do not zero the tensor before assembly               */
+
+typedef struct dfa_assembler_s* dfa_assembler;
+
+/* Last error message of the calling thread ("" if none). */
+const char* dfa_last_error(void);
+
+/* Library version string. */
+const char* dfa_version(void);
+
+/*
+ * Create an assembler for one (mesh, function space) pair and build the CSR structure once.
+ * Replaces AssemblerBase::init_global_tensor's first-call path: SparsityPatternBuilder::build
+ * (dolfin/fem/SparsityPatternBuilder.cpp:42-269) + EigenMatrix::init
+ * (dolfin/la/EigenMatrix.cpp:384-410).
+ *   gdim == tdim in {1,2,3}; h_cells [num_cells x (gdim+1)] vertex ids (Mesh::cells()).
+ *   element: DFA_ELEMENT_*; h_cell_dofs [num_cells x ndofs_cell] (DofMap::cell_dofs,
+ *     dolfin/fem/DofMap.h:200-206), num_dofs = DofMap::global_dimension().
+ *   h_coeff_dofs [num_cells x (gdim+1)]: dofmap of the P1 space holding the coefficients
+ *     (kappa, f, mu, lmbda); for P1 forms this is normally h_cell_dofs itself.
+ *   h_x [num_vertices x gdim]: coordinates used only to build a locality-preserving work
+ *     partition (no numerical use; later calls may pass different coordinates).
+ *   keep_diagonal: AssemblerBase::keep_diagonal (SparsityPatternBuilder.cpp:245-264).
+ * The assembler copies what it needs; the caller's arrays may be freed afterwards.
+ */
+int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t num_vertices,
+                         int64_t num_cells, const int32_t* h_cells, int32_t element,
+                         int32_t ndofs_cell, const int32_t* h_cell_dofs, int64_t num_dofs,
+                         const int32_t* h_coeff_dofs, int64_t num_coeff_dofs,
+                         const double* h_x, int32_t keep_diagonal, void* stream);
+
+int dfa_assembler_destroy(dfa_assembler h);
+
+/* Tensor layout: rows (= columns) and number of stored entries. GenericMatrix::size, nnz
+   (dolfin/la/GenericMatrix.h, EigenMatrix.cpp). */
+int dfa_csr_dims(dfa_assembler h, int64_t* num_rows, int64_t* nnz);
+
+/* Copy the device CSR structure (row_ptr [num_rows+1], col [nnz]) into caller buffers
+   (device pointers). Equivalent of EigenMatrix::data() (EigenMatrix.cpp:341-355). */
+int dfa_csr_structure(dfa_assembler h, int32_t* d_row_ptr, int32_t* d_col, void* stream);
+
+/*
+ * Bilinear form -> CSR values (device, [nnz]). Assembler::assemble for a rank-2 form:
+ * zero (unless DFA_ADD_VALUES) + assemble_cells (dolfin/fem/Assembler.cpp:105-194).
+ *   d_x [num_vertices x gdim]; d_c0, d_c1: P1 coefficient vectors (kappa; mu, lmbda).
+ */
+int dfa_assemble_matrix(dfa_assembler h, int32_t form, const double* d_x, const double* d_c0,
+                        const double* d_c1, double* d_values, int32_t scatter, int32_t flags,
+                        void* stream);
+
+/* Linear form -> vector (device, [num_dofs]). f3 (host, 3 doubles) for the body force. */
+int dfa_assemble_vector(dfa_assembler h, int32_t form, const double* d_x, const double* d_c0,
+                        const double* h_f3, double* d_b, int32_t scatter, int32_t flags,
+                        void* stream);
+
+/*
+ * SystemAssembler::assemble(A, b) for one square bilinear form a and linear form L, fused in
+ * one pass over the cells with the per-cell symmetric Dirichlet modification of
+ * SystemAssembler::apply_bc (dolfin/fem/SystemAssembler.cpp:303-492,1051-1075).
+ *   d_bc_mask [num_dofs] uint8 (nonzero = Dirichlet dof), d_bc_value [num_dofs] (may be NULL
+ *   when there are no boundary conditions). d_values or d_b may be NULL (assemble(A) or
+ *   assemble(b) only).
+ */
+int dfa_assemble_system(dfa_assembler h, int32_t a_form, int32_t L_form, const double* d_x,
+                        const double* d_a_c0, const double* d_a_c1, const double* d_L_c0,
+                        const double* h_f3, const uint8_t* d_bc_mask, const double* d_bc_value,
+                        double* d_values, double* d_b, int32_t scatter, int32_t flags,
+                        void* stream);
+
+/*
+ * Adjoint (reverse-mode) assembly of J = u^T K(c, x) u - F(x)^T u without forming K.
+ * Geometry is recomputed per cell. Outputs are overwritten (not accumulated).
+ *   P1 Poisson:  c0 = kappa, F = P1 load of d_f;  d_dJdc0 = dJ/dkappa [num_coeff_dofs],
+ *                d_dJdc1 unused (NULL).
+ *   vector P2 elasticity: c0 = mu, c1 = lmbda, F = body force h_f3;
+ *                d_dJdc0 = dJ/dmu, d_dJdc1 = dJ/dlmbda.
+ *   d_dJdx [num_vertices x gdim] = dJ/d(vertex coordinates); d_J (1 double) = J.
+ * Counterpart of assembling UFL derivative(J, kappa) / derivative(J, SpatialCoordinate)
+ * (python/test/unit/fem/test_coordinate_derivative.py:29-63).
+ */
+int dfa_adjoint(dfa_assembler h, int32_t a_form, const double* d_x, const double* d_c0,
+                const double* d_c1, const double* d_f, const double* h_f3, const double* d_u,
+                double* d_J, double* d_dJdc0, double* d_dJdc1, double* d_dJdx, void* stream);
+
+/*
+ * Batched assembly: many independent (mesh, space) assemblers, one bilinear form each,
+ * assembled by ONE persistent kernel launch (no counterpart in the reference, which
+ * assembles one form per call). forms[i], d_x[i], d_c0[i], d_c1[i], d_values[i] are per
+ * assembler; all assemblers must use DFA_SCATTER_ATOMIC-compatible elements.
+ * cells_per_item: cells each work item takes from the shared queue (0 = autotuned).
+ */
+int dfa_assemble_batch(int32_t n, const dfa_assembler* hs, const int32_t* forms,
+                       const double* const* d_x, const double* const* d_c0,
+                       const double* const* d_c1, double* const* d_values,
+                       int32_t cells_per_item, void* stream);
+
+/* Introspection for benchmarks: algorithmic bytes of one matrix assembly (the compulsory
+   traffic: connectivity, coordinates, coefficients, CSR values written once, col/row_ptr
+   read once) and the bytes the plan of the chosen strategy actually stores. */
+int dfa_plan_info(dfa_assembler h, int32_t scatter, int64_t* plan_bytes, int64_t* num_blocks,
+                  int64_t* cell_appearances, int32_t* lds_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DOLFIN_AMD_H */

@@ -1,0 +1,272 @@
+"""ctypes front-end of the C++ oracle (oracle.cpp) -- test infrastructure only.
+
+Every function here is a thin numpy wrapper; the arithmetic lives in oracle.cpp, which
+cites the DOLFIN 2019.1 source lines it restates.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "liboracle.so")
+
+P1_POISSON, P1_LOAD, P1_MASS, VP2_ELASTICITY, VP2_BODY_FORCE, VP1_ELASTICITY, VP1_BODY_FORCE = range(7)
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        _lib = ctypes.CDLL(_SO)
+        _declare(_lib)
+    return _lib
+
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_vp = ctypes.c_void_p
+
+
+def _declare(L):
+    L.orc_interval_mesh.argtypes = [_i32, _f64p, _i32p]
+    L.orc_rectangle_mesh_right.argtypes = [_i32, _i32, _f64p, _i32p]
+    L.orc_box_mesh.argtypes = [_i32, _i32, _i32, _f64p, _i32p]
+    L.orc_tet_edges.argtypes = [_i64, _i32p, _i32p]
+    L.orc_tet_edges.restype = _i64
+    L.orc_vp2_dofmap.argtypes = [_i64, _i32p, _i32p, _i64, _i64, _i32p]
+    L.orc_sparsity_build.argtypes = [_i64, _i32, _i32p, _i64, _i32]
+    L.orc_sparsity_build.restype = _vp
+    L.orc_sparsity_nnz.argtypes = [_vp]
+    L.orc_sparsity_nnz.restype = _i64
+    L.orc_sparsity_copy.argtypes = [_vp, _i32p, _i32p]
+    L.orc_sparsity_free.argtypes = [_vp]
+    for name in ("orc_p1_stiffness",):
+        getattr(L, name).argtypes = [_i32, _f64p, _f64p, _f64p]
+    L.orc_p1_mass.argtypes = [_i32, _f64p, _f64p]
+    L.orc_p1_load.argtypes = [_i32, _f64p, _f64p, _f64p]
+    L.orc_vp2_elasticity.argtypes = [_f64p, _f64p, _f64p, _f64p]
+    L.orc_vp1_elasticity.argtypes = [_f64p, _f64p, _f64p, _f64p]
+    L.orc_vp2_body_force.argtypes = [_f64p, _f64p, _f64p]
+    L.orc_vp1_body_force.argtypes = [_f64p, _f64p, _f64p]
+    L.orc_assemble_matrix.argtypes = [_i32, _i32, _i64, _i32p, _f64p, _i32, _i32p, _i32p,
+                                      _vp, _vp, _i32p, _i32p, _f64p]
+    L.orc_assemble_matrix.restype = _i32
+    L.orc_assemble_vector.argtypes = [_i32, _i32, _i64, _i32p, _f64p, _i32, _i32p, _i32p,
+                                      _vp, _vp, _f64p]
+    L.orc_assemble_vector.restype = _i32
+    L.orc_assemble_system.argtypes = [_i32, _i32, _i32, _i64, _i32p, _f64p, _i32, _i32p, _i32p,
+                                      _vp, _vp, _vp, _vp, _u8p, _f64p, _i32p, _i32p, _vp, _vp]
+    L.orc_assemble_system.restype = _i32
+    L.orc_adjoint_p1.argtypes = [_i32, _i64, _i32p, _f64p, _i64, _i32p, _i64, _f64p, _f64p,
+                                 _f64p, _f64p, _f64p]
+    L.orc_adjoint_p1.restype = ctypes.c_double
+    L.orc_adjoint_vp2.argtypes = [_i64, _i32p, _f64p, _i64, _i32p, _i32p, _i64, _f64p, _f64p,
+                                  _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_adjoint_vp2.restype = ctypes.c_double
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---- meshes -------------------------------------------------------------------------
+def interval_mesh(n):
+    x = np.empty(n + 1)
+    cells = np.empty((n, 2), np.int32)
+    lib().orc_interval_mesh(n, x, cells)
+    return x.reshape(-1, 1), cells
+
+
+def rectangle_mesh_right(nx, ny):
+    x = np.empty(((nx + 1) * (ny + 1), 2))
+    cells = np.empty((2 * nx * ny, 3), np.int32)
+    lib().orc_rectangle_mesh_right(nx, ny, x, cells)
+    return x, cells
+
+
+def box_mesh(nx, ny, nz):
+    x = np.empty(((nx + 1) * (ny + 1) * (nz + 1), 3))
+    cells = np.empty((6 * nx * ny * nz, 4), np.int32)
+    lib().orc_box_mesh(nx, ny, nz, x, cells)
+    return x, cells
+
+
+def tet_edges(cells):
+    cells = _c(cells, np.int32)
+    ce = np.empty((cells.shape[0], 6), np.int32)
+    ne = lib().orc_tet_edges(cells.shape[0], cells, ce)
+    return ce, int(ne)
+
+
+def vp2_dofmap(cells, cell_edges, nv, ne):
+    cells = _c(cells, np.int32)
+    cd = np.empty((cells.shape[0], 30), np.int32)
+    lib().orc_vp2_dofmap(cells.shape[0], cells, _c(cell_edges, np.int32), nv, ne, cd)
+    return cd
+
+
+def sparsity(cell_dofs, num_rows, keep_diagonal=False):
+    cd = _c(cell_dofs, np.int32)
+    h = lib().orc_sparsity_build(cd.shape[0], cd.shape[1], cd, num_rows, int(keep_diagonal))
+    nnz = lib().orc_sparsity_nnz(h)
+    rp = np.empty(num_rows + 1, np.int32)
+    col = np.empty(nnz, np.int32)
+    lib().orc_sparsity_copy(h, rp, col)
+    lib().orc_sparsity_free(h)
+    return rp, col
+
+
+# ---- element tensors ---------------------------------------------------------------
+def p1_stiffness(xc, kappa):
+    xc = _c(xc, np.float64)
+    g = xc.shape[1]
+    A = np.empty((g + 1) ** 2)
+    lib().orc_p1_stiffness(g, xc.ravel(), _c(kappa, np.float64), A)
+    return A.reshape(g + 1, g + 1)
+
+
+def p1_mass(xc):
+    xc = _c(xc, np.float64)
+    g = xc.shape[1]
+    A = np.empty((g + 1) ** 2)
+    lib().orc_p1_mass(g, xc.ravel(), A)
+    return A.reshape(g + 1, g + 1)
+
+
+def p1_load(xc, f):
+    xc = _c(xc, np.float64)
+    g = xc.shape[1]
+    b = np.empty(g + 1)
+    lib().orc_p1_load(g, xc.ravel(), _c(f, np.float64), b)
+    return b
+
+
+def vp2_elasticity(xc, mu, lmbda):
+    A = np.empty(900)
+    lib().orc_vp2_elasticity(_c(xc, np.float64).ravel(), _c(mu, np.float64), _c(lmbda, np.float64), A)
+    return A.reshape(30, 30)
+
+
+def vp1_elasticity(xc, mu, lmbda):
+    A = np.empty(144)
+    lib().orc_vp1_elasticity(_c(xc, np.float64).ravel(), _c(mu, np.float64), _c(lmbda, np.float64), A)
+    return A.reshape(12, 12)
+
+
+def vp2_body_force(xc, f3):
+    b = np.empty(30)
+    lib().orc_vp2_body_force(_c(xc, np.float64).ravel(), _c(f3, np.float64), b)
+    return b
+
+
+def vp1_body_force(xc, f3):
+    b = np.empty(12)
+    lib().orc_vp1_body_force(_c(xc, np.float64).ravel(), _c(f3, np.float64), b)
+    return b
+
+
+# ---- assembly ---------------------------------------------------------------------
+def assemble_matrix(kind, x, cells, cell_dofs, row_ptr, col, coeff_dofs=None, c0=None, c1=None):
+    x = _c(x, np.float64)
+    cells = _c(cells, np.int32)
+    cd = _c(cell_dofs, np.int32)
+    kd = _c(cells if coeff_dofs is None else coeff_dofs, np.int32)
+    c0 = None if c0 is None else _c(c0, np.float64)
+    c1 = None if c1 is None else _c(c1, np.float64)
+    vals = np.zeros(col.shape[0])
+    rc = lib().orc_assemble_matrix(kind, x.shape[1], cells.shape[0], cells, x.ravel(), cd.shape[1],
+                                   cd, kd, _ptr(c0), _ptr(c1), _c(row_ptr, np.int32),
+                                   _c(col, np.int32), vals)
+    if rc != 0:
+        raise RuntimeError(f"oracle assemble_matrix failed ({rc})")
+    return vals
+
+
+def assemble_vector(kind, x, cells, cell_dofs, ndofs, coeff_dofs=None, c0=None, f3=None):
+    x = _c(x, np.float64)
+    cells = _c(cells, np.int32)
+    cd = _c(cell_dofs, np.int32)
+    kd = _c(cells if coeff_dofs is None else coeff_dofs, np.int32)
+    c0 = None if c0 is None else _c(c0, np.float64)
+    f3 = None if f3 is None else _c(f3, np.float64)
+    b = np.zeros(ndofs)
+    rc = lib().orc_assemble_vector(kind, x.shape[1], cells.shape[0], cells, x.ravel(), cd.shape[1],
+                                   cd, kd, _ptr(c0), _ptr(f3), b)
+    if rc != 0:
+        raise RuntimeError(f"oracle assemble_vector failed ({rc})")
+    return b
+
+
+def assemble_system(a_kind, L_kind, x, cells, cell_dofs, row_ptr, col, bc_mask, bc_value,
+                    coeff_dofs=None, a_c0=None, a_c1=None, L_c0=None, f3=None,
+                    want_A=True, want_b=True):
+    x = _c(x, np.float64)
+    cells = _c(cells, np.int32)
+    cd = _c(cell_dofs, np.int32)
+    kd = _c(cells if coeff_dofs is None else coeff_dofs, np.int32)
+    arrs = [None if a is None else _c(a, np.float64) for a in (a_c0, a_c1, L_c0, f3)]
+    ndofs = row_ptr.shape[0] - 1
+    vals = np.zeros(col.shape[0]) if want_A else None
+    b = np.zeros(ndofs) if want_b else None
+    rc = lib().orc_assemble_system(a_kind, L_kind, x.shape[1], cells.shape[0], cells, x.ravel(),
+                                   cd.shape[1], cd, kd, *[_ptr(a) for a in arrs],
+                                   _c(bc_mask, np.uint8), _c(bc_value, np.float64),
+                                   _c(row_ptr, np.int32), _c(col, np.int32), _ptr(vals), _ptr(b))
+    if rc != 0:
+        raise RuntimeError(f"oracle assemble_system failed ({rc})")
+    return vals, b
+
+
+def adjoint_p1(x, cells, cell_dofs, kappa, f, u):
+    x = _c(x, np.float64)
+    cells = _c(cells, np.int32)
+    cd = _c(cell_dofs, np.int32)
+    nd = kappa.shape[0]
+    dk = np.empty(nd)
+    dx = np.empty_like(x)
+    J = lib().orc_adjoint_p1(x.shape[1], cells.shape[0], cells, x.ravel(), x.shape[0], cd, nd,
+                             _c(kappa, np.float64), _c(f, np.float64), _c(u, np.float64), dk,
+                             dx.ravel())
+    return J, dk, dx
+
+
+def adjoint_vp2(x, cells, cell_dofs, coeff_dofs, mu, lmbda, f3, u):
+    x = _c(x, np.float64)
+    cells = _c(cells, np.int32)
+    nk = mu.shape[0]
+    dmu = np.empty(nk)
+    dla = np.empty(nk)
+    dx = np.empty_like(x)
+    J = lib().orc_adjoint_vp2(cells.shape[0], cells, x.ravel(), x.shape[0],
+                              _c(cell_dofs, np.int32), _c(coeff_dofs, np.int32), nk,
+                              _c(mu, np.float64), _c(lmbda, np.float64), _c(f3, np.float64),
+                              _c(u, np.float64), dmu, dla, dx.ravel())
+    return J, dmu, dla, dx
+
+
+def frobenius(values):
+    """EigenMatrix::norm("frobenius") (EigenMatrix.cpp:77-105): sqrt of sum of squares."""
+    return float(np.sqrt(np.sum(np.asarray(values) ** 2)))
+
+
+def boundary_vertices(x):
+    """Vertices on the boundary of the unit interval/square/cube (DirichletBC 'on_boundary'
+    for P1, topological method)."""
+    x = np.asarray(x)
+    return np.any((np.abs(x) < 1e-14) | (np.abs(x - 1.0) < 1e-14), axis=1)
