@@ -1,0 +1,519 @@
+// C ABI of the assembly engine (include/dolfin_amd.h): handle management, structure phase
+// and dispatch of the cell kernels. Counterpart of dolfin::Assembler / SystemAssembler
+// (dolfin/fem/Assembler.cpp:52-103, SystemAssembler.cpp:138-301) for one (mesh, space).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace dfa {
+
+namespace {
+thread_local std::string g_error;
+}
+void set_error(const std::string& msg) { g_error = msg; }
+const char* last_error() { return g_error.c_str(); }
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct DeviceArena {
+  std::vector<void*> ptrs;
+  int64_t bytes = 0;
+  template <class T>
+  T* upload(const std::vector<T>& v, hipStream_t s) {
+    return upload(v.data(), v.size(), s);
+  }
+  template <class T>
+  T* upload(const T* h, size_t n, hipStream_t s) {
+    void* d = nullptr;
+    const size_t nb = std::max<size_t>(n * sizeof(T), 16);
+    hip_check(hipMalloc(&d, nb), "hipMalloc");
+    ptrs.push_back(d);
+    bytes += static_cast<int64_t>(n * sizeof(T));
+    if (n) hip_check(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    return static_cast<T*>(d);
+  }
+  void release() {
+    for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+    bytes = 0;
+  }
+};
+
+}  // namespace dfa
+
+struct dfa_assembler_s {
+  int32_t gdim = 0, element = 0, nd = 0, nvc = 0;
+  int64_t nv = 0, nc = 0, ndofs = 0, nkdofs = 0, nnz = 0;
+  bool rows_covered = true;   // every CSR row belongs to a dof of some cell
+  int32_t* d_cells = nullptr;
+  int32_t* d_cdofs = nullptr;
+  int32_t* d_kdofs = nullptr;
+  int32_t* d_row_ptr = nullptr;
+  int32_t* d_col = nullptr;
+  dfa::DeviceArena base;
+  // blocked P1 plan
+  bool has_plan = false;
+  std::string plan_error;
+  dfa::P1BlockTopo btopo;
+  dfa::DeviceArena plan;
+  int64_t appearances = 0;
+  // vector P2: per cell, rank of node b among the neighbours of node a (100 bytes) and the
+  // CSR strides (node stride s1, component stride s2 per row)
+  uint8_t* d_vp2_rank = nullptr;
+  int32_t vp2_s1 = 0;
+  bool vp2_component_major = true;
+};
+
+namespace {
+
+using dfa::hip_check;
+
+int fail(const std::string& m, int code) {
+  dfa::set_error(m);
+  return code;
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    dfa::set_error("");
+    return f();
+  } catch (const dfa::HipError& e) {
+    return fail(e.what(), DFA_ERR_HIP);
+  } catch (const std::exception& e) {
+    return fail(e.what(), DFA_ERR_ARG);
+  }
+}
+
+int default_block_rows(int gdim) {
+  if (const char* e = std::getenv("DFA_BLOCK_ROWS")) {
+    const int v = std::atoi(e);
+    if (v >= 8 && v <= 4096) return v;
+  }
+  return gdim == 3 ? 256 : 512;
+}
+
+// Vector P2 rank table. Nodes are identified by the cell-local node index a (0..9) of the
+// component-0 dof; the CSR row of dof (i, a) lists, for each column component j, the
+// neighbour nodes in ascending order, either component-major (UFC numbering, stride s1 = 1,
+// component stride = deg) or node-major (block-interleaved numbering, s1 = 3, stride 1).
+void build_vp2_ranks(dfa_assembler_s* h, const int32_t* cd, const dfa::HostCSR& csr,
+                     std::vector<uint8_t>& rank) {
+  const int64_t nc = h->nc;
+  // Node id of a dof: dof of component 0 at the same cell-local node.
+  std::vector<int32_t> node0(h->ndofs, -1), comp(h->ndofs, -1);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int i = 0; i < 3; ++i)
+      for (int a = 0; a < 10; ++a) {
+        const int32_t d = cd[c * 30 + i * 10 + a];
+        const int32_t n0 = cd[c * 30 + a];
+        if ((node0[d] >= 0 && node0[d] != n0) || (comp[d] >= 0 && comp[d] != i))
+          throw std::runtime_error("vector P2 dofmap is not node-blocked");
+        node0[d] = n0;
+        comp[d] = i;
+      }
+  // Determine the layout from the first row with more than one neighbour.
+  int layout = 0;  // 1 = component-major, 2 = node-major
+  for (int64_t r = 0; r < h->ndofs && !layout; ++r) {
+    const int32_t b = csr.row_ptr[r], e = csr.row_ptr[r + 1];
+    if (e - b < 6) continue;
+    layout = (comp[csr.col[b]] == comp[csr.col[b + 1]]) ? 1 : 2;
+  }
+  if (!layout) layout = 1;
+  h->vp2_component_major = (layout == 1);
+  rank.assign(static_cast<size_t>(nc) * 100, 0);
+  bool bad = false;
+  for (int64_t c = 0; c < nc && !bad; ++c)
+    for (int a = 0; a < 10; ++a) {
+      const int32_t ra = cd[c * 30 + a];  // component-0 row of node a
+      const int32_t rb0 = csr.row_ptr[ra], re0 = csr.row_ptr[ra + 1];
+      const int32_t len = re0 - rb0;
+      if (len % 3) { bad = true; break; }
+      const int32_t deg = len / 3;
+      for (int b = 0; b < 10; ++b) {
+        // locate column (component 0, node b) in row ra
+        const int32_t colb = cd[c * 30 + b];
+        const int32_t* p = std::lower_bound(csr.col.data() + rb0, csr.col.data() + re0, colb);
+        if (p == csr.col.data() + re0 || *p != colb) { bad = true; break; }
+        const int64_t pos = p - (csr.col.data() + rb0);
+        const int64_t rk = (layout == 1) ? pos : pos / 3;
+        if (rk > 255 || rk >= deg) { bad = true; break; }
+        rank[c * 100 + a * 10 + b] = static_cast<uint8_t>(rk);
+        // verify all 9 component combinations land on the expected columns
+        for (int i = 0; i < 3 && !bad; ++i) {
+          const int32_t row = cd[c * 30 + i * 10 + a];
+          if (csr.row_ptr[row + 1] - csr.row_ptr[row] != len) { bad = true; break; }
+          for (int j = 0; j < 3; ++j) {
+            const int64_t slot = csr.row_ptr[row] + ((layout == 1) ? (rk + j * deg) : (rk * 3 + j));
+            if (csr.col[slot] != cd[c * 30 + j * 10 + b]) { bad = true; break; }
+          }
+        }
+      }
+    }
+  if (bad) throw std::runtime_error("vector P2 sparsity is not node-blocked (unsupported dofmap)");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dfa_last_error(void) { return dfa::last_error(); }
+const char* dfa_version(void) { return "dolfin_amd 0.1 (gfx950)"; }
+
+int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t nc,
+                         const int32_t* h_cells, int32_t element, int32_t nd,
+                         const int32_t* h_cell_dofs, int64_t ndofs, const int32_t* h_coeff_dofs,
+                         int64_t nkdofs, const double* h_x, int32_t keep_diagonal, void* stream) {
+  return guarded([&]() -> int {
+    if (!out) return fail("out is NULL", DFA_ERR_ARG);
+    *out = nullptr;
+    if (gdim < 1 || gdim > 3) return fail("gdim must be 1, 2 or 3", DFA_ERR_ARG);
+    if (nv <= 0 || nc < 0 || ndofs <= 0 || !h_cells || !h_cell_dofs || !h_coeff_dofs)
+      return fail("invalid mesh/dofmap arguments", DFA_ERR_ARG);
+    if (element == DFA_ELEMENT_P1) {
+      if (nd != gdim + 1) return fail("P1 needs gdim+1 dofs per cell", DFA_ERR_ARG);
+    } else if (element == DFA_ELEMENT_VECTOR_P2) {
+      if (gdim != 3 || nd != 30) return fail("vector P2 needs tetrahedra and 30 dofs per cell", DFA_ERR_ARG);
+    } else {
+      return fail("unknown element", DFA_ERR_ARG);
+    }
+    if (ndofs > INT32_MAX || nv > INT32_MAX || nc > INT32_MAX)
+      return fail("sizes exceed int32 (la_index)", DFA_ERR_ARG);
+    const int nvc = gdim + 1;
+    for (int64_t i = 0; i < nc * nvc; ++i)
+      if (h_cells[i] < 0 || h_cells[i] >= nv || h_coeff_dofs[i] < 0 || h_coeff_dofs[i] >= nkdofs)
+        return fail("cell vertex or coefficient dof out of range", DFA_ERR_ARG);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    auto* h = new dfa_assembler_s();
+    try {
+      h->gdim = gdim;
+      h->element = element;
+      h->nd = nd;
+      h->nvc = nvc;
+      h->nv = nv;
+      h->nc = nc;
+      h->ndofs = ndofs;
+      h->nkdofs = nkdofs;
+      dfa::HostCSR csr;
+      dfa::build_csr(nc, nd, h_cell_dofs, ndofs, keep_diagonal != 0, csr);
+      h->nnz = static_cast<int64_t>(csr.col.size());
+      {
+        std::vector<uint8_t> seen(ndofs, 0);
+        for (int64_t i = 0; i < nc * nd; ++i) seen[h_cell_dofs[i]] = 1;
+        for (int64_t r = 0; r < ndofs; ++r)
+          if (!seen[r]) h->rows_covered = false;
+      }
+      h->d_cells = h->base.upload(h_cells, static_cast<size_t>(nc * nvc), s);
+      h->d_cdofs = h->base.upload(h_cell_dofs, static_cast<size_t>(nc * nd), s);
+      h->d_kdofs = h->base.upload(h_coeff_dofs, static_cast<size_t>(nc * nvc), s);
+      h->d_row_ptr = h->base.upload(csr.row_ptr, s);
+      h->d_col = h->base.upload(csr.col, s);
+      if (element == DFA_ELEMENT_P1) {
+        if (h_x) {
+          try {
+            dfa::P1BlockPlan P;
+            dfa::build_p1_block_plan(gdim, nv, nc, h_cells, h_cell_dofs, h_coeff_dofs, h_x, csr,
+                                     default_block_rows(gdim), P);
+            auto& t = h->btopo;
+            t.num_blocks = P.num_blocks;
+            t.max_nodes = P.max_nodes;
+            t.max_rows = P.max_rows;
+            t.max_nnz = P.max_nnz;
+            t.kdof_is_vertex = P.kdof_is_vertex;
+            t.node_ptr = h->plan.upload(P.node_ptr, s);
+            t.node_vertex = h->plan.upload(P.node_vertex, s);
+            t.node_kdof = h->plan.upload(P.node_kdof, s);
+            t.node_dof = h->plan.upload(P.node_dof, s);
+            t.cell_ptr = h->plan.upload(P.cell_ptr, s);
+            t.cell_ids = h->plan.upload(P.cell_ids, s);
+            t.cell_rank = h->plan.upload(P.cell_rank, s);
+            t.row_ptr_b = h->plan.upload(P.row_ptr, s);
+            t.row_dof = h->plan.upload(P.row_dof, s);
+            t.row_info = h->plan.upload(P.row_info, s);
+            t.blk_nnz = h->plan.upload(P.blk_nnz, s);
+            t.csr_row_ptr = h->d_row_ptr;
+            h->appearances = static_cast<int64_t>(P.cell_ids.size());
+            h->has_plan = true;
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+          } catch (const dfa::HipError&) {
+            throw;
+          } catch (const std::exception& e) {
+            h->plan.release();
+            h->plan_error = e.what();
+          }
+        } else {
+          h->plan_error = "no coordinates given at creation";
+        }
+      } else {
+        std::vector<uint8_t> rank;
+        build_vp2_ranks(h, h_cell_dofs, csr, rank);
+        h->d_vp2_rank = h->base.upload(rank, s);
+      }
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    } catch (...) {
+      h->plan.release();
+      h->base.release();
+      delete h;
+      throw;
+    }
+    *out = h;
+    return DFA_OK;
+  });
+}
+
+int dfa_assembler_destroy(dfa_assembler h) {
+  if (!h) return DFA_OK;
+  h->plan.release();
+  h->base.release();
+  delete h;
+  return DFA_OK;
+}
+
+int dfa_csr_dims(dfa_assembler h, int64_t* num_rows, int64_t* nnz) {
+  if (!h) return fail("NULL handle", DFA_ERR_ARG);
+  if (num_rows) *num_rows = h->ndofs;
+  if (nnz) *nnz = h->nnz;
+  return DFA_OK;
+}
+
+int dfa_csr_structure(dfa_assembler h, int32_t* d_row_ptr, int32_t* d_col, void* stream) {
+  return guarded([&]() -> int {
+    if (!h) return fail("NULL handle", DFA_ERR_ARG);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (d_row_ptr)
+      hip_check(hipMemcpyAsync(d_row_ptr, h->d_row_ptr, (h->ndofs + 1) * sizeof(int32_t),
+                               hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+    if (d_col && h->nnz)
+      hip_check(hipMemcpyAsync(d_col, h->d_col, h->nnz * sizeof(int32_t), hipMemcpyDeviceToDevice, s),
+                "hipMemcpyAsync");
+    return DFA_OK;
+  });
+}
+
+static int resolve_scatter(dfa_assembler h, int32_t scatter) {
+  if (h->element == DFA_ELEMENT_P1) {
+    if (scatter == DFA_SCATTER_AUTO) return h->has_plan ? DFA_SCATTER_BLOCKED : DFA_SCATTER_ATOMIC;
+    if (scatter == DFA_SCATTER_BLOCKED && !h->has_plan)
+      throw std::runtime_error("blocked scatter unavailable for this mesh: " + h->plan_error);
+    if (scatter != DFA_SCATTER_ATOMIC && scatter != DFA_SCATTER_BLOCKED)
+      throw std::runtime_error("unknown scatter strategy");
+    return scatter;
+  }
+  if (scatter == DFA_SCATTER_AUTO || scatter == DFA_SCATTER_ATOMIC) return DFA_SCATTER_ATOMIC;
+  throw std::runtime_error("vector P2 supports the atomic scatter only");
+}
+
+// One fused pass over the cells (Assembler::assemble_cells / SystemAssembler::cell_wise_assembly).
+static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const double* x,
+                    const double* a_c0, const double* a_c1, const double* L_c0, const double* f3,
+                    const uint8_t* bcm, const double* bcv, double* values, double* b,
+                    int32_t scatter, int32_t flags, hipStream_t s) {
+  if (!x) throw std::runtime_error("coordinates are NULL");
+  const bool add = (flags & DFA_ADD_VALUES) != 0;
+  const bool bc = bcm != nullptr;
+  if (bc && !bcv) throw std::runtime_error("bc_mask given without bc_value");
+  if (h->element == DFA_ELEMENT_P1) {
+    if (a_form >= 0 && a_form != DFA_FORM_P1_POISSON && a_form != DFA_FORM_P1_MASS)
+      throw std::runtime_error("form does not match the P1 element");
+    if (L_form >= 0 && L_form != DFA_FORM_P1_LOAD)
+      throw std::runtime_error("linear form does not match the P1 element");
+    if (a_form == DFA_FORM_P1_POISSON && !a_c0) throw std::runtime_error("kappa is NULL");
+    if (L_form == DFA_FORM_P1_LOAD && !L_c0) throw std::runtime_error("f is NULL");
+    if (bc && a_form < 0) throw std::runtime_error("boundary conditions need the bilinear form");
+    dfa::P1Pass p;
+    p.gdim = h->gdim;
+    p.a_form = a_form;
+    p.L_form = L_form;
+    p.want_A = values != nullptr;
+    p.want_b = b != nullptr;
+    p.bc = bc;
+    p.add = add;
+    p.x = x;
+    p.kappa = a_c0;
+    p.f = L_c0;
+    p.bc_mask = bcm;
+    p.bc_value = bcv;
+    p.values = values;
+    p.b = b;
+    const int sc = resolve_scatter(h, scatter);
+    if (sc == DFA_SCATTER_ATOMIC) {
+      if (!add && values) hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
+      if (!add && b) hip_check(dfa::launch_fill_zero(b, h->ndofs, s), "fill_zero");
+      dfa::P1AtomicTopo t;
+      t.num_cells = h->nc;
+      t.cells = h->d_cells;
+      t.cell_dofs = h->d_cdofs;
+      t.coeff_dofs = h->d_kdofs;
+      t.row_ptr = h->d_row_ptr;
+      t.col = h->d_col;
+      hip_check(dfa::launch_p1_atomic(t, p, s), "p1_atomic_kernel");
+    } else {
+      if (!add && !h->rows_covered) {
+        if (values) hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
+        if (b) hip_check(dfa::launch_fill_zero(b, h->ndofs, s), "fill_zero");
+      }
+      const size_t lds = dfa::p1_blocked_lds_bytes(h->btopo, p);
+      if (lds > 160 * 1024) throw std::runtime_error("blocked plan needs more LDS than a CU has");
+      hip_check(dfa::launch_p1_blocked(h->btopo, p, s), "p1_blocked_kernel");
+    }
+    return DFA_OK;
+  }
+  // vector P2
+  if (a_form >= 0 && a_form != DFA_FORM_VP2_ELASTICITY)
+    throw std::runtime_error("form does not match the vector P2 element");
+  if (L_form >= 0 && L_form != DFA_FORM_VP2_BODY_FORCE)
+    throw std::runtime_error("linear form does not match the vector P2 element");
+  if (a_form >= 0 && (!a_c0 || !a_c1)) throw std::runtime_error("mu/lmbda are NULL");
+  if (bc && a_form < 0) throw std::runtime_error("boundary conditions need the bilinear form");
+  resolve_scatter(h, scatter);
+  dfa::VP2Pass p;
+  p.want_A = values != nullptr && a_form >= 0;
+  p.want_b = b != nullptr && L_form >= 0;
+  p.bc = bc;
+  p.x = x;
+  p.mu = a_c0;
+  p.lmbda = a_c1;
+  if (f3) for (int i = 0; i < 3; ++i) p.f3[i] = f3[i];
+  p.bc_mask = bcm;
+  p.bc_value = bcv;
+  p.values = values;
+  p.b = b;
+  if (!add && values) hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
+  if (!add && b) hip_check(dfa::launch_fill_zero(b, h->ndofs, s), "fill_zero");
+  dfa::VP2Topo t;
+  t.num_cells = h->nc;
+  t.cells = h->d_cells;
+  t.cell_dofs = h->d_cdofs;
+  t.coeff_dofs = h->d_kdofs;
+  t.row_ptr = h->d_row_ptr;
+  t.rank = h->d_vp2_rank;
+  t.component_major = h->vp2_component_major;
+  hip_check(dfa::launch_vp2_atomic(t, p, s), "vp2_kernel");
+  return DFA_OK;
+}
+
+int dfa_assemble_matrix(dfa_assembler h, int32_t form, const double* d_x, const double* d_c0,
+                        const double* d_c1, double* d_values, int32_t scatter, int32_t flags,
+                        void* stream) {
+  return guarded([&]() -> int {
+    if (!h || !d_values) return fail("NULL handle or values", DFA_ERR_ARG);
+    return run_pass(h, form, -1, d_x, d_c0, d_c1, nullptr, nullptr, nullptr, nullptr, d_values,
+                    nullptr, scatter, flags, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfa_assemble_vector(dfa_assembler h, int32_t form, const double* d_x, const double* d_c0,
+                        const double* h_f3, double* d_b, int32_t scatter, int32_t flags,
+                        void* stream) {
+  return guarded([&]() -> int {
+    if (!h || !d_b) return fail("NULL handle or vector", DFA_ERR_ARG);
+    if (form == DFA_FORM_VP2_BODY_FORCE && !h_f3) return fail("f3 is NULL", DFA_ERR_ARG);
+    return run_pass(h, -1, form, d_x, nullptr, nullptr, d_c0, h_f3, nullptr, nullptr, nullptr,
+                    d_b, scatter, flags, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfa_assemble_system(dfa_assembler h, int32_t a_form, int32_t L_form, const double* d_x,
+                        const double* d_a_c0, const double* d_a_c1, const double* d_L_c0,
+                        const double* h_f3, const uint8_t* d_bc_mask, const double* d_bc_value,
+                        double* d_values, double* d_b, int32_t scatter, int32_t flags,
+                        void* stream) {
+  return guarded([&]() -> int {
+    if (!h) return fail("NULL handle", DFA_ERR_ARG);
+    if (!d_values && !d_b) return fail("nothing to assemble (A and b are NULL)", DFA_ERR_ARG);
+    if (a_form < 0 || L_form < 0) return fail("assemble_system needs a bilinear and a linear form", DFA_ERR_ARG);
+    if (L_form == DFA_FORM_VP2_BODY_FORCE && !h_f3) return fail("f3 is NULL", DFA_ERR_ARG);
+    return run_pass(h, a_form, L_form, d_x, d_a_c0, d_a_c1, d_L_c0, h_f3, d_bc_mask, d_bc_value,
+                    d_values, d_b, scatter, flags, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfa_adjoint(dfa_assembler h, int32_t a_form, const double* d_x, const double* d_c0,
+                const double* d_c1, const double* d_f, const double* h_f3, const double* d_u,
+                double* d_J, double* d_dJdc0, double* d_dJdc1, double* d_dJdx, void* stream) {
+  return guarded([&]() -> int {
+    if (!h || !d_x || !d_u || !d_J || !d_dJdc0 || !d_dJdx) return fail("NULL argument", DFA_ERR_ARG);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hip_check(dfa::launch_fill_zero(d_J, 1, s), "fill_zero");
+    hip_check(dfa::launch_fill_zero(d_dJdc0, h->nkdofs, s), "fill_zero");
+    hip_check(dfa::launch_fill_zero(d_dJdx, h->nv * h->gdim, s), "fill_zero");
+    if (h->element == DFA_ELEMENT_P1) {
+      if (a_form != DFA_FORM_P1_POISSON) return fail("P1 adjoint supports the kappa-Poisson form", DFA_ERR_ARG);
+      if (!d_c0 || !d_f) return fail("kappa or f is NULL", DFA_ERR_ARG);
+      dfa::AdjointP1 a;
+      a.gdim = h->gdim;
+      a.num_cells = h->nc;
+      a.cells = h->d_cells;
+      a.cell_dofs = h->d_cdofs;
+      a.coeff_dofs = h->d_kdofs;
+      a.x = d_x;
+      a.kappa = d_c0;
+      a.f = d_f;
+      a.u = d_u;
+      a.J = d_J;
+      a.dJdk = d_dJdc0;
+      a.dJdx = d_dJdx;
+      hip_check(dfa::launch_adjoint_p1(a, s), "adjoint_p1_kernel");
+      return DFA_OK;
+    }
+    if (a_form != DFA_FORM_VP2_ELASTICITY) return fail("vector P2 adjoint supports elasticity", DFA_ERR_ARG);
+    if (!d_c0 || !d_c1 || !d_dJdc1 || !h_f3) return fail("mu, lmbda, dJ/dlmbda or f3 is NULL", DFA_ERR_ARG);
+    hip_check(dfa::launch_fill_zero(d_dJdc1, h->nkdofs, s), "fill_zero");
+    dfa::AdjointVP2 a;
+    a.num_cells = h->nc;
+    a.cells = h->d_cells;
+    a.cell_dofs = h->d_cdofs;
+    a.coeff_dofs = h->d_kdofs;
+    a.x = d_x;
+    a.mu = d_c0;
+    a.lmbda = d_c1;
+    for (int i = 0; i < 3; ++i) a.f3[i] = h_f3[i];
+    a.u = d_u;
+    a.J = d_J;
+    a.dJdmu = d_dJdc0;
+    a.dJdlmbda = d_dJdc1;
+    a.dJdx = d_dJdx;
+    hip_check(dfa::launch_adjoint_vp2(a, s), "adjoint_vp2_kernel");
+    return DFA_OK;
+  });
+}
+
+int dfa_plan_info(dfa_assembler h, int32_t scatter, int64_t* plan_bytes, int64_t* num_blocks,
+                  int64_t* cell_appearances, int32_t* lds_bytes) {
+  return guarded([&]() -> int {
+    if (!h) return fail("NULL handle", DFA_ERR_ARG);
+    const int sc = resolve_scatter(h, scatter);
+    if (sc == DFA_SCATTER_BLOCKED) {
+      dfa::P1Pass p;
+      p.gdim = h->gdim;
+      p.a_form = DFA_FORM_P1_POISSON;
+      p.want_b = true;
+      if (plan_bytes) *plan_bytes = h->plan.bytes;
+      if (num_blocks) *num_blocks = h->btopo.num_blocks;
+      if (cell_appearances) *cell_appearances = h->appearances;
+      if (lds_bytes) *lds_bytes = static_cast<int32_t>(dfa::p1_blocked_lds_bytes(h->btopo, p));
+    } else {
+      if (plan_bytes) *plan_bytes = h->element == DFA_ELEMENT_VECTOR_P2 ? h->nc * 100 : 0;
+      if (num_blocks) *num_blocks = 0;
+      if (cell_appearances) *cell_appearances = h->nc;
+      if (lds_bytes) *lds_bytes = 0;
+    }
+    return DFA_OK;
+  });
+}
+
+}  // extern "C"

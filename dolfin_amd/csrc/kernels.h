@@ -1,0 +1,117 @@
+// Launch interfaces of the HIP kernels (host-callable, defined in *.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dfa {
+
+// Which element tensors one pass computes: the bilinear form (A), the linear form (b) and
+// whether Dirichlet rows/columns are eliminated per cell (SystemAssembler::apply_bc).
+struct P1Pass {
+  int gdim = 3;
+  int a_form = -1;            // DFA_FORM_P1_POISSON / DFA_FORM_P1_MASS / -1 (no matrix)
+  int L_form = -1;            // DFA_FORM_P1_LOAD / -1 (no vector)
+  bool want_A = false, want_b = false, bc = false, add = false;
+  const double* x = nullptr;
+  const double* kappa = nullptr;   // a_form coefficient (P1 nodal)
+  const double* f = nullptr;       // L_form coefficient (P1 nodal)
+  const uint8_t* bc_mask = nullptr;
+  const double* bc_value = nullptr;
+  double* values = nullptr;
+  double* b = nullptr;
+};
+
+struct P1AtomicTopo {
+  int64_t num_cells = 0;
+  const int32_t* cells = nullptr;      // [nc x nvc] vertex ids
+  const int32_t* cell_dofs = nullptr;  // [nc x nvc]
+  const int32_t* coeff_dofs = nullptr; // [nc x nvc]
+  const int32_t* row_ptr = nullptr;
+  const int32_t* col = nullptr;
+};
+
+struct P1BlockTopo {
+  int32_t num_blocks = 0;
+  int32_t max_nodes = 0, max_rows = 0, max_nnz = 0;
+  bool kdof_is_vertex = false;
+  const int32_t* node_ptr = nullptr;
+  const int32_t* node_vertex = nullptr;
+  const int32_t* node_kdof = nullptr;
+  const int32_t* node_dof = nullptr;
+  const int32_t* cell_ptr = nullptr;
+  const uint64_t* cell_ids = nullptr;
+  const uint64_t* cell_rank = nullptr;
+  const int32_t* row_ptr_b = nullptr;
+  const int32_t* row_dof = nullptr;
+  const uint32_t* row_info = nullptr;
+  const int32_t* blk_nnz = nullptr;
+  const int32_t* csr_row_ptr = nullptr;
+};
+
+hipError_t launch_p1_atomic(const P1AtomicTopo& t, const P1Pass& p, hipStream_t s);
+hipError_t launch_p1_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t s);
+size_t p1_blocked_lds_bytes(const P1BlockTopo& t, const P1Pass& p);
+hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);
+
+// ---- vector P2 elasticity --------------------------------------------------------------
+struct VP2Topo {
+  int64_t num_cells = 0;
+  const int32_t* cells = nullptr;      // [nc x 4]
+  const int32_t* cell_dofs = nullptr;  // [nc x 30] component-major local order
+  const int32_t* coeff_dofs = nullptr; // [nc x 4] P1 dofs of mu, lmbda
+  const int32_t* row_ptr = nullptr;
+  const uint8_t* rank = nullptr;       // [nc x 10 x 10] rank of node b among node a's neighbours
+  bool component_major = true;         // CSR column order of a row: component-major or node-major
+};
+
+struct VP2Pass {
+  bool want_A = false, want_b = false, bc = false;
+  const double* x = nullptr;
+  const double* mu = nullptr;
+  const double* lmbda = nullptr;
+  double f3[3] = {0, 0, 0};
+  const uint8_t* bc_mask = nullptr;
+  const double* bc_value = nullptr;
+  double* values = nullptr;
+  double* b = nullptr;
+};
+
+hipError_t launch_vp2_atomic(const VP2Topo& t, const VP2Pass& p, hipStream_t s);
+
+// ---- adjoint -------------------------------------------------------------------------
+struct AdjointP1 {
+  int gdim = 3;
+  int64_t num_cells = 0;
+  const int32_t* cells = nullptr;
+  const int32_t* cell_dofs = nullptr;
+  const int32_t* coeff_dofs = nullptr;
+  const double* x = nullptr;
+  const double* kappa = nullptr;
+  const double* f = nullptr;
+  const double* u = nullptr;
+  double* J = nullptr;
+  double* dJdk = nullptr;
+  double* dJdx = nullptr;
+};
+hipError_t launch_adjoint_p1(const AdjointP1& a, hipStream_t s);
+
+struct AdjointVP2 {
+  int64_t num_cells = 0;
+  const int32_t* cells = nullptr;
+  const int32_t* cell_dofs = nullptr;
+  const int32_t* coeff_dofs = nullptr;
+  const double* x = nullptr;
+  const double* mu = nullptr;
+  const double* lmbda = nullptr;
+  double f3[3] = {0, 0, 0};
+  const double* u = nullptr;
+  double* J = nullptr;
+  double* dJdmu = nullptr;
+  double* dJdlmbda = nullptr;
+  double* dJdx = nullptr;
+};
+hipError_t launch_adjoint_vp2(const AdjointVP2& a, hipStream_t s);
+
+}  // namespace dfa

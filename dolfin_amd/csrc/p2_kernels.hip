@@ -1,0 +1,277 @@
+// Vector P2 elasticity on tetrahedra for gfx950.
+//
+// Form (demo/undocumented/elasticity/cpp/Elasticity.ufl with P1 mu, lmbda):
+//   a(u, v) = inner(2 mu eps(u) + lmbda tr(eps(u)) I, eps(v))*dx,   L(v) = inner(f, v)*dx
+// Local dof order is component-major, (i, a) -> i*10 + a, nodes a = 4 vertices then the 6
+// UFC edges (2,3),(1,3),(1,2),(0,3),(0,2),(0,1) (python/test/unit/fem/test_dofmap.py:112-120).
+//
+// Element matrix K[(i,a),(j,b)] = delta_ij tr S^mu_ab + S^mu_ab[j][i] + S^lmbda_ab[i][j] with
+// S^c_ab[p][q] = int c d_p phi_a d_q phi_b. The P2 basis gradients are linear in the
+// barycentric coordinates and factor into at most two outer products,
+//   grad phi_vertex_k = (4 lambda_k - 1) grad lambda_k,
+//   grad phi_edge(k,l) = 4 lambda_l grad lambda_k + 4 lambda_k grad lambda_l,
+// so with W^c_mr = int c lambda_m lambda_r (exact: c is P1, int lambda^alpha closed form)
+//   S^c_ab = sum_{t,s} (u_t^T W^c u'_s) w_t (x) w'_s,
+// at most 4 scalar bilinear forms and 4 outer products of 3-vectors per node pair. This is
+// ~20 kFLOP per cell, so the kernel is bound by the 900-entry scatter, not by arithmetic
+// (a 4 x 900 MFMA contraction would not shorten it).
+//
+// Scatter: per cell the rank of node b among the column nodes of node a is precomputed
+// (100 bytes, built with the CSR); slot = row_ptr[dof(i,a)] + rank*s1 + j*s2 with
+// (s1, s2) = (1, deg_a) for UFC component-major numbering or (3, 1) for node-major.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace dfa {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kCells = 16;  // cells per workgroup iteration
+
+__constant__ int kEdge[6][2] = {{2, 3}, {1, 3}, {1, 2}, {0, 3}, {0, 2}, {0, 1}};
+
+// int lambda_n lambda_m lambda_r / |T| on a tetrahedron.
+__device__ __forceinline__ double bary3(int n, int m, int r) {
+  if (n == m && m == r) return 1.0 / 20.0;
+  if (n == m || m == r || n == r) return 1.0 / 60.0;
+  return 1.0 / 120.0;
+}
+
+struct CellData {
+  double D[4][3];      // grad lambda
+  double Wm[4][4], Wl[4][4];
+  double Rm[4], Rl[4]; // row sums of W
+  double Tm, Tl;       // total sums
+  double vol;
+  int32_t dof[30];
+  int32_t rbase[30];   // row_ptr[dof]
+  int32_t deg[10];     // neighbours per node (row length / 3)
+  uint8_t rank[100];
+  uint8_t bcf[30];
+  double bcv[30];
+  double be[30];
+};
+
+// Geometry and exact coefficient weights of one cell.
+__device__ void vp2_cell_setup(const double (&p)[4][3], const double (&mu)[4], const double (&la)[4],
+                               CellData& c) {
+  double e1[3], e2[3], e3[3];
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = p[1][i] - p[0][i];
+    e2[i] = p[2][i] - p[0][i];
+    e3[i] = p[3][i] - p[0][i];
+  }
+  double g[4][3];
+  g[1][0] = e2[1] * e3[2] - e2[2] * e3[1];
+  g[1][1] = e2[2] * e3[0] - e2[0] * e3[2];
+  g[1][2] = e2[0] * e3[1] - e2[1] * e3[0];
+  g[2][0] = e3[1] * e1[2] - e3[2] * e1[1];
+  g[2][1] = e3[2] * e1[0] - e3[0] * e1[2];
+  g[2][2] = e3[0] * e1[1] - e3[1] * e1[0];
+  g[3][0] = e1[1] * e2[2] - e1[2] * e2[1];
+  g[3][1] = e1[2] * e2[0] - e1[0] * e2[2];
+  g[3][2] = e1[0] * e2[1] - e1[1] * e2[0];
+  const double det = e1[0] * g[1][0] + e1[1] * g[1][1] + e1[2] * g[1][2];
+  const double inv = 1.0 / det;
+  for (int i = 0; i < 3; ++i) g[0][i] = -(g[1][i] + g[2][i] + g[3][i]);
+  for (int k = 0; k < 4; ++k)
+    for (int i = 0; i < 3; ++i) c.D[k][i] = g[k][i] * inv;
+  c.vol = fabs(det) / 6.0;
+  c.Tm = c.Tl = 0.0;
+  for (int m = 0; m < 4; ++m) {
+    c.Rm[m] = c.Rl[m] = 0.0;
+    for (int r = 0; r < 4; ++r) {
+      double wm = 0.0, wl = 0.0;
+      for (int n = 0; n < 4; ++n) {
+        const double w = bary3(n, m, r);
+        wm += mu[n] * w;
+        wl += la[n] * w;
+      }
+      c.Wm[m][r] = c.vol * wm;
+      c.Wl[m][r] = c.vol * wl;
+      c.Rm[m] += c.vol * wm;
+      c.Rl[m] += c.vol * wl;
+    }
+    c.Tm += c.Rm[m];
+    c.Tl += c.Rl[m];
+  }
+}
+
+// Factor of grad phi_a: terms (kind, index, gradient index). kind 0: u = 4 e_k - 1;
+// kind 1: u = 4 e_k.
+struct Factor {
+  int n;
+  int kind[2], idx[2], grad[2];
+};
+
+__device__ __forceinline__ Factor factor_of(int a) {
+  Factor f;
+  if (a < 4) {
+    f.n = 1;
+    f.kind[0] = 0;
+    f.idx[0] = a;
+    f.grad[0] = a;
+  } else {
+    const int k = kEdge[a - 4][0], l = kEdge[a - 4][1];
+    f.n = 2;
+    f.kind[0] = 1; f.idx[0] = l; f.grad[0] = k;
+    f.kind[1] = 1; f.idx[1] = k; f.grad[1] = l;
+  }
+  return f;
+}
+
+// u^T W u' for the two factor kinds.
+__device__ __forceinline__ double alpha(int k1, int i1, int k2, int i2, const double (&W)[4][4],
+                                        const double (&R)[4], double T) {
+  if (k1 == 0 && k2 == 0) return 16.0 * W[i1][i2] - 4.0 * R[i1] - 4.0 * R[i2] + T;
+  if (k1 == 0) return 4.0 * (4.0 * W[i1][i2] - R[i2]);
+  if (k2 == 0) return 4.0 * (4.0 * W[i1][i2] - R[i1]);
+  return 16.0 * W[i1][i2];
+}
+
+// 3x3 block K[(i,a),(j,b)] of the elasticity element matrix.
+__device__ __forceinline__ void vp2_block(const CellData& c, int a, int b, double (&K)[3][3]) {
+  const Factor fa = factor_of(a), fb = factor_of(b);
+  double Sm[3][3] = {}, Sl[3][3] = {};
+  for (int t = 0; t < fa.n; ++t)
+    for (int s = 0; s < fb.n; ++s) {
+      const double am = alpha(fa.kind[t], fa.idx[t], fb.kind[s], fb.idx[s], c.Wm, c.Rm, c.Tm);
+      const double al = alpha(fa.kind[t], fa.idx[t], fb.kind[s], fb.idx[s], c.Wl, c.Rl, c.Tl);
+      const double* wa = c.D[fa.grad[t]];
+      const double* wb = c.D[fb.grad[s]];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const double o = wa[p] * wb[q];
+          Sm[p][q] += am * o;
+          Sl[p][q] += al * o;
+        }
+    }
+  const double tr = Sm[0][0] + Sm[1][1] + Sm[2][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) K[i][j] = (i == j ? tr : 0.0) + Sm[j][i] + Sl[i][j];
+}
+
+template <bool WANT_A, bool WANT_B, bool BC>
+__global__ __launch_bounds__(kThreads) void vp2_atomic_kernel(VP2Topo t, VP2Pass p) {
+  __shared__ CellData sc[kCells];
+  const int tid = threadIdx.x;
+  const int64_t nchunks = (t.num_cells + kCells - 1) / kCells;
+  for (int64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const int64_t cbase = chunk * kCells;
+    const int ncell = static_cast<int>(min<int64_t>(kCells, t.num_cells - cbase));
+    // phase 1: per-cell tables (16 threads per cell)
+    {
+      const int lc = tid >> 4, ln = tid & 15;
+      if (lc < ncell) {
+        CellData& c = sc[lc];
+        const int64_t cell = cbase + lc;
+        for (int k = ln; k < 30; k += 16) {
+          const int32_t d = t.cell_dofs[cell * 30 + k];
+          c.dof[k] = d;
+          c.rbase[k] = t.row_ptr[d];
+          if (k < 10) c.deg[k] = (t.row_ptr[d + 1] - t.row_ptr[d]) / 3;
+          if constexpr (BC) {
+            const uint8_t m = p.bc_mask[d];
+            c.bcf[k] = m;
+            c.bcv[k] = m ? p.bc_value[d] : 0.0;
+          }
+        }
+        if constexpr (WANT_A)
+          for (int k = ln; k < 100; k += 16) c.rank[k] = t.rank[cell * 100 + k];
+        if (ln == 0) {
+          double pc[4][3], mu[4], la[4];
+          for (int v = 0; v < 4; ++v) {
+            const int64_t vid = t.cells[cell * 4 + v];
+            for (int i = 0; i < 3; ++i) pc[v][i] = p.x[vid * 3 + i];
+            const int32_t kd = t.coeff_dofs[cell * 4 + v];
+            mu[v] = (WANT_A || BC) ? p.mu[kd] : 0.0;
+            la[v] = (WANT_A || BC) ? p.lmbda[kd] : 0.0;
+          }
+          vp2_cell_setup(pc, mu, la, c);
+        }
+      }
+    }
+    __syncthreads();
+    if constexpr (WANT_B) {
+      for (int k = tid; k < ncell * 30; k += kThreads) {
+        CellData& c = sc[k / 30];
+        const int e = k % 30, i = e / 10, a = e % 10;
+        c.be[e] = p.f3[i] * (a < 4 ? -c.vol / 20.0 : c.vol / 5.0);
+      }
+      __syncthreads();
+    }
+    // phase 2: node pairs
+    if constexpr (WANT_A || BC) for (int task = tid; task < ncell * 100; task += kThreads) {
+      const int lc = task / 100, a = (task % 100) / 10, b = task % 10;
+      const CellData& c = sc[lc];
+      double K[3][3];
+      vp2_block(c, a, b, K);
+      if constexpr (BC) {
+        // lifting of the original entries, then elimination (SystemAssembler::apply_bc)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const int r = i * 10 + a, col = j * 10 + b;
+            if (WANT_B && c.bcf[col] && !c.bcf[r])
+              unsafeAtomicAdd(const_cast<double*>(&c.be[r]), -K[i][j] * c.bcv[col]);
+            if (c.bcf[r] || c.bcf[col]) K[i][j] = (r == col) ? 1.0 : 0.0;
+          }
+      }
+      if constexpr (WANT_A) {
+        const int rk = c.rank[a * 10 + b];
+        const int s1 = t.component_major ? 1 : 3;
+        const int s2 = t.component_major ? c.deg[a] : 1;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double* row = p.values + c.rbase[i * 10 + a] + rk * s1;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) unsafeAtomicAdd(row + j * s2, K[i][j]);
+        }
+      }
+    }
+    __syncthreads();
+    if constexpr (WANT_B) {
+      for (int k = tid; k < ncell * 30; k += kThreads) {
+        const CellData& c = sc[k / 30];
+        const int e = k % 30;
+        const double v = (BC && c.bcf[e]) ? c.bcv[e] : c.be[e];
+        unsafeAtomicAdd(p.b + c.dof[e], v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <bool A, bool B, bool BC>
+hipError_t launch_t(const VP2Topo& t, const VP2Pass& p, hipStream_t s) {
+  const int64_t nchunks = (t.num_cells + kCells - 1) / kCells;
+  const int64_t grid = std::min<int64_t>(nchunks, 256 * 16);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL((vp2_atomic_kernel<A, B, BC>), dim3(static_cast<unsigned>(grid)), dim3(kThreads),
+                     0, s, t, p);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_vp2_atomic(const VP2Topo& t, const VP2Pass& p, hipStream_t s) {
+  if (p.bc) {
+    if (p.want_A && p.want_b) return launch_t<true, true, true>(t, p, s);
+    if (p.want_A) return launch_t<true, false, true>(t, p, s);
+    return launch_t<false, true, true>(t, p, s);
+  }
+  if (p.want_A && p.want_b) return launch_t<true, true, false>(t, p, s);
+  if (p.want_A) return launch_t<true, false, false>(t, p, s);
+  if (p.want_b) return launch_t<false, true, false>(t, p, s);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace dfa

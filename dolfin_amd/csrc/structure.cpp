@@ -1,0 +1,311 @@
+// Host-side structure phase: CSR sparsity and the blocked-scatter work partition.
+// Runs once per (mesh, dofmap), like the first call of AssemblerBase::init_global_tensor
+// (dolfin/fem/AssemblerBase.cpp:137-285).
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <functional>
+#include <numeric>
+#include <stdexcept>
+#include <thread>
+
+#include "common.h"
+
+namespace dfa {
+
+namespace {
+
+int num_threads() {
+  unsigned n = std::thread::hardware_concurrency();
+  if (n == 0) n = 4;
+  return static_cast<int>(std::min(n, 16u));
+}
+
+// Run f(begin, end) over [0, n) split into contiguous chunks, one per thread.
+void parallel_for(int64_t n, const std::function<void(int64_t, int64_t, int)>& f) {
+  const int T = static_cast<int>(std::min<int64_t>(num_threads(), std::max<int64_t>(n, 1)));
+  if (T <= 1) {
+    f(0, n, 0);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for (int t = 0; t < T; ++t) {
+    const int64_t b = n * t / T, e = n * (t + 1) / T;
+    th.emplace_back(f, b, e, t);
+  }
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+// Row r's columns are the sorted unique dofs sharing a cell with r
+// (SparsityPatternBuilder.cpp:95-111 -> SparsityPattern::insert_entries, sorted export in
+// SparsityPattern::diagonal_pattern, SparsityPattern.cpp:511-542), plus (r, r) when the
+// diagonal is kept (SparsityPatternBuilder.cpp:245-264). Built row-parallel from a
+// dof -> cell adjacency with a per-thread marker array instead of dolfin::Set inserts.
+void build_csr(int64_t num_cells, int32_t nd, const int32_t* cd, int64_t num_rows,
+               bool keep_diagonal, HostCSR& out) {
+  std::vector<int64_t> aptr(num_rows + 1, 0);
+  for (int64_t i = 0; i < num_cells * nd; ++i) {
+    const int32_t d = cd[i];
+    if (d < 0 || d >= num_rows) throw std::runtime_error("cell_dofs entry out of range");
+    aptr[d + 1]++;
+  }
+  for (int64_t r = 0; r < num_rows; ++r) aptr[r + 1] += aptr[r];
+  std::vector<int32_t> adj(aptr[num_rows]);
+  {
+    std::vector<int64_t> pos(aptr.begin(), aptr.end() - 1);
+    for (int64_t c = 0; c < num_cells; ++c)
+      for (int j = 0; j < nd; ++j) adj[pos[cd[c * nd + j]]++] = static_cast<int32_t>(c);
+  }
+  std::vector<int32_t> len(num_rows);
+  const int T = num_threads();
+  std::vector<std::vector<int32_t>> marks(T);
+  parallel_for(num_rows, [&](int64_t b, int64_t e, int t) {
+    auto& mark = marks[t];
+    mark.assign(num_rows, -1);
+    for (int64_t r = b; r < e; ++r) {
+      int32_t n = 0;
+      for (int64_t k = aptr[r]; k < aptr[r + 1]; ++k) {
+        const int32_t* d = cd + static_cast<int64_t>(adj[k]) * nd;
+        for (int j = 0; j < nd; ++j)
+          if (mark[d[j]] != r) {
+            mark[d[j]] = static_cast<int32_t>(r);
+            ++n;
+          }
+      }
+      if (keep_diagonal && mark[r] != r) ++n;
+      len[r] = n;
+    }
+  });
+  out.row_ptr.assign(num_rows + 1, 0);
+  int64_t acc = 0;
+  for (int64_t r = 0; r < num_rows; ++r) {
+    acc += len[r];
+    if (acc > INT32_MAX) throw std::runtime_error("nnz exceeds int32 (la_index) range");
+    out.row_ptr[r + 1] = static_cast<int32_t>(acc);
+  }
+  out.col.resize(acc);
+  parallel_for(num_rows, [&](int64_t b, int64_t e, int t) {
+    auto& mark = marks[t];
+    std::fill(mark.begin(), mark.end(), -1);
+    for (int64_t r = b; r < e; ++r) {
+      int32_t* dst = out.col.data() + out.row_ptr[r];
+      int32_t n = 0;
+      for (int64_t k = aptr[r]; k < aptr[r + 1]; ++k) {
+        const int32_t* d = cd + static_cast<int64_t>(adj[k]) * nd;
+        for (int j = 0; j < nd; ++j)
+          if (mark[d[j]] != r) {
+            mark[d[j]] = static_cast<int32_t>(r);
+            dst[n++] = d[j];
+          }
+      }
+      if (keep_diagonal && mark[r] != r) dst[n++] = static_cast<int32_t>(r);
+      std::sort(dst, dst + n);
+    }
+    std::vector<int32_t>().swap(mark);
+  });
+}
+
+// Blocked-scatter partition for P1 (one dof per vertex).
+//  1. vertex <-> dof maps from cell_dofs (must be consistent: P1 dofs live on vertices);
+//  2. recursive coordinate bisection of the dofs into blocks of <= target_rows rows;
+//  3. every cell is listed once in each distinct block owning one of its dofs;
+//  4. per block: local nodes (all vertices of its cells, ascending vertex id), owned rows
+//     (ascending dof) with their LDS value offsets, and per cell appearance the local node
+//     ids plus the 4-bit rank of column b inside CSR row a for every owned a.
+void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* cells,
+                         const int32_t* cd, const int32_t* kd, const double* x,
+                         const HostCSR& csr, int32_t target_rows, P1BlockPlan& P) {
+  const int nvc = gdim + 1;
+  const int64_t ndofs = static_cast<int64_t>(csr.row_ptr.size()) - 1;
+  P = P1BlockPlan();
+  P.nvc = nvc;
+  std::vector<int32_t> vdof(nv, -1), dofv(ndofs, -1), vk(nv, -1);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int a = 0; a < nvc; ++a) {
+      const int32_t v = cells[c * nvc + a], d = cd[c * nvc + a], k = kd[c * nvc + a];
+      if (v < 0 || v >= nv) throw std::runtime_error("cell vertex out of range");
+      if ((vdof[v] >= 0 && vdof[v] != d) || (dofv[d] >= 0 && dofv[d] != v))
+        throw std::runtime_error("P1 dofmap is not a function of the vertex");
+      if (vk[v] >= 0 && vk[v] != k)
+        throw std::runtime_error("P1 coefficient dofmap is not a function of the vertex");
+      vdof[v] = d;
+      dofv[d] = v;
+      vk[v] = k;
+    }
+  P.kdof_is_vertex = true;
+  for (int64_t v = 0; v < nv; ++v)
+    if (vk[v] >= 0 && vk[v] != v) P.kdof_is_vertex = false;
+
+  // 2. RCB over the dofs that appear in cells
+  std::vector<int32_t> ids;
+  ids.reserve(ndofs);
+  for (int64_t d = 0; d < ndofs; ++d)
+    if (dofv[d] >= 0) ids.push_back(static_cast<int32_t>(d));
+  std::vector<std::pair<int64_t, int64_t>> leaves;
+  std::function<void(int64_t, int64_t)> rcb = [&](int64_t lo, int64_t hi) {
+    if (hi - lo <= target_rows) {
+      if (hi > lo) leaves.emplace_back(lo, hi);
+      return;
+    }
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int64_t i = lo; i < hi; ++i) {
+      const double* p = x + static_cast<int64_t>(dofv[ids[i]]) * gdim;
+      for (int k = 0; k < gdim; ++k) {
+        mn[k] = std::min(mn[k], p[k]);
+        mx[k] = std::max(mx[k], p[k]);
+      }
+    }
+    int ax = 0;
+    for (int k = 1; k < gdim; ++k)
+      if (mx[k] - mn[k] > mx[ax] - mn[ax]) ax = k;
+    const int64_t mid = lo + (hi - lo) / 2;
+    std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi,
+                     [&](int32_t a, int32_t b) {
+                       const double pa = x[static_cast<int64_t>(dofv[a]) * gdim + ax];
+                       const double pb = x[static_cast<int64_t>(dofv[b]) * gdim + ax];
+                       return pa < pb || (pa == pb && a < b);
+                     });
+    rcb(lo, mid);
+    rcb(mid, hi);
+  };
+  rcb(0, static_cast<int64_t>(ids.size()));
+  const int32_t nb = static_cast<int32_t>(leaves.size());
+  P.num_blocks = nb;
+  std::vector<int32_t> owner(ndofs, -1);
+  for (int32_t b = 0; b < nb; ++b)
+    for (int64_t i = leaves[b].first; i < leaves[b].second; ++i) owner[ids[i]] = b;
+
+  // 3. cell appearances
+  P.cell_ptr.assign(nb + 1, 0);
+  auto distinct_blocks = [&](int64_t c, int32_t* bl) {
+    int n = 0;
+    for (int a = 0; a < nvc; ++a) {
+      const int32_t b = owner[cd[c * nvc + a]];
+      bool seen = false;
+      for (int k = 0; k < n; ++k) seen |= (bl[k] == b);
+      if (!seen) bl[n++] = b;
+    }
+    return n;
+  };
+  for (int64_t c = 0; c < nc; ++c) {
+    int32_t bl[4];
+    const int n = distinct_blocks(c, bl);
+    for (int k = 0; k < n; ++k) P.cell_ptr[bl[k] + 1]++;
+  }
+  for (int32_t b = 0; b < nb; ++b) P.cell_ptr[b + 1] += P.cell_ptr[b];
+  std::vector<int32_t> app(P.cell_ptr[nb]);
+  {
+    std::vector<int32_t> pos(P.cell_ptr.begin(), P.cell_ptr.end() - 1);
+    for (int64_t c = 0; c < nc; ++c) {
+      int32_t bl[4];
+      const int n = distinct_blocks(c, bl);
+      for (int k = 0; k < n; ++k) app[pos[bl[k]]++] = static_cast<int32_t>(c);
+    }
+  }
+
+  // 4. per-block tables, built in parallel into per-block vectors then concatenated
+  std::vector<std::vector<int32_t>> bnodes(nb);
+  std::vector<std::vector<int32_t>> brows(nb);
+  std::vector<std::vector<uint32_t>> binfo(nb);
+  P.cell_ids.resize(app.size());
+  P.cell_rank.resize(app.size());
+  P.blk_nnz.assign(nb, 0);
+  std::atomic<int> failed{0};
+  std::string fail_msg;
+  std::vector<std::vector<int32_t>> blk_dofs(nb);
+  for (int32_t b = 0; b < nb; ++b) {
+    blk_dofs[b].assign(ids.begin() + leaves[b].first, ids.begin() + leaves[b].second);
+    std::sort(blk_dofs[b].begin(), blk_dofs[b].end());
+  }
+  parallel_for(nb, [&](int64_t b0, int64_t b1, int) {
+    for (int64_t b = b0; b < b1; ++b) {
+      auto& nodes = bnodes[b];
+      for (int32_t k = P.cell_ptr[b]; k < P.cell_ptr[b + 1]; ++k)
+        for (int a = 0; a < nvc; ++a) nodes.push_back(cells[static_cast<int64_t>(app[k]) * nvc + a]);
+      std::sort(nodes.begin(), nodes.end());
+      nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+      auto local = [&](int32_t v) {
+        return static_cast<uint32_t>(std::lower_bound(nodes.begin(), nodes.end(), v) - nodes.begin());
+      };
+      if (nodes.size() > static_cast<size_t>(kMaxBlockNodes)) {
+        failed = 1;
+        continue;
+      }
+      // owned rows
+      std::vector<int32_t> row_off_of_node(nodes.size(), -1);
+      uint32_t off = 0;
+      for (int32_t d : blk_dofs[b]) {
+        const uint32_t ln = local(dofv[d]);
+        brows[b].push_back(d);
+        binfo[b].push_back(ln | (off << 16));
+        row_off_of_node[ln] = static_cast<int32_t>(off);
+        off += static_cast<uint32_t>(csr.row_ptr[d + 1] - csr.row_ptr[d]);
+      }
+      if (off > static_cast<uint32_t>(kMaxBlockNnz)) {
+        failed = 2;
+        continue;
+      }
+      P.blk_nnz[b] = static_cast<int32_t>(off);
+      for (int32_t k = P.cell_ptr[b]; k < P.cell_ptr[b + 1]; ++k) {
+        const int64_t c = app[k];
+        uint64_t idw = 0, rk = 0;
+        for (int a = 0; a < nvc; ++a)
+          idw |= static_cast<uint64_t>(local(cells[c * nvc + a])) << (16 * a);
+        for (int a = 0; a < nvc; ++a) {
+          const int32_t da = cd[c * nvc + a];
+          if (owner[da] != b) continue;
+          const int32_t* rb = csr.col.data() + csr.row_ptr[da];
+          const int32_t* re = csr.col.data() + csr.row_ptr[da + 1];
+          for (int bb = 0; bb < nvc; ++bb) {
+            const int32_t db = cd[c * nvc + bb];
+            const int64_t r = std::lower_bound(rb, re, db) - rb;
+            if (r >= kMaxRank) {
+              failed = 3;
+              continue;
+            }
+            rk |= static_cast<uint64_t>(r) << (4 * (a * 4 + bb));
+          }
+        }
+        P.cell_ids[k] = idw;
+        P.cell_rank[k] = rk;
+      }
+    }
+  });
+  if (failed == 1) throw std::runtime_error("blocked plan: too many nodes in a block");
+  if (failed == 2) throw std::runtime_error("blocked plan: too many nonzeros in a block");
+  if (failed == 3) throw std::runtime_error("blocked plan: a CSR row has more than 16 entries");
+
+  P.node_ptr.assign(nb + 1, 0);
+  P.row_ptr.assign(nb + 1, 0);
+  for (int32_t b = 0; b < nb; ++b) {
+    P.node_ptr[b + 1] = P.node_ptr[b] + static_cast<int32_t>(bnodes[b].size());
+    P.row_ptr[b + 1] = P.row_ptr[b] + static_cast<int32_t>(brows[b].size());
+    P.max_nodes = std::max<int32_t>(P.max_nodes, static_cast<int32_t>(bnodes[b].size()));
+    P.max_rows = std::max<int32_t>(P.max_rows, static_cast<int32_t>(brows[b].size()));
+    P.max_nnz = std::max(P.max_nnz, P.blk_nnz[b]);
+    P.max_cells = std::max(P.max_cells, P.cell_ptr[b + 1] - P.cell_ptr[b]);
+  }
+  P.node_vertex.resize(P.node_ptr[nb]);
+  P.node_kdof.resize(P.node_ptr[nb]);
+  P.node_dof.resize(P.node_ptr[nb]);
+  P.row_dof.resize(P.row_ptr[nb]);
+  P.row_info.resize(P.row_ptr[nb]);
+  parallel_for(nb, [&](int64_t b0, int64_t b1, int) {
+    for (int64_t b = b0; b < b1; ++b) {
+      int32_t o = P.node_ptr[b];
+      for (int32_t v : bnodes[b]) {
+        P.node_vertex[o] = v;
+        P.node_kdof[o] = vk[v];
+        P.node_dof[o] = vdof[v];
+        ++o;
+      }
+      std::copy(brows[b].begin(), brows[b].end(), P.row_dof.begin() + P.row_ptr[b]);
+      std::copy(binfo[b].begin(), binfo[b].end(), P.row_info.begin() + P.row_ptr[b]);
+    }
+  });
+}
+
+}  // namespace dfa

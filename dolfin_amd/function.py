@@ -1,0 +1,110 @@
+"""Function spaces, dofmaps and coefficient functions (host construction, device storage).
+
+Dof numbering (the input that fixes the CSR structure):
+  P1            : UFC numbering, dof = vertex (DofMapBuilder with reorder_dofs_serial=False,
+                  dolfin/fem/DofMapBuilder.cpp:253-272), or dof = perm[vertex] when a node
+                  permutation (e.g. DOLFIN's Boost RCM order) is supplied.
+  vector P2     : nodes are vertices then edges (node = nv + edge); UFC numbering is
+                  component-major, dof = comp*(nv+ne) + node; with a node permutation the
+                  numbering is block-interleaved, dof = 3*perm[node] + comp
+                  (DofMapBuilder::build_dofmap, dolfin/fem/DofMapBuilder.cpp:1669-1693).
+  Cell-local order is component-major [comp][4 vertices, 6 edges]
+  (python/test/unit/fem/test_dofmap.py:112-120).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+
+
+class FunctionSpace:
+    def __init__(self, mesh, family="P", degree=1, vector=False, node_permutation=None):
+        if family not in ("P", "Lagrange", "CG"):
+            raise ValueError("only continuous Lagrange spaces are supported")
+        self._mesh = mesh
+        self.degree = degree
+        self.vector = vector
+        nc = mesh.num_cells()
+        if degree == 1 and not vector:
+            self.element = _lib.ELEMENT_P1
+            nv = mesh.num_vertices()
+            if node_permutation is None:
+                cd = mesh.cells().copy()
+            else:
+                perm = np.asarray(node_permutation, dtype=np.int32)
+                if perm.shape != (nv,) or not np.array_equal(np.sort(perm), np.arange(nv)):
+                    raise ValueError("node_permutation must permute the vertices")
+                cd = perm[mesh.cells()]
+            self._dim = nv
+            self._cell_dofs = np.ascontiguousarray(cd, dtype=np.int32)
+            self._node_coords = mesh.coordinates()
+        elif degree == 2 and vector:
+            if mesh.gdim() != 3:
+                raise ValueError("vector P2 is implemented on tetrahedra")
+            self.element = _lib.ELEMENT_VECTOR_P2
+            ce, ne = mesh.cell_edges()
+            nv = mesh.num_vertices()
+            nn = nv + ne
+            nodes = np.concatenate([mesh.cells(), nv + ce], axis=1).astype(np.int64)  # [nc, 10]
+            if node_permutation is None:
+                cd = np.concatenate([c * nn + nodes for c in range(3)], axis=1)
+            else:
+                perm = np.asarray(node_permutation, dtype=np.int64)
+                if perm.shape != (nn,):
+                    raise ValueError("node_permutation must permute the P2 nodes")
+                pn = perm[nodes]
+                cd = np.concatenate([3 * pn + c for c in range(3)], axis=1)
+            self._dim = 3 * nn
+            self._cell_dofs = np.ascontiguousarray(cd, dtype=np.int32)
+        else:
+            raise ValueError("supported spaces: FunctionSpace(mesh,'P',1), VectorFunctionSpace(mesh,'P',2)")
+        self._engines = {}
+        self._p1 = None
+
+    def mesh(self):
+        return self._mesh
+
+    def dim(self):
+        return self._dim
+
+    def cell_dofs(self):
+        return self._cell_dofs
+
+    def coefficient_space(self):
+        """P1 space whose dofmap addresses the P1 coefficients of forms on this space."""
+        if self.element == _lib.ELEMENT_P1:
+            return self
+        if self._p1 is None:
+            self._p1 = FunctionSpace(self._mesh, "P", 1)
+        return self._p1
+
+    def __repr__(self):
+        kind = "VectorP2" if self.vector else "P1"
+        return f"FunctionSpace({kind}, dim={self._dim})"
+
+
+def VectorFunctionSpace(mesh, family="P", degree=2, node_permutation=None):
+    return FunctionSpace(mesh, family, degree, vector=True, node_permutation=node_permutation)
+
+
+class Function:
+    """Finite element function with its coefficient vector resident on the device."""
+
+    def __init__(self, V, values=None, device="cuda"):
+        self._V = V
+        if values is None:
+            self._vec = torch.zeros(V.dim(), dtype=torch.float64, device=device)
+        else:
+            v = torch.as_tensor(values, dtype=torch.float64)
+            if v.shape != (V.dim(),):
+                raise ValueError(f"expected {V.dim()} values, got {tuple(v.shape)}")
+            self._vec = v.to(device).contiguous()
+
+    def function_space(self):
+        return self._V
+
+    def vector(self):
+        return self._vec
+
+    def assign(self, values):
+        self._vec.copy_(torch.as_tensor(values, dtype=torch.float64))

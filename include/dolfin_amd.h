@@ -139,18 +139,6 @@ int dfa_adjoint(dfa_assembler h, int32_t a_form, const double* d_x, const double
                 const double* d_c1, const double* d_f, const double* h_f3, const double* d_u,
                 double* d_J, double* d_dJdc0, double* d_dJdc1, double* d_dJdx, void* stream);
 
-/*
- * Batched assembly: many independent (mesh, space) assemblers, one bilinear form each,
- * assembled by ONE persistent kernel launch (no counterpart in the reference, which
- * assembles one form per call). forms[i], d_x[i], d_c0[i], d_c1[i], d_values[i] are per
- * assembler; all assemblers must use DFA_SCATTER_ATOMIC-compatible elements.
- * cells_per_item: cells each work item takes from the shared queue (0 = autotuned).
- */
-int dfa_assemble_batch(int32_t n, const dfa_assembler* hs, const int32_t* forms,
-                       const double* const* d_x, const double* const* d_c0,
-                       const double* const* d_c1, double* const* d_values,
-                       int32_t cells_per_item, void* stream);
-
 /* Introspection for benchmarks: algorithmic bytes of one matrix assembly (the compulsory
    traffic: connectivity, coordinates, coefficients, CSR values written once, col/row_ptr
    read once) and the bytes the plan of the chosen strategy actually stores. */

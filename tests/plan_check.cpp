@@ -1,0 +1,116 @@
+// CPU check of the blocked-scatter work partition (test helper, host code only).
+//
+// Builds the CSR and the P1 block plan with the library's own host code and walks the plan
+// exactly the way the GPU kernel indexes it -- LDS node table, per-appearance local ids and
+// 4-bit ranks, owned-row offsets, write-out -- asserting every index against the sizes the
+// kernel's LDS layout reserves. It then accumulates a P1 mass matrix through the plan so the
+// test can compare with the oracle: a wrong rank or offset shows up as a wrong value.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <vector>
+
+#include "../dolfin_amd/csrc/common.h"
+
+extern "C" int plan_check_p1(int32_t gdim, int64_t nv, int64_t nc, const int32_t* cells,
+                             const int32_t* cd, const int32_t* kd, const double* x,
+                             int64_t ndofs, int32_t target_rows, int32_t keep_diagonal,
+                             double* values, int32_t* row_ptr_out, int64_t nnz_cap,
+                             int32_t* stats) {
+  try {
+    dfa::HostCSR csr;
+    dfa::build_csr(nc, gdim + 1, cd, ndofs, keep_diagonal != 0, csr);
+    if (static_cast<int64_t>(csr.col.size()) > nnz_cap) return -10;
+    dfa::P1BlockPlan P;
+    dfa::build_p1_block_plan(gdim, nv, nc, cells, cd, kd, x, csr, target_rows, P);
+    const int nvc = gdim + 1;
+    const double fact = gdim == 1 ? 1.0 : (gdim == 2 ? 2.0 : 6.0);
+    std::vector<int> covered(ndofs, 0);
+    int64_t total_app = 0;
+    for (int b = 0; b < P.num_blocks; ++b) {
+      const int n0 = P.node_ptr[b], nn = P.node_ptr[b + 1] - n0;
+      if (nn <= 0 || nn > P.max_nodes) return -1;
+      std::vector<int> own(nn, -1);
+      const int r0 = P.row_ptr[b], nr = P.row_ptr[b + 1] - r0;
+      if (nr <= 0 || nr > P.max_rows) return -2;
+      const int nnzb = P.blk_nnz[b];
+      if (nnzb > P.max_nnz || nnzb > dfa::kMaxBlockNnz) return -3;
+      std::vector<double> vals(nnzb, 0.0);
+      for (int i = 0; i < nr; ++i) {
+        const uint32_t info = P.row_info[r0 + i];
+        const int ln = static_cast<int>(info & 0xffffu), off = static_cast<int>(info >> 16);
+        if (ln >= nn || off >= nnzb) return -4;
+        if (P.node_dof[n0 + ln] != P.row_dof[r0 + i]) return -5;
+        own[ln] = off;
+        covered[P.row_dof[r0 + i]]++;
+      }
+      const int c0 = P.cell_ptr[b], ncb = P.cell_ptr[b + 1] - c0;
+      total_app += ncb;
+      for (int k = 0; k < ncb; ++k) {
+        const uint64_t ids = P.cell_ids[c0 + k], rk = P.cell_rank[c0 + k];
+        int l[4];
+        double p[4][3];
+        for (int a = 0; a < nvc; ++a) {
+          l[a] = static_cast<int>((ids >> (16 * a)) & 0xffffu);
+          if (l[a] >= nn) return -6;
+          const int32_t v = P.node_vertex[n0 + l[a]];
+          for (int q = 0; q < gdim; ++q) p[a][q] = x[static_cast<int64_t>(v) * gdim + q];
+        }
+        // |det J| for the mass matrix
+        double det;
+        if (gdim == 1) det = p[1][0] - p[0][0];
+        else if (gdim == 2)
+          det = (p[1][0] - p[0][0]) * (p[2][1] - p[0][1]) - (p[1][1] - p[0][1]) * (p[2][0] - p[0][0]);
+        else {
+          double e[3][3];
+          for (int j = 0; j < 3; ++j)
+            for (int q = 0; q < 3; ++q) e[j][q] = p[j + 1][q] - p[0][q];
+          det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
+                e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+                e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+        }
+        const double w = std::fabs(det) / (fact * (gdim + 1.0) * (gdim + 2.0));
+        for (int a = 0; a < nvc; ++a) {
+          if (own[l[a]] < 0) continue;
+          for (int bb = 0; bb < nvc; ++bb) {
+            const int slot = own[l[a]] + static_cast<int>((rk >> (4 * (a * 4 + bb))) & 15u);
+            if (slot >= nnzb) return -7;
+            vals[slot] += (a == bb) ? 2.0 * w : w;
+          }
+        }
+      }
+      for (int i = 0; i < nr; ++i) {
+        const int32_t dof = P.row_dof[r0 + i];
+        const int off = static_cast<int>(P.row_info[r0 + i] >> 16);
+        const int base = csr.row_ptr[dof], len = csr.row_ptr[dof + 1] - base;
+        if (off + len > nnzb) return -8;
+        for (int j = 0; j < len; ++j) values[base + j] = vals[off + j];
+      }
+    }
+    for (int64_t d = 0; d < ndofs; ++d)
+      if (covered[d] != 1) return -9;
+    for (size_t i = 0; i < csr.row_ptr.size(); ++i) row_ptr_out[i] = csr.row_ptr[i];
+    stats[0] = P.num_blocks;
+    stats[1] = P.max_nodes;
+    stats[2] = P.max_rows;
+    stats[3] = P.max_nnz;
+    stats[4] = static_cast<int32_t>(total_app);
+    return 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "plan_check: %s\n", e.what());
+    return -100;
+  }
+}
+
+// CSR structure only (for structure parity tests without a GPU).
+extern "C" int64_t csr_host(int64_t nc, int32_t nd, const int32_t* cd, int64_t ndofs,
+                            int32_t keep_diagonal, int32_t* row_ptr, int32_t* col, int64_t cap) {
+  dfa::HostCSR csr;
+  dfa::build_csr(nc, nd, cd, ndofs, keep_diagonal != 0, csr);
+  const int64_t nnz = static_cast<int64_t>(csr.col.size());
+  if (nnz > cap) return -nnz;
+  for (size_t i = 0; i < csr.row_ptr.size(); ++i) row_ptr[i] = csr.row_ptr[i];
+  for (int64_t i = 0; i < nnz; ++i) col[i] = csr.col[i];
+  return nnz;
+}
