@@ -161,7 +161,9 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
     int ax = 0;
     for (int k = 1; k < gdim; ++k)
       if (mx[k] - mn[k] > mx[ax] - mn[ax]) ax = k;
-    const int64_t mid = lo + (hi - lo) / 2;
+    // split so that the leaves end up as equal as possible with <= target_rows each
+    const int64_t nleaf = (hi - lo + target_rows - 1) / target_rows;
+    const int64_t mid = lo + (hi - lo) * (nleaf / 2) / nleaf;
     std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi,
                      [&](int32_t a, int32_t b) {
                        const double pa = x[static_cast<int64_t>(dofv[a]) * gdim + ax];
