@@ -71,6 +71,7 @@ struct dfa_assembler_s {
   dfa::P1BlockTopo btopo;
   dfa::DeviceArena plan;
   int64_t appearances = 0;
+  bool kdof_injective = false;
   // vector P2: per cell, rank of node b among the neighbours of node a (100 bytes) and the
   // CSR strides (node stride s1, component stride s2 per row)
   uint8_t* d_vp2_rank = nullptr;
@@ -245,6 +246,9 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
             t.row_dof = h->plan.upload(P.row_dof, s);
             t.row_info = h->plan.upload(P.row_info, s);
             t.blk_nnz = h->plan.upload(P.blk_nnz, s);
+            t.run_ptr = h->plan.upload(P.run_ptr, s);
+            t.runs = h->plan.upload(P.runs, s);
+            h->kdof_injective = P.kdof_injective;
             t.csr_row_ptr = h->d_row_ptr;
             h->appearances = static_cast<int64_t>(P.cell_ids.size());
             h->has_plan = true;
@@ -467,7 +471,10 @@ int dfa_adjoint(dfa_assembler h, int32_t a_form, const double* d_x, const double
       a.J = d_J;
       a.dJdk = d_dJdc0;
       a.dJdx = d_dJdx;
-      hip_check(dfa::launch_adjoint_p1(a, s), "adjoint_p1_kernel");
+      if (h->has_plan && h->kdof_injective)
+        hip_check(dfa::launch_p1_adjoint_blocked(h->btopo, a, s), "p1_adjoint_blocked_kernel");
+      else
+        hip_check(dfa::launch_adjoint_p1(a, s), "adjoint_p1_kernel");
       return DFA_OK;
     }
     if (a_form != DFA_FORM_VP2_ELASTICITY) return fail("vector P2 adjoint supports elasticity", DFA_ERR_ARG);

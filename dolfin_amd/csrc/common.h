@@ -32,6 +32,7 @@ struct P1BlockPlan {
   int32_t num_blocks = 0;
   int32_t max_nodes = 0, max_rows = 0, max_nnz = 0, max_cells = 0;
   bool kdof_is_vertex = false;          // coefficient dof == vertex id everywhere
+  bool kdof_injective = false;          // distinct vertices carry distinct coefficient dofs
   std::vector<int32_t> node_ptr;        // [nb+1]
   std::vector<int32_t> node_vertex;     // global vertex of each local node
   std::vector<int32_t> node_kdof;       // coefficient dof of each local node
@@ -43,6 +44,10 @@ struct P1BlockPlan {
   std::vector<int32_t> row_dof;         // global row (dof) of each owned row
   std::vector<uint32_t> row_info;       // (local node id) | (LDS value offset << 16)
   std::vector<int32_t> blk_nnz;         // [nb] values per block
+  // Write-out runs: maximal sequences of owned rows with consecutive dofs are one contiguous
+  // CSR range. Packed as csr_start | local_offset << 32 | length << 48.
+  std::vector<int32_t> run_ptr;         // [nb+1]
+  std::vector<uint64_t> runs;
 };
 
 // Throws std::runtime_error on inconsistent input or exceeded limits.

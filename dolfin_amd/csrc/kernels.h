@@ -47,6 +47,8 @@ struct P1BlockTopo {
   const int32_t* row_dof = nullptr;
   const uint32_t* row_info = nullptr;
   const int32_t* blk_nnz = nullptr;
+  const int32_t* run_ptr = nullptr;
+  const uint64_t* runs = nullptr;
   const int32_t* csr_row_ptr = nullptr;
 };
 
@@ -96,6 +98,7 @@ struct AdjointP1 {
   double* dJdx = nullptr;
 };
 hipError_t launch_adjoint_p1(const AdjointP1& a, hipStream_t s);
+hipError_t launch_p1_adjoint_blocked(const P1BlockTopo& t, const AdjointP1& a, hipStream_t s);
 
 struct AdjointVP2 {
   int64_t num_cells = 0;

@@ -200,17 +200,20 @@ __global__ __launch_bounds__(kThreads) void p1_atomic_kernel(P1AtomicTopo t, P1P
 // Blocked (owner-computes) scatter: workgroup per row block.
 // ---------------------------------------------------------------------------------------
 struct LdsLayout {
-  int x, kap, f, bcv, bcf, own, vals, bv, rdof, roff, total;
+  int node, f, bcv, bcf, own, vals, bv, rdof, total;
 };
 
 __host__ __device__ inline int align16(int n) { return (n + 15) & ~15; }
 
-__host__ __device__ inline LdsLayout lds_layout(int D, int maxN, int maxR, int maxNnz, bool kap,
-                                                bool f, bool bc) {
+// Node record: coordinates and kappa packed in 32 bytes (16 for intervals) so one cell vertex
+// is one or two 128-bit LDS reads.
+__host__ __device__ constexpr int node_stride(int D) { return D == 1 ? 2 : 4; }
+
+__host__ __device__ inline LdsLayout lds_layout(int D, int maxN, int maxR, int maxNnz, bool f,
+                                                bool bc) {
   LdsLayout L;
   int o = 0;
-  L.x = o;    o += align16(maxN * D * 8);
-  L.kap = o;  o += kap ? align16(maxN * 8) : 0;
+  L.node = o; o += align16(maxN * node_stride(D) * 8);
   L.f = o;    o += f ? align16(maxN * 8) : 0;
   L.bcv = o;  o += bc ? align16(maxN * 8) : 0;
   L.bcf = o;  o += bc ? align16(maxN) : 0;
@@ -218,19 +221,84 @@ __host__ __device__ inline LdsLayout lds_layout(int D, int maxN, int maxR, int m
   L.vals = o; o += align16(maxNnz * 8);
   L.bv = o;   o += f ? align16(maxR * 8) : 0;
   L.rdof = o; o += align16(maxR * 4);
-  L.roff = o; o += align16(maxR * 4);
   L.total = o;
   return L;
+}
+
+template <int D>
+__device__ __forceinline__ void load_node(const double* __restrict__ rec, double (&pc)[D], double& kap) {
+  if constexpr (D == 1) {
+    const double2 r = *reinterpret_cast<const double2*>(rec);
+    pc[0] = r.x;
+    kap = r.y;
+  } else {
+    const double2 r0 = *reinterpret_cast<const double2*>(rec);
+    const double2 r1 = *reinterpret_cast<const double2*>(rec + 2);
+    pc[0] = r0.x;
+    pc[1] = r0.y;
+    if constexpr (D == 3) {
+      pc[2] = r1.x;
+      kap = r1.y;
+    } else {
+      kap = r1.x;
+    }
+  }
+}
+
+// XCD-aware block order: workgroup i runs on XCD i % 8; each XCD gets a contiguous range of
+// (spatially adjacent) blocks so that shared ghost nodes hit the same L2.
+__device__ __forceinline__ int xcd_block(int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+  return xcd * q + min(xcd, r) + j;
+}
+
+// Prologue shared by the assembly and adjoint kernels: gather the block's nodes into LDS.
+template <int D, bool NEED_K, bool NEED_F, bool BC>
+__device__ __forceinline__ void gather_nodes(const P1BlockTopo& t, const double* __restrict__ x,
+                                             const double* __restrict__ kappa,
+                                             const double* __restrict__ f,
+                                             const uint8_t* __restrict__ bcm,
+                                             const double* __restrict__ bcv, int blk, double* snode,
+                                             double* sf, double* sbcv, uint8_t* sbcf, int32_t* sown) {
+  constexpr int NS = node_stride(D);
+  const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
+  for (int i = threadIdx.x; i < nn; i += kThreads) {
+    const int32_t v = t.node_vertex[n0 + i];
+    double rec[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) rec[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) rec[k] = x[static_cast<int64_t>(v) * D + k];
+    if constexpr (NEED_K || NEED_F) {
+      const int32_t kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + i];
+      if constexpr (NEED_K) rec[D] = kappa[kd];
+      if constexpr (NEED_F) sf[i] = f[kd];
+    }
+    if constexpr (NS == 2) {
+      *reinterpret_cast<double2*>(snode + i * NS) = make_double2(rec[0], rec[1]);
+    } else {
+      *reinterpret_cast<double2*>(snode + i * NS) = make_double2(rec[0], rec[1]);
+      *reinterpret_cast<double2*>(snode + i * NS + 2) = make_double2(rec[2], rec[3]);
+    }
+    if constexpr (BC) {
+      const int32_t dd = t.node_dof[n0 + i];
+      const uint8_t m = bcm[dd];
+      sbcf[i] = m;
+      sbcv[i] = m ? bcv[dd] : 0.0;
+    }
+    sown[i] = -1;
+  }
 }
 
 template <int D, int AFORM, bool WANT_B, bool BC>
 __global__ __launch_bounds__(kThreads) void p1_blocked_kernel(P1BlockTopo t, P1Pass p, LdsLayout L) {
   constexpr int N = D + 1;
+  constexpr int NS = node_stride(D);
   constexpr bool WANT_A = AFORM >= 0;
   constexpr bool NEED_K = AFORM == DFA_FORM_P1_POISSON;
   extern __shared__ __align__(16) unsigned char lds[];
-  double* sx = reinterpret_cast<double*>(lds + L.x);
-  double* sk = reinterpret_cast<double*>(lds + L.kap);
+  double* snode = reinterpret_cast<double*>(lds + L.node);
   double* sf = reinterpret_cast<double*>(lds + L.f);
   double* sbcv = reinterpret_cast<double*>(lds + L.bcv);
   uint8_t* sbcf = lds + L.bcf;
@@ -238,62 +306,38 @@ __global__ __launch_bounds__(kThreads) void p1_blocked_kernel(P1BlockTopo t, P1P
   double* svals = reinterpret_cast<double*>(lds + L.vals);
   double* sbv = reinterpret_cast<double*>(lds + L.bv);
   int32_t* srdof = reinterpret_cast<int32_t*>(lds + L.rdof);
-  int32_t* sroff = reinterpret_cast<int32_t*>(lds + L.roff);
 
-  // XCD-aware block order: workgroup i runs on XCD i % 8; give each XCD a contiguous range
-  // of (spatially adjacent) blocks so that shared ghost nodes hit the same L2.
-  const int nb = t.num_blocks;
-  const int q = nb >> 3, r = nb & 7;
-  const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
-  const int blk = xcd * q + min(xcd, r) + j;
+  const int blk = xcd_block(t.num_blocks);
   const int tid = threadIdx.x;
+  const bool do_A = WANT_A && p.want_A;
 
-  const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
-  for (int i = tid; i < nn; i += kThreads) {
-    const int32_t v = t.node_vertex[n0 + i];
-#pragma unroll
-    for (int k = 0; k < D; ++k) sx[i * D + k] = p.x[static_cast<int64_t>(v) * D + k];
-    if constexpr (NEED_K || WANT_B) {
-      const int32_t kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + i];
-      if constexpr (NEED_K) sk[i] = p.kappa[kd];
-      if constexpr (WANT_B) sf[i] = p.f[kd];
-    }
-    if constexpr (BC) {
-      const int32_t dd = t.node_dof[n0 + i];
-      const uint8_t m = p.bc_mask[dd];
-      sbcf[i] = m;
-      sbcv[i] = m ? p.bc_value[dd] : 0.0;
-    }
-    sown[i] = -1;
-  }
+  gather_nodes<D, NEED_K, WANT_B, BC>(t, p.x, p.kappa, p.f, p.bc_mask, p.bc_value, blk, snode, sf,
+                                      sbcv, sbcf, sown);
   const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
   const int nnz = t.blk_nnz[blk];
-  if (WANT_A && p.want_A)
-    for (int i = tid; i < nnz; i += kThreads) svals[i] = 0.0;
+  if (do_A)
+    for (int i = tid; i < (nnz + 1) / 2; i += kThreads)
+      reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
   if constexpr (WANT_B)
     for (int i = tid; i < nr; i += kThreads) sbv[i] = 0.0;
   __syncthreads();
   for (int i = tid; i < nr; i += kThreads) {
     const uint32_t info = t.row_info[r0 + i];
-    const int off = static_cast<int>(info >> 16);
-    sown[info & 0xffffu] = off | (i << 16);
+    sown[info & 0xffffu] = static_cast<int>(info >> 16) | (i << 16);
     srdof[i] = t.row_dof[r0 + i];
-    sroff[i] = off;
   }
   __syncthreads();
 
   const int c0 = t.cell_ptr[blk], ncb = t.cell_ptr[blk + 1] - c0;
   for (int k = tid; k < ncb; k += kThreads) {
     const uint64_t ids = t.cell_ids[c0 + k];
-    const uint64_t rk = (WANT_A && p.want_A) ? t.cell_rank[c0 + k] : 0;
+    const uint64_t rk = do_A ? t.cell_rank[c0 + k] : 0;
     int l[N];
     double pc[N][D], kap[N], fv[N], A[N][N], be[N];
 #pragma unroll
     for (int a = 0; a < N; ++a) {
       l[a] = static_cast<int>((ids >> (16 * a)) & 0xffffu);
-#pragma unroll
-      for (int i = 0; i < D; ++i) pc[a][i] = sx[l[a] * D + i];
-      kap[a] = NEED_K ? sk[l[a]] : 0.0;
+      load_node<D>(snode + l[a] * NS, pc[a], kap[a]);
       fv[a] = WANT_B ? sf[l[a]] : 0.0;
       be[a] = 0.0;
     }
@@ -312,34 +356,158 @@ __global__ __launch_bounds__(kThreads) void p1_blocked_kernel(P1BlockTopo t, P1P
     for (int a = 0; a < N; ++a) {
       const int own = sown[l[a]];
       if (own < 0) continue;
-      if (WANT_A && p.want_A) {
+      if (do_A) {
         const int off = own & 0xffff;
+        const uint32_t ra = static_cast<uint32_t>(rk >> (16 * a));
 #pragma unroll
         for (int b = 0; b < N; ++b)
-          atomic_add_lds(svals + off + static_cast<int>((rk >> (4 * (a * 4 + b))) & 15u), A[a][b]);
+          atomic_add_lds(svals + off + static_cast<int>((ra >> (4 * b)) & 15u), A[a][b]);
       }
       if constexpr (WANT_B) atomic_add_lds(sbv + (own >> 16), be[a]);
     }
   }
   __syncthreads();
 
-  // Write-out: 16 lanes per row; rows are ascending dofs, so a wave stores runs of
-  // consecutive CSR rows.
-  const int grp = tid >> 4, ln = tid & 15;
-  for (int i = grp; i < nr; i += kThreads / 16) {
-    const int32_t dof = srdof[i];
-    if (WANT_A && p.want_A) {
-      const int base = t.csr_row_ptr[dof];
-      const int len = t.csr_row_ptr[dof + 1] - base;
-      const int off = sroff[i];
-      for (int jj = ln; jj < len; jj += 16) {
-        double v = svals[off + jj];
-        if (p.add) v += p.values[base + jj];
-        p.values[base + jj] = v;
+  // Write-out by runs of consecutive rows: each run is one contiguous CSR range.
+  if (do_A) {
+    const int q0 = t.run_ptr[blk], nq = t.run_ptr[blk + 1] - q0;
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int q = wave; q < nq; q += kThreads / 64) {
+      const uint64_t run = t.runs[q0 + q];
+      const int64_t gstart = static_cast<int64_t>(static_cast<uint32_t>(run));
+      const int loff = static_cast<int>((run >> 32) & 0xffffu);
+      const int len = static_cast<int>(run >> 48);
+      for (int j = lane; j < len; j += 64) {
+        double v = svals[loff + j];
+        if (p.add) v += p.values[gstart + j];
+        p.values[gstart + j] = v;
       }
     }
-    if constexpr (WANT_B)
-      if (ln == 0) p.b[dof] = p.add ? p.b[dof] + sbv[i] : sbv[i];
+  }
+  if constexpr (WANT_B)
+    for (int i = tid; i < nr; i += kThreads) {
+      const int32_t dof = srdof[i];
+      p.b[dof] = p.add ? p.b[dof] + sbv[i] : sbv[i];
+    }
+}
+
+// Adjoint of J = u^T K(kappa, x) u - F(f, x)^T u on the same blocks: every owned vertex
+// accumulates dJ/dkappa and dJ/dx in LDS from the cells around it and writes them once;
+// J is summed over the cells whose local vertex 0 is owned (each cell exactly once).
+template <int D>
+__global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTopo t, AdjointP1 a,
+                                                                      int lds_node, int lds_f,
+                                                                      int lds_u, int lds_own,
+                                                                      int lds_acc) {
+  constexpr int N = D + 1;
+  constexpr int NS = node_stride(D);
+  extern __shared__ __align__(16) unsigned char lds[];
+  double* snode = reinterpret_cast<double*>(lds + lds_node);
+  double* sf = reinterpret_cast<double*>(lds + lds_f);
+  double* su = reinterpret_cast<double*>(lds + lds_u);
+  int32_t* sown = reinterpret_cast<int32_t*>(lds + lds_own);
+  double* sacc = reinterpret_cast<double*>(lds + lds_acc);  // [rows][1 + D]
+  __shared__ double sJ[kThreads / 64];
+
+  const int blk = xcd_block(t.num_blocks);
+  const int tid = threadIdx.x;
+  gather_nodes<D, true, true, false>(t, a.x, a.kappa, a.f, nullptr, nullptr, blk, snode, sf,
+                                     nullptr, nullptr, sown);
+  {
+    const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
+    for (int i = tid; i < nn; i += kThreads) su[i] = a.u[t.node_dof[n0 + i]];
+  }
+  const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
+  for (int i = tid; i < nr * (1 + D); i += kThreads) sacc[i] = 0.0;
+  __syncthreads();
+  for (int i = tid; i < nr; i += kThreads) sown[t.row_info[r0 + i] & 0xffffu] = i;
+  __syncthreads();
+
+  double Jl = 0.0;
+  const int c0 = t.cell_ptr[blk], ncb = t.cell_ptr[blk + 1] - c0;
+  for (int k = tid; k < ncb; k += kThreads) {
+    const uint64_t ids = t.cell_ids[c0 + k];
+    int l[N];
+    double pc[N][D], kap[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      l[q] = static_cast<int>((ids >> (16 * q)) & 0xffffu);
+      load_node<D>(snode + l[q] * NS, pc[q], kap[q]);
+    }
+    double g[N][D], absdet;
+    p1_geometry<D>(pc, g, absdet);
+    // grad lambda = g / det; work with |det|-scaled quantities: vol = |det|/d!
+    double det_sign_inv = 0.0;
+    {
+      // recover the signed determinant from g: det = e1 . g1 (D == 3), analogous in 2D/1D
+      double det;
+      if constexpr (D == 3) {
+        det = (pc[1][0] - pc[0][0]) * g[1][0] + (pc[1][1] - pc[0][1]) * g[1][1] +
+              (pc[1][2] - pc[0][2]) * g[1][2];
+      } else if constexpr (D == 2) {
+        det = (pc[1][0] - pc[0][0]) * g[1][0] + (pc[1][1] - pc[0][1]) * g[1][1];
+      } else {
+        det = pc[1][0] - pc[0][0];
+      }
+      det_sign_inv = 1.0 / det;
+    }
+    double G[N][D];
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+#pragma unroll
+      for (int i = 0; i < D; ++i) G[q][i] = g[q][i] * det_sign_inv;
+    const double vol = absdet / Fact<D>::value;
+    double gu[D] = {}, ks = 0.0, us = 0.0, fs = 0.0, uf = 0.0;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      const double uq = su[l[q]], fq = sf[l[q]];
+#pragma unroll
+      for (int i = 0; i < D; ++i) gu[i] += uq * G[q][i];
+      ks += kap[q];
+      us += uq;
+      fs += fq;
+      uf += uq * fq;
+    }
+    const double kb = ks / N;
+    double g2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) g2 += gu[i] * gu[i];
+    const double E = kb * vol * g2;
+    const double Gl = vol / ((D + 1.0) * (D + 2.0)) * (uf + us * fs);
+    if (sown[l[0]] >= 0) Jl += E - Gl;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      const int row = sown[l[q]];
+      if (row < 0) continue;
+      double* acc = sacc + row * (1 + D);
+      atomic_add_lds(acc, vol * g2 / N);
+      double gdl = 0.0;
+#pragma unroll
+      for (int i = 0; i < D; ++i) gdl += gu[i] * G[q][i];
+#pragma unroll
+      for (int i = 0; i < D; ++i)
+        atomic_add_lds(acc + 1 + i, kb * vol * (g2 * G[q][i] - 2.0 * gu[i] * gdl) - Gl * G[q][i]);
+    }
+  }
+  // J: wave reduction, then one global atomic per workgroup
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) Jl += __shfl_down(Jl, off, 64);
+  if ((tid & 63) == 0) sJ[tid >> 6] = Jl;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) s += sJ[w];
+    unsafeAtomicAdd(a.J, s);
+  }
+  const int n0 = t.node_ptr[blk];
+  for (int i = tid; i < nr; i += kThreads) {
+    const int ln = static_cast<int>(t.row_info[r0 + i] & 0xffffu);
+    const int32_t v = t.node_vertex[n0 + ln];
+    const int32_t kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + ln];
+    a.dJdk[kd] = sacc[i * (1 + D)];
+#pragma unroll
+    for (int q = 0; q < D; ++q) a.dJdx[static_cast<int64_t>(v) * D + q] = sacc[i * (1 + D) + 1 + q];
   }
 }
 
@@ -361,8 +529,7 @@ hipError_t launch_atomic_t(const P1AtomicTopo& t, const P1Pass& p, hipStream_t s
 template <int D, int AFORM, bool WANT_B, bool BC>
 hipError_t launch_blocked_t(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
   if (t.num_blocks == 0) return hipSuccess;
-  const LdsLayout L = lds_layout(D, t.max_nodes, t.max_rows, t.max_nnz,
-                                 AFORM == DFA_FORM_P1_POISSON, WANT_B, BC);
+  const LdsLayout L = lds_layout(D, t.max_nodes, t.max_rows, t.max_nnz, WANT_B, BC);
   auto kern = p1_blocked_kernel<D, AFORM, WANT_B, BC>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, L.total);
@@ -430,9 +597,36 @@ hipError_t launch_p1_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t 
 }
 
 size_t p1_blocked_lds_bytes(const P1BlockTopo& t, const P1Pass& p) {
-  return static_cast<size_t>(lds_layout(p.gdim, t.max_nodes, t.max_rows, t.max_nnz,
-                                        p.a_form == DFA_FORM_P1_POISSON, p.want_b, p.bc)
-                                 .total);
+  return static_cast<size_t>(lds_layout(p.gdim, t.max_nodes, t.max_rows, t.max_nnz, p.want_b, p.bc).total);
+}
+
+namespace {
+template <int D>
+hipError_t launch_adjoint_blocked_t(const P1BlockTopo& t, const AdjointP1& a, hipStream_t s) {
+  constexpr int NS = node_stride(D);
+  int o = 0;
+  const int node = o; o += align16(t.max_nodes * NS * 8);
+  const int f = o;    o += align16(t.max_nodes * 8);
+  const int u = o;    o += align16(t.max_nodes * 8);
+  const int own = o;  o += align16(t.max_nodes * 4);
+  const int acc = o;  o += align16(t.max_rows * (1 + D) * 8);
+  auto kern = p1_adjoint_blocked_kernel<D>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, o);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(t.num_blocks), dim3(kThreads), o, s, t, a, node, f, u, own, acc);
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_p1_adjoint_blocked(const P1BlockTopo& t, const AdjointP1& a, hipStream_t s) {
+  if (t.num_blocks == 0) return hipSuccess;
+  switch (a.gdim) {
+    case 1: return launch_adjoint_blocked_t<1>(t, a, s);
+    case 2: return launch_adjoint_blocked_t<2>(t, a, s);
+    case 3: return launch_adjoint_blocked_t<3>(t, a, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_fill_zero(double* ptr, int64_t n, hipStream_t s) {

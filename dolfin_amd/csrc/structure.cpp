@@ -138,6 +138,16 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   P.kdof_is_vertex = true;
   for (int64_t v = 0; v < nv; ++v)
     if (vk[v] >= 0 && vk[v] != v) P.kdof_is_vertex = false;
+  {
+    P.kdof_injective = true;
+    std::vector<int32_t> kv;
+    kv.reserve(nv);
+    for (int64_t v = 0; v < nv; ++v)
+      if (vk[v] >= 0) kv.push_back(vk[v]);
+    std::sort(kv.begin(), kv.end());
+    for (size_t i = 1; i < kv.size(); ++i)
+      if (kv[i] == kv[i - 1]) P.kdof_injective = false;
+  }
 
   // 2. RCB over the dofs that appear in cells
   std::vector<int32_t> ids;
@@ -212,6 +222,7 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   std::vector<std::vector<int32_t>> bnodes(nb);
   std::vector<std::vector<int32_t>> brows(nb);
   std::vector<std::vector<uint32_t>> binfo(nb);
+  std::vector<std::vector<uint64_t>> bruns(nb);
   P.cell_ids.resize(app.size());
   P.cell_rank.resize(app.size());
   P.blk_nnz.assign(nb, 0);
@@ -250,7 +261,50 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
         failed = 2;
         continue;
       }
+      {
+        const auto& bd = blk_dofs[b];
+        uint32_t loff = 0;
+        size_t i = 0;
+        while (i < bd.size()) {
+          size_t j = i + 1;
+          while (j < bd.size() && bd[j] == bd[j - 1] + 1) ++j;
+          const int32_t start = csr.row_ptr[bd[i]];
+          const uint32_t len = static_cast<uint32_t>(csr.row_ptr[bd[j - 1] + 1] - start);
+          bruns[b].push_back(static_cast<uint64_t>(static_cast<uint32_t>(start)) |
+                             (static_cast<uint64_t>(loff) << 32) | (static_cast<uint64_t>(len) << 48));
+          loff += len;
+          i = j;
+        }
+      }
       P.blk_nnz[b] = static_cast<int32_t>(off);
+      // Order the block's cells by a greedy colouring of "shares an owned row": cells of one
+      // colour never add to the same LDS entry, so a wavefront working through one colour
+      // issues conflict-free LDS atomics (the order does not change any sum's terms).
+      const int32_t kb0 = P.cell_ptr[b], kb1 = P.cell_ptr[b + 1];
+      {
+        std::vector<uint64_t> used(nodes.size(), 0);
+        std::vector<std::pair<int, int32_t>> col(kb1 - kb0);
+        for (int32_t k = kb0; k < kb1; ++k) {
+          const int64_t c = app[k];
+          uint64_t busy = 0;
+          uint32_t ln[4];
+          for (int a = 0; a < nvc; ++a) {
+            ln[a] = local(cells[c * nvc + a]);
+            if (row_off_of_node[ln[a]] >= 0) busy |= used[ln[a]];
+          }
+          int colour = 63;
+          for (int q = 0; q < 63; ++q)
+            if (!(busy >> q & 1)) { colour = q; break; }
+          for (int a = 0; a < nvc; ++a)
+            if (row_off_of_node[ln[a]] >= 0) used[ln[a]] |= (1ull << colour);
+          col[k - kb0] = {colour, static_cast<int32_t>(c)};
+        }
+        std::stable_sort(col.begin(), col.end(),
+                         [](const std::pair<int, int32_t>& x, const std::pair<int, int32_t>& y) {
+                           return x.first < y.first;
+                         });
+        for (int32_t k = kb0; k < kb1; ++k) app[k] = col[k - kb0].second;
+      }
       for (int32_t k = P.cell_ptr[b]; k < P.cell_ptr[b + 1]; ++k) {
         const int64_t c = app[k];
         uint64_t idw = 0, rk = 0;
@@ -282,6 +336,10 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
 
   P.node_ptr.assign(nb + 1, 0);
   P.row_ptr.assign(nb + 1, 0);
+  P.run_ptr.assign(nb + 1, 0);
+  for (int32_t b = 0; b < nb; ++b) P.run_ptr[b + 1] = P.run_ptr[b] + static_cast<int32_t>(bruns[b].size());
+  P.runs.resize(P.run_ptr[nb]);
+  for (int32_t b = 0; b < nb; ++b) std::copy(bruns[b].begin(), bruns[b].end(), P.runs.begin() + P.run_ptr[b]);
   for (int32_t b = 0; b < nb; ++b) {
     P.node_ptr[b + 1] = P.node_ptr[b] + static_cast<int32_t>(bnodes[b].size());
     P.row_ptr[b + 1] = P.row_ptr[b] + static_cast<int32_t>(brows[b].size());

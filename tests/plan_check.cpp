@@ -80,13 +80,18 @@ extern "C" int plan_check_p1(int32_t gdim, int64_t nv, int64_t nc, const int32_t
           }
         }
       }
-      for (int i = 0; i < nr; ++i) {
-        const int32_t dof = P.row_dof[r0 + i];
-        const int off = static_cast<int>(P.row_info[r0 + i] >> 16);
-        const int base = csr.row_ptr[dof], len = csr.row_ptr[dof + 1] - base;
-        if (off + len > nnzb) return -8;
-        for (int j = 0; j < len; ++j) values[base + j] = vals[off + j];
+      // write-out exactly as the kernel does it: by runs of consecutive rows
+      int covered_vals = 0;
+      for (int q = P.run_ptr[b]; q < P.run_ptr[b + 1]; ++q) {
+        const uint64_t run = P.runs[q];
+        const int64_t gstart = static_cast<int64_t>(static_cast<uint32_t>(run));
+        const int loff = static_cast<int>((run >> 32) & 0xffffu);
+        const int len = static_cast<int>(run >> 48);
+        if (loff + len > nnzb || gstart + len > static_cast<int64_t>(csr.col.size())) return -8;
+        for (int j = 0; j < len; ++j) values[gstart + j] = vals[loff + j];
+        covered_vals += len;
       }
+      if (covered_vals != nnzb) return -11;
     }
     for (int64_t d = 0; d < ndofs; ++d)
       if (covered[d] != 1) return -9;
