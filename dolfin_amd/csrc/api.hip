@@ -100,6 +100,13 @@ int guarded(F&& f) {
   }
 }
 
+int default_block_threads() {
+  if (const char* e = std::getenv("DFA_BLOCK_THREADS")) {
+    if (std::atoi(e) == 512) return 512;
+  }
+  return 256;
+}
+
 int default_block_rows(int gdim) {
   if (const char* e = std::getenv("DFA_BLOCK_ROWS")) {
     const int v = std::atoi(e);
@@ -234,6 +241,8 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
             t.max_nodes = P.max_nodes;
             t.max_rows = P.max_rows;
             t.max_nnz = P.max_nnz;
+            t.max_row_len = P.max_row_len;
+            t.threads = default_block_threads();
             t.kdof_is_vertex = P.kdof_is_vertex;
             t.node_ptr = h->plan.upload(P.node_ptr, s);
             t.node_vertex = h->plan.upload(P.node_vertex, s);

@@ -30,9 +30,10 @@ void build_csr(int64_t num_cells, int32_t nd, const int32_t* cell_dofs, int64_t 
 struct P1BlockPlan {
   int32_t nvc = 0;                      // vertices per cell (gdim + 1)
   int32_t num_blocks = 0;
-  int32_t max_nodes = 0, max_rows = 0, max_nnz = 0, max_cells = 0;
+  int32_t max_nodes = 0, max_rows = 0, max_nnz = 0, max_cells = 0, max_row_len = 0;
   bool kdof_is_vertex = false;          // coefficient dof == vertex id everywhere
   bool kdof_injective = false;          // distinct vertices carry distinct coefficient dofs
+  // Local nodes 0..(owned rows - 1) are the owned rows in order; ghosts follow.
   std::vector<int32_t> node_ptr;        // [nb+1]
   std::vector<int32_t> node_vertex;     // global vertex of each local node
   std::vector<int32_t> node_kdof;       // coefficient dof of each local node
@@ -60,5 +61,8 @@ void build_p1_block_plan(int32_t gdim, int64_t num_vertices, int64_t num_cells,
 constexpr int kMaxBlockNodes = 2048;
 constexpr int kMaxBlockNnz = 8192;
 constexpr int kMaxRank = 16;  // 4-bit column rank
+// LDS row stride (doubles) of the strided variant: rows of <= 16 entries, odd stride so that
+// equal ranks of different rows fall into different LDS banks.
+constexpr int kRowStride = 17;
 
 }  // namespace dfa

@@ -78,6 +78,16 @@ struct Fact {
   static constexpr double value = D == 1 ? 1.0 : (D == 2 ? 2.0 : 6.0);
 };
 
+// 1/d to full FP64 accuracy without the IEEE division sequence: hardware reciprocal
+// estimate refined by two Newton steps (d is a nonzero, finite |det J|).
+__device__ __forceinline__ double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
 // Element matrix (row-major, N x N) and vector of one P1 cell.
 template <int D, int AFORM, bool WANT_B>
 __device__ __forceinline__ void p1_element(const double (&p)[D + 1][D], const double (&kap)[D + 1],
@@ -88,32 +98,35 @@ __device__ __forceinline__ void p1_element(const double (&p)[D + 1][D], const do
   double absdet;
   p1_geometry<D>(p, g, absdet);
   if constexpr (AFORM == DFA_FORM_P1_POISSON) {
-    double ks = 0.0;
+    double ks = kap[0];
 #pragma unroll
-    for (int a = 0; a < N; ++a) ks += kap[a];
-    const double w = ks / (N * Fact<D>::value * absdet);
+    for (int a = 1; a < N; ++a) ks += kap[a];
+    constexpr double c = 1.0 / (N * Fact<D>::value);
+    const double w = ks * c * recip(absdet);
 #pragma unroll
     for (int a = 0; a < N; ++a)
 #pragma unroll
       for (int b = a; b < N; ++b) {
-        double s = 0.0;
+        double s = g[a][0] * g[b][0];
 #pragma unroll
-        for (int i = 0; i < D; ++i) s += g[a][i] * g[b][i];
+        for (int i = 1; i < D; ++i) s = fma(g[a][i], g[b][i], s);
         A[a][b] = w * s;
         A[b][a] = w * s;
       }
   } else if constexpr (AFORM == DFA_FORM_P1_MASS) {
-    const double w = absdet / (Fact<D>::value * (D + 1.0) * (D + 2.0));
+    constexpr double c = 1.0 / (Fact<D>::value * (D + 1.0) * (D + 2.0));
+    const double w = absdet * c;
 #pragma unroll
     for (int a = 0; a < N; ++a)
 #pragma unroll
       for (int b = 0; b < N; ++b) A[a][b] = (a == b) ? 2.0 * w : w;
   }
   if constexpr (WANT_B) {
-    double fs = 0.0;
+    double fs = fv[0];
 #pragma unroll
-    for (int a = 0; a < N; ++a) fs += fv[a];
-    const double w = absdet / (Fact<D>::value * (D + 1.0) * (D + 2.0));
+    for (int a = 1; a < N; ++a) fs += fv[a];
+    constexpr double c = 1.0 / (Fact<D>::value * (D + 1.0) * (D + 2.0));
+    const double w = absdet * c;
 #pragma unroll
     for (int a = 0; a < N; ++a) be[a] = w * (fv[a] + fs);
   }
@@ -209,16 +222,19 @@ __host__ __device__ inline int align16(int n) { return (n + 15) & ~15; }
 // is one or two 128-bit LDS reads.
 __host__ __device__ constexpr int node_stride(int D) { return D == 1 ? 2 : 4; }
 
+// LDS layout of one block. Strided: owned row i occupies values [i*kRowStride, +16), ranks
+// index it directly. Compact: rows packed back to back (any row length), located through a
+// per-node table.
 __host__ __device__ inline LdsLayout lds_layout(int D, int maxN, int maxR, int maxNnz, bool f,
-                                                bool bc) {
+                                                bool bc, bool strided) {
   LdsLayout L;
   int o = 0;
   L.node = o; o += align16(maxN * node_stride(D) * 8);
   L.f = o;    o += f ? align16(maxN * 8) : 0;
   L.bcv = o;  o += bc ? align16(maxN * 8) : 0;
   L.bcf = o;  o += bc ? align16(maxN) : 0;
-  L.own = o;  o += align16(maxN * 4);
-  L.vals = o; o += align16(maxNnz * 8);
+  L.own = o;  o += strided ? 0 : align16(maxN * 4);
+  L.vals = o; o += align16((strided ? maxR * kRowStride : maxNnz) * 8);
   L.bv = o;   o += f ? align16(maxR * 8) : 0;
   L.rdof = o; o += align16(maxR * 4);
   L.total = o;
@@ -254,7 +270,7 @@ __device__ __forceinline__ int xcd_block(int nb) {
 }
 
 // Prologue shared by the assembly and adjoint kernels: gather the block's nodes into LDS.
-template <int D, bool NEED_K, bool NEED_F, bool BC>
+template <int D, int T, bool NEED_K, bool NEED_F, bool BC, bool OWN>
 __device__ __forceinline__ void gather_nodes(const P1BlockTopo& t, const double* __restrict__ x,
                                              const double* __restrict__ kappa,
                                              const double* __restrict__ f,
@@ -263,7 +279,7 @@ __device__ __forceinline__ void gather_nodes(const P1BlockTopo& t, const double*
                                              double* sf, double* sbcv, uint8_t* sbcf, int32_t* sown) {
   constexpr int NS = node_stride(D);
   const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
-  for (int i = threadIdx.x; i < nn; i += kThreads) {
+  for (int i = threadIdx.x; i < nn; i += T) {
     const int32_t v = t.node_vertex[n0 + i];
     double rec[NS];
 #pragma unroll
@@ -275,24 +291,20 @@ __device__ __forceinline__ void gather_nodes(const P1BlockTopo& t, const double*
       if constexpr (NEED_K) rec[D] = kappa[kd];
       if constexpr (NEED_F) sf[i] = f[kd];
     }
-    if constexpr (NS == 2) {
-      *reinterpret_cast<double2*>(snode + i * NS) = make_double2(rec[0], rec[1]);
-    } else {
-      *reinterpret_cast<double2*>(snode + i * NS) = make_double2(rec[0], rec[1]);
-      *reinterpret_cast<double2*>(snode + i * NS + 2) = make_double2(rec[2], rec[3]);
-    }
+    *reinterpret_cast<double2*>(snode + i * NS) = make_double2(rec[0], rec[1]);
+    if constexpr (NS == 4) *reinterpret_cast<double2*>(snode + i * NS + 2) = make_double2(rec[2], rec[3]);
     if constexpr (BC) {
       const int32_t dd = t.node_dof[n0 + i];
       const uint8_t m = bcm[dd];
       sbcf[i] = m;
       sbcv[i] = m ? bcv[dd] : 0.0;
     }
-    sown[i] = -1;
+    if constexpr (OWN) sown[i] = -1;
   }
 }
 
-template <int D, int AFORM, bool WANT_B, bool BC>
-__global__ __launch_bounds__(kThreads) void p1_blocked_kernel(P1BlockTopo t, P1Pass p, LdsLayout L) {
+template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T>
+__global__ __launch_bounds__(T) void p1_blocked_kernel(P1BlockTopo t, P1Pass p, LdsLayout L) {
   constexpr int N = D + 1;
   constexpr int NS = node_stride(D);
   constexpr bool WANT_A = AFORM >= 0;
@@ -311,27 +323,40 @@ __global__ __launch_bounds__(kThreads) void p1_blocked_kernel(P1BlockTopo t, P1P
   const int tid = threadIdx.x;
   const bool do_A = WANT_A && p.want_A;
 
-  gather_nodes<D, NEED_K, WANT_B, BC>(t, p.x, p.kappa, p.f, p.bc_mask, p.bc_value, blk, snode, sf,
-                                      sbcv, sbcf, sown);
+  gather_nodes<D, T, NEED_K, WANT_B, BC, !STRIDED>(t, p.x, p.kappa, p.f, p.bc_mask, p.bc_value,
+                                                   blk, snode, sf, sbcv, sbcf, sown);
   const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
-  const int nnz = t.blk_nnz[blk];
-  if (do_A)
-    for (int i = tid; i < (nnz + 1) / 2; i += kThreads)
+  if (do_A) {
+    const int nz = STRIDED ? nr * kRowStride : t.blk_nnz[blk];
+    for (int i = tid; i < (nz + 1) / 2; i += T)
       reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
-  if constexpr (WANT_B)
-    for (int i = tid; i < nr; i += kThreads) sbv[i] = 0.0;
-  __syncthreads();
-  for (int i = tid; i < nr; i += kThreads) {
-    const uint32_t info = t.row_info[r0 + i];
-    sown[info & 0xffffu] = static_cast<int>(info >> 16) | (i << 16);
+  }
+  for (int i = tid; i < nr; i += T) {
     srdof[i] = t.row_dof[r0 + i];
+    if constexpr (WANT_B) sbv[i] = 0.0;
+  }
+  if constexpr (!STRIDED) {
+    __syncthreads();
+    for (int i = tid; i < nr; i += T) {
+      const uint32_t info = t.row_info[r0 + i];
+      sown[info & 0xffffu] = static_cast<int>(info >> 16) | (i << 16);
+    }
   }
   __syncthreads();
 
   const int c0 = t.cell_ptr[blk], ncb = t.cell_ptr[blk + 1] - c0;
-  for (int k = tid; k < ncb; k += kThreads) {
-    const uint64_t ids = t.cell_ids[c0 + k];
-    const uint64_t rk = do_A ? t.cell_rank[c0 + k] : 0;
+  // cell records are prefetched one iteration ahead
+  uint64_t ids_n = 0, rk_n = 0;
+  if (tid < ncb) {
+    ids_n = t.cell_ids[c0 + tid];
+    if (do_A) rk_n = t.cell_rank[c0 + tid];
+  }
+  for (int k = tid; k < ncb; k += T) {
+    const uint64_t ids = ids_n, rk = rk_n;
+    if (k + T < ncb) {
+      ids_n = t.cell_ids[c0 + k + T];
+      if (do_A) rk_n = t.cell_rank[c0 + k + T];
+    }
     int l[N];
     double pc[N][D], kap[N], fv[N], A[N][N], be[N];
 #pragma unroll
@@ -354,38 +379,62 @@ __global__ __launch_bounds__(kThreads) void p1_blocked_kernel(P1BlockTopo t, P1P
     }
 #pragma unroll
     for (int a = 0; a < N; ++a) {
-      const int own = sown[l[a]];
-      if (own < 0) continue;
+      int off, row;
+      if constexpr (STRIDED) {
+        if (l[a] >= nr) continue;
+        row = l[a];
+        off = l[a] * kRowStride;
+      } else {
+        const int own = sown[l[a]];
+        if (own < 0) continue;
+        off = own & 0xffff;
+        row = own >> 16;
+      }
       if (do_A) {
-        const int off = own & 0xffff;
         const uint32_t ra = static_cast<uint32_t>(rk >> (16 * a));
 #pragma unroll
         for (int b = 0; b < N; ++b)
           atomic_add_lds(svals + off + static_cast<int>((ra >> (4 * b)) & 15u), A[a][b]);
       }
-      if constexpr (WANT_B) atomic_add_lds(sbv + (own >> 16), be[a]);
+      if constexpr (WANT_B) atomic_add_lds(sbv + row, be[a]);
     }
   }
   __syncthreads();
 
-  // Write-out by runs of consecutive rows: each run is one contiguous CSR range.
   if (do_A) {
-    const int q0 = t.run_ptr[blk], nq = t.run_ptr[blk + 1] - q0;
-    const int wave = tid >> 6, lane = tid & 63;
-    for (int q = wave; q < nq; q += kThreads / 64) {
-      const uint64_t run = t.runs[q0 + q];
-      const int64_t gstart = static_cast<int64_t>(static_cast<uint32_t>(run));
-      const int loff = static_cast<int>((run >> 32) & 0xffffu);
-      const int len = static_cast<int>(run >> 48);
-      for (int j = lane; j < len; j += 64) {
-        double v = svals[loff + j];
-        if (p.add) v += p.values[gstart + j];
-        p.values[gstart + j] = v;
+    if constexpr (STRIDED) {
+      // 16 lanes per row; consecutive rows are consecutive dofs, so a wave stores 4 adjacent
+      // CSR rows.
+      const int grp = tid >> 4, ln = tid & 15;
+      for (int i = grp; i < nr; i += T / 16) {
+        const int32_t dof = srdof[i];
+        const int base = t.csr_row_ptr[dof];
+        const int len = t.csr_row_ptr[dof + 1] - base;
+        if (ln < len) {
+          double v = svals[i * kRowStride + ln];
+          if (p.add) v += p.values[base + ln];
+          p.values[base + ln] = v;
+        }
+      }
+    } else {
+      // runs of consecutive rows: each run is one contiguous CSR range
+      const int q0 = t.run_ptr[blk], nq = t.run_ptr[blk + 1] - q0;
+      const int wave = tid >> 6, lane = tid & 63;
+      for (int q = wave; q < nq; q += T / 64) {
+        const uint64_t run = t.runs[q0 + q];
+        const int64_t gstart = static_cast<int64_t>(static_cast<uint32_t>(run));
+        const int loff = static_cast<int>((run >> 32) & 0xffffu);
+        const int len = static_cast<int>(run >> 48);
+        for (int jj = lane; jj < len; jj += 64) {
+          double v = svals[loff + jj];
+          if (p.add) v += p.values[gstart + jj];
+          p.values[gstart + jj] = v;
+        }
       }
     }
   }
   if constexpr (WANT_B)
-    for (int i = tid; i < nr; i += kThreads) {
+    for (int i = tid; i < nr; i += T) {
       const int32_t dof = srdof[i];
       p.b[dof] = p.add ? p.b[dof] + sbv[i] : sbv[i];
     }
@@ -405,22 +454,19 @@ __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTop
   double* snode = reinterpret_cast<double*>(lds + lds_node);
   double* sf = reinterpret_cast<double*>(lds + lds_f);
   double* su = reinterpret_cast<double*>(lds + lds_u);
-  int32_t* sown = reinterpret_cast<int32_t*>(lds + lds_own);
   double* sacc = reinterpret_cast<double*>(lds + lds_acc);  // [rows][1 + D]
   __shared__ double sJ[kThreads / 64];
 
   const int blk = xcd_block(t.num_blocks);
   const int tid = threadIdx.x;
-  gather_nodes<D, true, true, false>(t, a.x, a.kappa, a.f, nullptr, nullptr, blk, snode, sf,
-                                     nullptr, nullptr, sown);
+  gather_nodes<D, kThreads, true, true, false, false>(t, a.x, a.kappa, a.f, nullptr, nullptr, blk,
+                                                      snode, sf, nullptr, nullptr, nullptr);
   {
     const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
     for (int i = tid; i < nn; i += kThreads) su[i] = a.u[t.node_dof[n0 + i]];
   }
   const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
   for (int i = tid; i < nr * (1 + D); i += kThreads) sacc[i] = 0.0;
-  __syncthreads();
-  for (int i = tid; i < nr; i += kThreads) sown[t.row_info[r0 + i] & 0xffffu] = i;
   __syncthreads();
 
   double Jl = 0.0;
@@ -474,11 +520,11 @@ __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTop
     for (int i = 0; i < D; ++i) g2 += gu[i] * gu[i];
     const double E = kb * vol * g2;
     const double Gl = vol / ((D + 1.0) * (D + 2.0)) * (uf + us * fs);
-    if (sown[l[0]] >= 0) Jl += E - Gl;
+    if (l[0] < nr) Jl += E - Gl;  // local nodes below nr are the owned rows
 #pragma unroll
     for (int q = 0; q < N; ++q) {
-      const int row = sown[l[q]];
-      if (row < 0) continue;
+      const int row = l[q];
+      if (row >= nr) continue;
       double* acc = sacc + row * (1 + D);
       atomic_add_lds(acc, vol * g2 / N);
       double gdl = 0.0;
@@ -502,9 +548,8 @@ __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTop
   }
   const int n0 = t.node_ptr[blk];
   for (int i = tid; i < nr; i += kThreads) {
-    const int ln = static_cast<int>(t.row_info[r0 + i] & 0xffffu);
-    const int32_t v = t.node_vertex[n0 + ln];
-    const int32_t kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + ln];
+    const int32_t v = t.node_vertex[n0 + i];
+    const int32_t kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + i];
     a.dJdk[kd] = sacc[i * (1 + D)];
 #pragma unroll
     for (int q = 0; q < D; ++q) a.dJdx[static_cast<int64_t>(v) * D + q] = sacc[i * (1 + D) + 1 + q];
@@ -526,16 +571,26 @@ hipError_t launch_atomic_t(const P1AtomicTopo& t, const P1Pass& p, hipStream_t s
   return hipGetLastError();
 }
 
-template <int D, int AFORM, bool WANT_B, bool BC>
-hipError_t launch_blocked_t(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
-  if (t.num_blocks == 0) return hipSuccess;
-  const LdsLayout L = lds_layout(D, t.max_nodes, t.max_rows, t.max_nnz, WANT_B, BC);
-  auto kern = p1_blocked_kernel<D, AFORM, WANT_B, BC>;
+template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T>
+hipError_t launch_blocked_tt(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
+  const LdsLayout L = lds_layout(D, t.max_nodes, t.max_rows, t.max_nnz, WANT_B, BC, STRIDED);
+  auto kern = p1_blocked_kernel<D, AFORM, WANT_B, BC, STRIDED, T>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, L.total);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(t.num_blocks), dim3(kThreads), L.total, s, t, p, L);
+  hipLaunchKernelGGL(kern, dim3(t.num_blocks), dim3(T), L.total, s, t, p, L);
   return hipGetLastError();
+}
+
+template <int D, int AFORM, bool WANT_B, bool BC>
+hipError_t launch_blocked_t(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
+  if (t.num_blocks == 0) return hipSuccess;
+  const bool strided = t.max_row_len <= kMaxRank;
+  if (strided) {
+    if (t.threads == 512) return launch_blocked_tt<D, AFORM, WANT_B, BC, true, 512>(t, p, s);
+    return launch_blocked_tt<D, AFORM, WANT_B, BC, true, 256>(t, p, s);
+  }
+  return launch_blocked_tt<D, AFORM, WANT_B, BC, false, 256>(t, p, s);
 }
 
 template <int D>
@@ -597,7 +652,8 @@ hipError_t launch_p1_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t 
 }
 
 size_t p1_blocked_lds_bytes(const P1BlockTopo& t, const P1Pass& p) {
-  return static_cast<size_t>(lds_layout(p.gdim, t.max_nodes, t.max_rows, t.max_nnz, p.want_b, p.bc).total);
+  return static_cast<size_t>(
+      lds_layout(p.gdim, t.max_nodes, t.max_rows, t.max_nnz, p.want_b, p.bc, t.max_row_len <= kMaxRank).total);
 }
 
 namespace {
@@ -608,7 +664,7 @@ hipError_t launch_adjoint_blocked_t(const P1BlockTopo& t, const AdjointP1& a, hi
   const int node = o; o += align16(t.max_nodes * NS * 8);
   const int f = o;    o += align16(t.max_nodes * 8);
   const int u = o;    o += align16(t.max_nodes * 8);
-  const int own = o;  o += align16(t.max_nodes * 4);
+  const int own = o;
   const int acc = o;  o += align16(t.max_rows * (1 + D) * 8);
   auto kern = p1_adjoint_blocked_kernel<D>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),

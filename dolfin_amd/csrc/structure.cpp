@@ -235,13 +235,22 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   }
   parallel_for(nb, [&](int64_t b0, int64_t b1, int) {
     for (int64_t b = b0; b < b1; ++b) {
+      // Local node numbering: the owned rows first, in ascending dof order (local node i is
+      // owned row i), then the ghost vertices in ascending vertex order.
       auto& nodes = bnodes[b];
+      std::vector<int32_t> all;
       for (int32_t k = P.cell_ptr[b]; k < P.cell_ptr[b + 1]; ++k)
-        for (int a = 0; a < nvc; ++a) nodes.push_back(cells[static_cast<int64_t>(app[k]) * nvc + a]);
-      std::sort(nodes.begin(), nodes.end());
-      nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+        for (int a = 0; a < nvc; ++a) all.push_back(cells[static_cast<int64_t>(app[k]) * nvc + a]);
+      std::sort(all.begin(), all.end());
+      all.erase(std::unique(all.begin(), all.end()), all.end());
+      for (int32_t d : blk_dofs[b]) nodes.push_back(dofv[d]);
+      for (int32_t v : all)
+        if (owner[vdof[v]] != static_cast<int32_t>(b)) nodes.push_back(v);
+      std::vector<std::pair<int32_t, uint32_t>> vmap(nodes.size());
+      for (size_t i = 0; i < nodes.size(); ++i) vmap[i] = {nodes[i], static_cast<uint32_t>(i)};
+      std::sort(vmap.begin(), vmap.end());
       auto local = [&](int32_t v) {
-        return static_cast<uint32_t>(std::lower_bound(nodes.begin(), nodes.end(), v) - nodes.begin());
+        return std::lower_bound(vmap.begin(), vmap.end(), std::make_pair(v, 0u))->second;
       };
       if (nodes.size() > static_cast<size_t>(kMaxBlockNodes)) {
         failed = 1;
@@ -348,6 +357,8 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
     P.max_nnz = std::max(P.max_nnz, P.blk_nnz[b]);
     P.max_cells = std::max(P.max_cells, P.cell_ptr[b + 1] - P.cell_ptr[b]);
   }
+  for (size_t r = 0; r + 1 < csr.row_ptr.size(); ++r)
+    P.max_row_len = std::max(P.max_row_len, csr.row_ptr[r + 1] - csr.row_ptr[r]);
   P.node_vertex.resize(P.node_ptr[nb]);
   P.node_kdof.resize(P.node_ptr[nb]);
   P.node_dof.resize(P.node_ptr[nb]);

@@ -41,6 +41,7 @@ extern "C" int plan_check_p1(int32_t gdim, int64_t nv, int64_t nc, const int32_t
         const uint32_t info = P.row_info[r0 + i];
         const int ln = static_cast<int>(info & 0xffffu), off = static_cast<int>(info >> 16);
         if (ln >= nn || off >= nnzb) return -4;
+        if (ln != i) return -12;  // owned rows are local nodes 0..nr-1 in order
         if (P.node_dof[n0 + ln] != P.row_dof[r0 + i]) return -5;
         own[ln] = off;
         covered[P.row_dof[r0 + i]]++;
@@ -72,9 +73,14 @@ extern "C" int plan_check_p1(int32_t gdim, int64_t nv, int64_t nc, const int32_t
         }
         const double w = std::fabs(det) / (fact * (gdim + 1.0) * (gdim + 2.0));
         for (int a = 0; a < nvc; ++a) {
+          if ((own[l[a]] >= 0) != (l[a] < nr)) return -13;  // strided kernel's ownership test
           if (own[l[a]] < 0) continue;
+          const int32_t rdof = P.row_dof[r0 + l[a]];
+          const int rlen = csr.row_ptr[rdof + 1] - csr.row_ptr[rdof];
           for (int bb = 0; bb < nvc; ++bb) {
-            const int slot = own[l[a]] + static_cast<int>((rk >> (4 * (a * 4 + bb))) & 15u);
+            const int rank = static_cast<int>((rk >> (4 * (a * 4 + bb))) & 15u);
+            if (rank >= rlen || (rlen <= 16 && l[a] * 17 + rank >= nr * 17)) return -14;
+            const int slot = own[l[a]] + rank;
             if (slot >= nnzb) return -7;
             vals[slot] += (a == bb) ? 2.0 * w : w;
           }
