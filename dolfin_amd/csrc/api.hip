@@ -102,9 +102,10 @@ int guarded(F&& f) {
 
 int default_block_threads() {
   if (const char* e = std::getenv("DFA_BLOCK_THREADS")) {
-    if (std::atoi(e) == 512) return 512;
+    const int v = std::atoi(e);
+    if (v == 256 || v == 512) return v;
   }
-  return 256;
+  return 512;
 }
 
 int default_block_rows(int gdim) {
@@ -112,7 +113,7 @@ int default_block_rows(int gdim) {
     const int v = std::atoi(e);
     if (v >= 8 && v <= 4096) return v;
   }
-  return gdim == 3 ? 256 : 512;
+  return gdim == 3 ? 192 : 512;
 }
 
 // Vector P2 rank table. Nodes are identified by the cell-local node index a (0..9) of the
@@ -243,6 +244,7 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
             t.max_nnz = P.max_nnz;
             t.max_row_len = P.max_row_len;
             t.threads = default_block_threads();
+            if (const char* e = std::getenv("DFA_NODE_RECORD")) t.wide_nodes = std::atoi(e) != 32;
             t.kdof_is_vertex = P.kdof_is_vertex;
             t.node_ptr = h->plan.upload(P.node_ptr, s);
             t.node_vertex = h->plan.upload(P.node_vertex, s);

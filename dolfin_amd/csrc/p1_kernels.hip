@@ -218,20 +218,24 @@ struct LdsLayout {
 
 __host__ __device__ inline int align16(int n) { return (n + 15) & ~15; }
 
-// Node record: coordinates and kappa packed in 32 bytes (16 for intervals) so one cell vertex
-// is one or two 128-bit LDS reads.
-__host__ __device__ constexpr int node_stride(int D) { return D == 1 ? 2 : 4; }
+// Node record in LDS, read with 128-bit loads. Small: coordinates and kappa (32 bytes for
+// triangles/tets; f and the Dirichlet value in separate arrays). Wide: coordinates, kappa, f,
+// Dirichlet value (48 bytes): a 12-dword stride starts consecutive records on 8 different
+// 4-bank groups, while a 32-byte stride reuses only 4 of them.
+__host__ __device__ constexpr int node_stride(int D, bool wide = false) {
+  return wide ? (D == 1 ? 4 : 6) : (D == 1 ? 2 : 4);
+}
 
 // LDS layout of one block. Strided: owned row i occupies values [i*kRowStride, +16), ranks
 // index it directly. Compact: rows packed back to back (any row length), located through a
 // per-node table.
 __host__ __device__ inline LdsLayout lds_layout(int D, int maxN, int maxR, int maxNnz, bool f,
-                                                bool bc, bool strided) {
+                                                bool bc, bool strided, bool wide) {
   LdsLayout L;
   int o = 0;
-  L.node = o; o += align16(maxN * node_stride(D) * 8);
-  L.f = o;    o += f ? align16(maxN * 8) : 0;
-  L.bcv = o;  o += bc ? align16(maxN * 8) : 0;
+  L.node = o; o += align16(maxN * node_stride(D, wide) * 8);
+  L.f = o;    o += (f && !wide) ? align16(maxN * 8) : 0;
+  L.bcv = o;  o += (bc && !wide) ? align16(maxN * 8) : 0;
   L.bcf = o;  o += bc ? align16(maxN) : 0;
   L.own = o;  o += strided ? 0 : align16(maxN * 4);
   L.vals = o; o += align16((strided ? maxR * kRowStride : maxNnz) * 8);
@@ -270,14 +274,14 @@ __device__ __forceinline__ int xcd_block(int nb) {
 }
 
 // Prologue shared by the assembly and adjoint kernels: gather the block's nodes into LDS.
-template <int D, int T, bool NEED_K, bool NEED_F, bool BC, bool OWN>
+template <int D, int T, bool NEED_K, bool NEED_F, bool BC, bool OWN, bool WIDE>
 __device__ __forceinline__ void gather_nodes(const P1BlockTopo& t, const double* __restrict__ x,
                                              const double* __restrict__ kappa,
                                              const double* __restrict__ f,
                                              const uint8_t* __restrict__ bcm,
                                              const double* __restrict__ bcv, int blk, double* snode,
                                              double* sf, double* sbcv, uint8_t* sbcf, int32_t* sown) {
-  constexpr int NS = node_stride(D);
+  constexpr int NS = node_stride(D, WIDE);
   const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
   for (int i = threadIdx.x; i < nn; i += T) {
     const int32_t v = t.node_vertex[n0 + i];
@@ -289,24 +293,40 @@ __device__ __forceinline__ void gather_nodes(const P1BlockTopo& t, const double*
     if constexpr (NEED_K || NEED_F) {
       const int32_t kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + i];
       if constexpr (NEED_K) rec[D] = kappa[kd];
-      if constexpr (NEED_F) sf[i] = f[kd];
+      if constexpr (NEED_F) {
+        if constexpr (WIDE) rec[D + 1] = f[kd];
+        else sf[i] = f[kd];
+      }
     }
-    *reinterpret_cast<double2*>(snode + i * NS) = make_double2(rec[0], rec[1]);
-    if constexpr (NS == 4) *reinterpret_cast<double2*>(snode + i * NS + 2) = make_double2(rec[2], rec[3]);
     if constexpr (BC) {
       const int32_t dd = t.node_dof[n0 + i];
       const uint8_t m = bcm[dd];
       sbcf[i] = m;
-      sbcv[i] = m ? bcv[dd] : 0.0;
+      const double g = m ? bcv[dd] : 0.0;
+      if constexpr (WIDE) rec[D + 2] = g;
+      else sbcv[i] = g;
     }
+#pragma unroll
+    for (int k = 0; k < NS; k += 2)
+      *reinterpret_cast<double2*>(snode + i * NS + k) = make_double2(rec[k], rec[k + 1]);
     if constexpr (OWN) sown[i] = -1;
   }
 }
 
-template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T>
+template <int NS>
+__device__ __forceinline__ void load_rec(const double* __restrict__ p, double (&rec)[NS]) {
+#pragma unroll
+  for (int k = 0; k < NS; k += 2) {
+    const double2 r = *reinterpret_cast<const double2*>(p + k);
+    rec[k] = r.x;
+    rec[k + 1] = r.y;
+  }
+}
+
+template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE>
 __global__ __launch_bounds__(T) void p1_blocked_kernel(P1BlockTopo t, P1Pass p, LdsLayout L) {
   constexpr int N = D + 1;
-  constexpr int NS = node_stride(D);
+  constexpr int NS = node_stride(D, WIDE);
   constexpr bool WANT_A = AFORM >= 0;
   constexpr bool NEED_K = AFORM == DFA_FORM_P1_POISSON;
   extern __shared__ __align__(16) unsigned char lds[];
@@ -323,8 +343,9 @@ __global__ __launch_bounds__(T) void p1_blocked_kernel(P1BlockTopo t, P1Pass p, 
   const int tid = threadIdx.x;
   const bool do_A = WANT_A && p.want_A;
 
-  gather_nodes<D, T, NEED_K, WANT_B, BC, !STRIDED>(t, p.x, p.kappa, p.f, p.bc_mask, p.bc_value,
-                                                   blk, snode, sf, sbcv, sbcf, sown);
+  gather_nodes<D, T, NEED_K, WANT_B, BC, !STRIDED, WIDE>(t, p.x, p.kappa, p.f, p.bc_mask,
+                                                         p.bc_value, blk, snode, sf, sbcv, sbcf,
+                                                         sown);
   const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
   if (do_A) {
     const int nz = STRIDED ? nr * kRowStride : t.blk_nnz[blk];
@@ -359,22 +380,30 @@ __global__ __launch_bounds__(T) void p1_blocked_kernel(P1BlockTopo t, P1Pass p, 
     }
     int l[N];
     double pc[N][D], kap[N], fv[N], A[N][N], be[N];
+    double gv[N];
 #pragma unroll
     for (int a = 0; a < N; ++a) {
       l[a] = static_cast<int>((ids >> (16 * a)) & 0xffffu);
-      load_node<D>(snode + l[a] * NS, pc[a], kap[a]);
-      fv[a] = WANT_B ? sf[l[a]] : 0.0;
+      if constexpr (WIDE) {
+        double rec[NS];
+        load_rec<NS>(snode + l[a] * NS, rec);
+#pragma unroll
+        for (int i = 0; i < D; ++i) pc[a][i] = rec[i];
+        kap[a] = rec[D];
+        fv[a] = WANT_B ? rec[D + 1] : 0.0;
+        gv[a] = BC ? rec[D + 2] : 0.0;
+      } else {
+        load_node<D>(snode + l[a] * NS, pc[a], kap[a]);
+        fv[a] = WANT_B ? sf[l[a]] : 0.0;
+        gv[a] = BC ? sbcv[l[a]] : 0.0;
+      }
       be[a] = 0.0;
     }
     p1_element<D, AFORM, WANT_B>(pc, kap, fv, A, be);
     if constexpr (BC) {
       bool isbc[N];
-      double gv[N];
 #pragma unroll
-      for (int a = 0; a < N; ++a) {
-        isbc[a] = sbcf[l[a]] != 0;
-        gv[a] = sbcv[l[a]];
-      }
+      for (int a = 0; a < N; ++a) isbc[a] = sbcf[l[a]] != 0;
       apply_bc<N>(A, be, isbc, gv);
     }
 #pragma unroll
@@ -459,8 +488,9 @@ __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTop
 
   const int blk = xcd_block(t.num_blocks);
   const int tid = threadIdx.x;
-  gather_nodes<D, kThreads, true, true, false, false>(t, a.x, a.kappa, a.f, nullptr, nullptr, blk,
-                                                      snode, sf, nullptr, nullptr, nullptr);
+  gather_nodes<D, kThreads, true, true, false, false, false>(t, a.x, a.kappa, a.f, nullptr,
+                                                             nullptr, blk, snode, sf, nullptr,
+                                                             nullptr, nullptr);
   {
     const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
     for (int i = tid; i < nn; i += kThreads) su[i] = a.u[t.node_dof[n0 + i]];
@@ -571,10 +601,10 @@ hipError_t launch_atomic_t(const P1AtomicTopo& t, const P1Pass& p, hipStream_t s
   return hipGetLastError();
 }
 
-template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T>
+template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE>
 hipError_t launch_blocked_tt(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
-  const LdsLayout L = lds_layout(D, t.max_nodes, t.max_rows, t.max_nnz, WANT_B, BC, STRIDED);
-  auto kern = p1_blocked_kernel<D, AFORM, WANT_B, BC, STRIDED, T>;
+  const LdsLayout L = lds_layout(D, t.max_nodes, t.max_rows, t.max_nnz, WANT_B, BC, STRIDED, WIDE);
+  auto kern = p1_blocked_kernel<D, AFORM, WANT_B, BC, STRIDED, T, WIDE>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, L.total);
   if (e != hipSuccess) return e;
@@ -587,10 +617,14 @@ hipError_t launch_blocked_t(const P1BlockTopo& t, const P1Pass& p, hipStream_t s
   if (t.num_blocks == 0) return hipSuccess;
   const bool strided = t.max_row_len <= kMaxRank;
   if (strided) {
-    if (t.threads == 512) return launch_blocked_tt<D, AFORM, WANT_B, BC, true, 512>(t, p, s);
-    return launch_blocked_tt<D, AFORM, WANT_B, BC, true, 256>(t, p, s);
+    if (t.wide_nodes) {
+      if (t.threads == 512) return launch_blocked_tt<D, AFORM, WANT_B, BC, true, 512, true>(t, p, s);
+      return launch_blocked_tt<D, AFORM, WANT_B, BC, true, 256, true>(t, p, s);
+    }
+    if (t.threads == 512) return launch_blocked_tt<D, AFORM, WANT_B, BC, true, 512, false>(t, p, s);
+    return launch_blocked_tt<D, AFORM, WANT_B, BC, true, 256, false>(t, p, s);
   }
-  return launch_blocked_tt<D, AFORM, WANT_B, BC, false, 256>(t, p, s);
+  return launch_blocked_tt<D, AFORM, WANT_B, BC, false, 256, false>(t, p, s);
 }
 
 template <int D>
@@ -653,7 +687,8 @@ hipError_t launch_p1_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t 
 
 size_t p1_blocked_lds_bytes(const P1BlockTopo& t, const P1Pass& p) {
   return static_cast<size_t>(
-      lds_layout(p.gdim, t.max_nodes, t.max_rows, t.max_nnz, p.want_b, p.bc, t.max_row_len <= kMaxRank).total);
+      lds_layout(p.gdim, t.max_nodes, t.max_rows, t.max_nnz, p.want_b, p.bc,
+                 t.max_row_len <= kMaxRank, t.max_row_len <= kMaxRank && t.wide_nodes).total);
 }
 
 namespace {
