@@ -77,6 +77,8 @@ struct dfa_assembler_s {
   uint8_t* d_vp2_rank = nullptr;
   int32_t vp2_s1 = 0;
   bool vp2_component_major = true;
+  bool has_vp2_plan = false;
+  dfa::VP2BlockTopo vtopo;
 };
 
 namespace {
@@ -108,72 +110,20 @@ int default_block_threads() {
   return 512;
 }
 
+int default_p2_block_vals() {
+  if (const char* e = std::getenv("DFA_P2_BLOCK_VALS")) {
+    const int v = std::atoi(e);
+    if (v >= 1024 && v <= 16384) return v;
+  }
+  return 6144;
+}
+
 int default_block_rows(int gdim) {
   if (const char* e = std::getenv("DFA_BLOCK_ROWS")) {
     const int v = std::atoi(e);
     if (v >= 8 && v <= 4096) return v;
   }
   return gdim == 3 ? 192 : 512;
-}
-
-// Vector P2 rank table. Nodes are identified by the cell-local node index a (0..9) of the
-// component-0 dof; the CSR row of dof (i, a) lists, for each column component j, the
-// neighbour nodes in ascending order, either component-major (UFC numbering, stride s1 = 1,
-// component stride = deg) or node-major (block-interleaved numbering, s1 = 3, stride 1).
-void build_vp2_ranks(dfa_assembler_s* h, const int32_t* cd, const dfa::HostCSR& csr,
-                     std::vector<uint8_t>& rank) {
-  const int64_t nc = h->nc;
-  // Node id of a dof: dof of component 0 at the same cell-local node.
-  std::vector<int32_t> node0(h->ndofs, -1), comp(h->ndofs, -1);
-  for (int64_t c = 0; c < nc; ++c)
-    for (int i = 0; i < 3; ++i)
-      for (int a = 0; a < 10; ++a) {
-        const int32_t d = cd[c * 30 + i * 10 + a];
-        const int32_t n0 = cd[c * 30 + a];
-        if ((node0[d] >= 0 && node0[d] != n0) || (comp[d] >= 0 && comp[d] != i))
-          throw std::runtime_error("vector P2 dofmap is not node-blocked");
-        node0[d] = n0;
-        comp[d] = i;
-      }
-  // Determine the layout from the first row with more than one neighbour.
-  int layout = 0;  // 1 = component-major, 2 = node-major
-  for (int64_t r = 0; r < h->ndofs && !layout; ++r) {
-    const int32_t b = csr.row_ptr[r], e = csr.row_ptr[r + 1];
-    if (e - b < 6) continue;
-    layout = (comp[csr.col[b]] == comp[csr.col[b + 1]]) ? 1 : 2;
-  }
-  if (!layout) layout = 1;
-  h->vp2_component_major = (layout == 1);
-  rank.assign(static_cast<size_t>(nc) * 100, 0);
-  bool bad = false;
-  for (int64_t c = 0; c < nc && !bad; ++c)
-    for (int a = 0; a < 10; ++a) {
-      const int32_t ra = cd[c * 30 + a];  // component-0 row of node a
-      const int32_t rb0 = csr.row_ptr[ra], re0 = csr.row_ptr[ra + 1];
-      const int32_t len = re0 - rb0;
-      if (len % 3) { bad = true; break; }
-      const int32_t deg = len / 3;
-      for (int b = 0; b < 10; ++b) {
-        // locate column (component 0, node b) in row ra
-        const int32_t colb = cd[c * 30 + b];
-        const int32_t* p = std::lower_bound(csr.col.data() + rb0, csr.col.data() + re0, colb);
-        if (p == csr.col.data() + re0 || *p != colb) { bad = true; break; }
-        const int64_t pos = p - (csr.col.data() + rb0);
-        const int64_t rk = (layout == 1) ? pos : pos / 3;
-        if (rk > 255 || rk >= deg) { bad = true; break; }
-        rank[c * 100 + a * 10 + b] = static_cast<uint8_t>(rk);
-        // verify all 9 component combinations land on the expected columns
-        for (int i = 0; i < 3 && !bad; ++i) {
-          const int32_t row = cd[c * 30 + i * 10 + a];
-          if (csr.row_ptr[row + 1] - csr.row_ptr[row] != len) { bad = true; break; }
-          for (int j = 0; j < 3; ++j) {
-            const int64_t slot = csr.row_ptr[row] + ((layout == 1) ? (rk + j * deg) : (rk * 3 + j));
-            if (csr.col[slot] != cd[c * 30 + j * 10 + b]) { bad = true; break; }
-          }
-        }
-      }
-    }
-  if (bad) throw std::runtime_error("vector P2 sparsity is not node-blocked (unsupported dofmap)");
 }
 
 }  // namespace
@@ -275,8 +225,41 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
         }
       } else {
         std::vector<uint8_t> rank;
-        build_vp2_ranks(h, h_cell_dofs, csr, rank);
+        h->vp2_component_major = dfa::build_vp2_ranks(nc, h_cell_dofs, ndofs, csr, rank);
         h->d_vp2_rank = h->base.upload(rank, s);
+        if (h_x) {
+          try {
+            dfa::VP2BlockPlan P;
+            dfa::build_vp2_block_plan(nc, h_cells, h_cell_dofs, h_coeff_dofs, h_x, csr, rank,
+                                      h->vp2_component_major, default_p2_block_vals(), P);
+            auto& t = h->vtopo;
+            t.num_blocks = P.num_blocks;
+            t.max_verts = P.max_verts;
+            t.max_owned = P.max_owned;
+            t.max_vals = P.max_vals;
+            t.component_major = P.component_major;
+            t.vert_ptr = h->plan.upload(P.vert_ptr, s);
+            t.vert_id = h->plan.upload(P.vert_id, s);
+            t.vert_kdof = h->plan.upload(P.vert_kdof, s);
+            t.own_ptr = h->plan.upload(P.own_ptr, s);
+            t.own_rows = h->plan.upload(P.own_rows, s);
+            t.own_info = h->plan.upload(P.own_info, s);
+            t.task_ptr = h->plan.upload(P.task_ptr, s);
+            t.task = h->plan.upload(P.task, s);
+            t.task_cell = h->plan.upload(P.task_cell, s);
+            t.blk_vals = h->plan.upload(P.blk_vals, s);
+            t.row_ptr = h->d_row_ptr;
+            t.cell_dofs = h->d_cdofs;
+            h->appearances = static_cast<int64_t>(P.task_cell.size());
+            h->has_vp2_plan = true;
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+          } catch (const dfa::HipError&) {
+            throw;
+          } catch (const std::exception& e) {
+            h->plan.release();
+            h->plan_error = e.what();
+          }
+        }
       }
       hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     } catch (...) {
@@ -328,8 +311,12 @@ static int resolve_scatter(dfa_assembler h, int32_t scatter) {
       throw std::runtime_error("unknown scatter strategy");
     return scatter;
   }
-  if (scatter == DFA_SCATTER_AUTO || scatter == DFA_SCATTER_ATOMIC) return DFA_SCATTER_ATOMIC;
-  throw std::runtime_error("vector P2 supports the atomic scatter only");
+  if (scatter == DFA_SCATTER_AUTO) return h->has_vp2_plan ? DFA_SCATTER_BLOCKED : DFA_SCATTER_ATOMIC;
+  if (scatter == DFA_SCATTER_BLOCKED && !h->has_vp2_plan)
+    throw std::runtime_error("blocked scatter unavailable for this mesh: " + h->plan_error);
+  if (scatter != DFA_SCATTER_ATOMIC && scatter != DFA_SCATTER_BLOCKED)
+    throw std::runtime_error("unknown scatter strategy");
+  return scatter;
 }
 
 // One fused pass over the cells (Assembler::assemble_cells / SystemAssembler::cell_wise_assembly).
@@ -394,7 +381,7 @@ static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const doubl
     throw std::runtime_error("linear form does not match the vector P2 element");
   if (a_form >= 0 && (!a_c0 || !a_c1)) throw std::runtime_error("mu/lmbda are NULL");
   if (bc && a_form < 0) throw std::runtime_error("boundary conditions need the bilinear form");
-  resolve_scatter(h, scatter);
+  const int sc = resolve_scatter(h, scatter);
   dfa::VP2Pass p;
   p.want_A = values != nullptr && a_form >= 0;
   p.want_b = b != nullptr && L_form >= 0;
@@ -407,6 +394,17 @@ static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const doubl
   p.bc_value = bcv;
   p.values = values;
   p.b = b;
+  if (sc == DFA_SCATTER_BLOCKED) {
+    if (add) throw std::runtime_error("add_values is implemented by the atomic scatter for vector P2");
+    if (!h->rows_covered) {
+      if (values) hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
+      if (b) hip_check(dfa::launch_fill_zero(b, h->ndofs, s), "fill_zero");
+    }
+    if (dfa::vp2_blocked_lds_bytes(h->vtopo, p.want_b) > 160 * 1024)
+      throw std::runtime_error("vector P2 blocked plan needs more LDS than a CU has");
+    hip_check(dfa::launch_vp2_blocked(h->vtopo, p, s), "vp2_blocked_kernel");
+    return DFA_OK;
+  }
   if (!add && values) hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
   if (!add && b) hip_check(dfa::launch_fill_zero(b, h->ndofs, s), "fill_zero");
   dfa::VP2Topo t;
@@ -515,7 +513,12 @@ int dfa_plan_info(dfa_assembler h, int32_t scatter, int64_t* plan_bytes, int64_t
   return guarded([&]() -> int {
     if (!h) return fail("NULL handle", DFA_ERR_ARG);
     const int sc = resolve_scatter(h, scatter);
-    if (sc == DFA_SCATTER_BLOCKED) {
+    if (sc == DFA_SCATTER_BLOCKED && h->element == DFA_ELEMENT_VECTOR_P2) {
+      if (plan_bytes) *plan_bytes = h->plan.bytes;
+      if (num_blocks) *num_blocks = h->vtopo.num_blocks;
+      if (cell_appearances) *cell_appearances = h->appearances;
+      if (lds_bytes) *lds_bytes = static_cast<int32_t>(dfa::vp2_blocked_lds_bytes(h->vtopo, true));
+    } else if (sc == DFA_SCATTER_BLOCKED) {
       dfa::P1Pass p;
       p.gdim = h->gdim;
       p.a_form = DFA_FORM_P1_POISSON;

@@ -57,6 +57,36 @@ void build_p1_block_plan(int32_t gdim, int64_t num_vertices, int64_t num_cells,
                          const int32_t* coeff_dofs, const double* x, const HostCSR& csr,
                          int32_t target_rows, P1BlockPlan& plan);
 
+// Blocked-scatter partition for vector P2 (3 dofs per node, node-blocked CSR). P2 nodes
+// (vertices and edges) are grouped into blocks whose rows fit in LDS; a work item is a pair
+// (cell, owned node a) and adds the 10 3x3 blocks K[(.,a),(.,b)] of that cell to node a's rows.
+struct VP2BlockPlan {
+  int32_t num_blocks = 0;
+  int32_t max_verts = 0, max_owned = 0, max_vals = 0, max_tasks = 0;
+  bool component_major = true;
+  std::vector<int32_t> vert_ptr;   // [nb+1] local vertices per block
+  std::vector<int32_t> vert_id;    // global vertex of each local vertex
+  std::vector<int32_t> vert_kdof;  // P1 coefficient dof of each local vertex
+  std::vector<int32_t> own_ptr;    // [nb+1] owned nodes per block
+  std::vector<int32_t> own_rows;   // 3 global rows (component dofs) per owned node
+  std::vector<uint32_t> own_info;  // LDS value offset | (neighbour count << 16)
+  std::vector<int32_t> task_ptr;   // [nb+1]
+  std::vector<uint64_t> task;      // 3 words per work item (see build_vp2_block_plan)
+  std::vector<int32_t> task_cell;  // cell of each work item (Dirichlet path only)
+  std::vector<int32_t> blk_vals;   // [nb] LDS values per block
+};
+
+// Rank of node b among node a's column nodes for every (cell, a, b) (100 bytes per cell);
+// verifies that the CSR is node-blocked and returns true for component-major column order
+// (UFC numbering), false for node-major (block-interleaved numbering).
+bool build_vp2_ranks(int64_t num_cells, const int32_t* cell_dofs, int64_t ndofs, const HostCSR& csr,
+                     std::vector<uint8_t>& rank);
+
+void build_vp2_block_plan(int64_t num_cells, const int32_t* cells, const int32_t* cell_dofs,
+                          const int32_t* coeff_dofs, const double* x, const HostCSR& csr,
+                          const std::vector<uint8_t>& rank, bool component_major,
+                          int32_t max_block_vals, VP2BlockPlan& plan);
+
 // Limits of the blocked kernel's LDS-resident block.
 constexpr int kMaxBlockNodes = 2048;
 constexpr int kMaxBlockNnz = 8192;

@@ -84,6 +84,26 @@ struct VP2Pass {
 
 hipError_t launch_vp2_atomic(const VP2Topo& t, const VP2Pass& p, hipStream_t s);
 
+struct VP2BlockTopo {
+  int32_t num_blocks = 0;
+  int32_t max_verts = 0, max_owned = 0, max_vals = 0;
+  bool component_major = true;
+  const int32_t* vert_ptr = nullptr;
+  const int32_t* vert_id = nullptr;
+  const int32_t* vert_kdof = nullptr;
+  const int32_t* own_ptr = nullptr;
+  const int32_t* own_rows = nullptr;
+  const uint32_t* own_info = nullptr;
+  const int32_t* task_ptr = nullptr;
+  const uint64_t* task = nullptr;
+  const int32_t* task_cell = nullptr;
+  const int32_t* blk_vals = nullptr;
+  const int32_t* row_ptr = nullptr;
+  const int32_t* cell_dofs = nullptr;
+};
+hipError_t launch_vp2_blocked(const VP2BlockTopo& t, const VP2Pass& p, hipStream_t s);
+size_t vp2_blocked_lds_bytes(const VP2BlockTopo& t, bool want_b);
+
 // ---- adjoint -------------------------------------------------------------------------
 struct AdjointP1 {
   int gdim = 3;

@@ -55,9 +55,19 @@ struct CellData {
   double be[30];
 };
 
-// Geometry and exact coefficient weights of one cell.
-__device__ void vp2_cell_setup(const double (&p)[4][3], const double (&mu)[4], const double (&la)[4],
-                               CellData& c) {
+// Geometry and exact coefficient weights of one cell (held in LDS by the atomic kernel, in
+// registers by the blocked one).
+struct CellGeom {
+  double D[4][3];
+  double Wm[4][4], Wl[4][4];
+  double Rm[4], Rl[4];
+  double Tm, Tl;
+  double vol;
+};
+
+template <class C>
+__device__ __forceinline__ void vp2_cell_setup(const double (&p)[4][3], const double (&mu)[4],
+                                               const double (&la)[4], C& c) {
   double e1[3], e2[3], e3[3];
   for (int i = 0; i < 3; ++i) {
     e1[i] = p[1][i] - p[0][i];
@@ -133,7 +143,8 @@ __device__ __forceinline__ double alpha(int k1, int i1, int k2, int i2, const do
 }
 
 // 3x3 block K[(i,a),(j,b)] of the elasticity element matrix.
-__device__ __forceinline__ void vp2_block(const CellData& c, int a, int b, double (&K)[3][3]) {
+template <class C>
+__device__ __forceinline__ void vp2_block(const C& c, int a, int b, double (&K)[3][3]) {
   const Factor fa = factor_of(a), fb = factor_of(b);
   double Sm[3][3] = {}, Sl[3][3] = {};
   for (int t = 0; t < fa.n; ++t)
@@ -271,6 +282,204 @@ hipError_t launch_vp2_atomic(const VP2Topo& t, const VP2Pass& p, hipStream_t s) 
   if (p.want_A && p.want_b) return launch_t<true, true, false>(t, p, s);
   if (p.want_A) return launch_t<true, false, false>(t, p, s);
   if (p.want_b) return launch_t<false, true, false>(t, p, s);
+  return hipErrorInvalidValue;
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Blocked (owner-computes) scatter: one workgroup per block of P2 nodes. Work item =
+// (cell, owned node a): geometry from the LDS vertex records, the 10 blocks K[(.,a),(.,b)]
+// accumulated with LDS atomics into node a's 3 rows, every CSR value written once.
+// ---------------------------------------------------------------------------------------
+namespace {
+
+constexpr int kBT = 256;
+
+struct VP2Lds {
+  int vert, vals, info, rows, bacc, total;
+};
+
+__host__ __device__ inline int al16(int n) { return (n + 15) & ~15; }
+
+__host__ __device__ inline VP2Lds vp2_lds(int maxV, int maxO, int maxVals, bool b) {
+  VP2Lds L;
+  int o = 0;
+  L.vert = o; o += al16(maxV * 6 * 8);
+  L.vals = o; o += al16(maxVals * 8);
+  L.info = o; o += al16(maxO * 4);
+  L.rows = o; o += al16(maxO * 3 * 4);
+  L.bacc = o; o += b ? al16(maxO * 3 * 8) : 0;
+  L.total = o;
+  return L;
+}
+
+template <bool WANT_A, bool WANT_B, bool BC>
+__global__ __launch_bounds__(kBT) void vp2_blocked_kernel(VP2BlockTopo t, VP2Pass p, VP2Lds L) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  double* svert = reinterpret_cast<double*>(lds + L.vert);
+  double* svals = reinterpret_cast<double*>(lds + L.vals);
+  uint32_t* sinfo = reinterpret_cast<uint32_t*>(lds + L.info);
+  int32_t* srows = reinterpret_cast<int32_t*>(lds + L.rows);
+  double* sbacc = reinterpret_cast<double*>(lds + L.bacc);
+  const int nb = t.num_blocks;
+  const int q = nb >> 3, r = nb & 7;
+  const int xcd = blockIdx.x & 7, jj = blockIdx.x >> 3;
+  const int blk = xcd * q + min(xcd, r) + jj;
+  const int tid = threadIdx.x;
+
+  const int v0 = t.vert_ptr[blk], nv = t.vert_ptr[blk + 1] - v0;
+  for (int i = tid; i < nv; i += kBT) {
+    const int64_t v = t.vert_id[v0 + i];
+    const int32_t kd = t.vert_kdof[v0 + i];
+    double* rec = svert + i * 6;
+    *reinterpret_cast<double2*>(rec) = make_double2(p.x[v * 3], p.x[v * 3 + 1]);
+    const bool need_c = WANT_A || BC;  // the body force alone needs no coefficients
+    *reinterpret_cast<double2*>(rec + 2) = make_double2(p.x[v * 3 + 2], need_c ? p.mu[kd] : 0.0);
+    *reinterpret_cast<double2*>(rec + 4) = make_double2(need_c ? p.lmbda[kd] : 0.0, 0.0);
+  }
+  const int o0 = t.own_ptr[blk], no = t.own_ptr[blk + 1] - o0;
+  for (int i = tid; i < no; i += kBT) {
+    sinfo[i] = t.own_info[o0 + i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) srows[3 * i + k] = t.own_rows[3 * (o0 + i) + k];
+  }
+  if (WANT_A) {
+    const int nvals = t.blk_vals[blk];
+    for (int i = tid; i < (nvals + 1) / 2; i += kBT)
+      reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
+  }
+  if (WANT_B)
+    for (int i = tid; i < 3 * no; i += kBT) sbacc[i] = 0.0;
+  __syncthreads();
+
+  // Each work item is split over two threads (node pairs b = 0..4 and b = 5..9): blocks hold
+  // ~130 items, so one item per thread would leave half of the workgroup idle.
+  const int t0 = t.task_ptr[blk], nt = t.task_ptr[blk + 1] - t0;
+  for (int kk = tid; kk < 2 * nt; kk += kBT) {
+    const int k = kk >> 1, half = kk & 1;
+    const uint64_t w0 = t.task[3 * (t0 + k)];
+    const uint64_t w1 = t.task[3 * (t0 + k) + 1];
+    const uint64_t w2 = t.task[3 * (t0 + k) + 2];
+    const int o = static_cast<int>(w1 & 0xffffu);
+    const int a = static_cast<int>((w1 >> 16) & 0xffu);
+    double pc[4][3], mu[4], la[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double* rec = svert + static_cast<int>((w0 >> (16 * v)) & 0xffffu) * 6;
+      const double2 r0 = *reinterpret_cast<const double2*>(rec);
+      const double2 r1 = *reinterpret_cast<const double2*>(rec + 2);
+      const double2 r2 = *reinterpret_cast<const double2*>(rec + 4);
+      pc[v][0] = r0.x;
+      pc[v][1] = r0.y;
+      pc[v][2] = r1.x;
+      mu[v] = r1.y;
+      la[v] = r2.x;
+    }
+    CellGeom c;
+    vp2_cell_setup(pc, mu, la, c);
+    const uint32_t info = sinfo[o];
+    const int off = static_cast<int>(info & 0xffffu);
+    const int deg = static_cast<int>(info >> 16);
+    double be[3] = {0.0, 0.0, 0.0};
+    if (WANT_B && half == 0) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) be[i] = p.f3[i] * (a < 4 ? -c.vol / 20.0 : c.vol / 5.0);
+    }
+    // Dirichlet data of the cell (rows of node a, columns of every node b)
+    uint32_t bcrow = 0, bccol = 0;  // bit i: row (i,a); bit j*10+b: column (j,b)
+    double growv[3] = {0, 0, 0};
+    const int32_t* cdofs = nullptr;
+    if constexpr (BC) {
+      cdofs = t.cell_dofs + static_cast<int64_t>(t.task_cell[t0 + k]) * 30;
+      for (int e = 0; e < 30; ++e)
+        if (p.bc_mask[cdofs[e]]) bccol |= 1u << e;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (bccol >> (i * 10 + a) & 1u) {
+          bcrow |= 1u << i;
+          growv[i] = p.bc_value[cdofs[i * 10 + a]];
+        }
+    }
+#pragma unroll 1
+    for (int b = 5 * half; b < ((WANT_A || BC) ? 5 * half + 5 : 0); ++b) {
+      double K[3][3];
+      vp2_block(c, a, b, K);
+      if constexpr (BC) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const bool cb = (bccol >> (j * 10 + b)) & 1u;
+            if (WANT_B && cb && !(bcrow >> i & 1u))
+              be[i] -= K[i][j] * p.bc_value[cdofs[j * 10 + b]];
+            if (cb || (bcrow >> i & 1u)) K[i][j] = (i == j && a == b) ? 1.0 : 0.0;
+          }
+      }
+      if constexpr (WANT_A) {
+        const int rk = static_cast<int>(b < 5 ? (w1 >> (24 + 8 * b)) & 0xffu : (w2 >> (8 * (b - 5))) & 0xffu);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          double* row = svals + off + i * 3 * deg + (t.component_major ? rk : 3 * rk);
+          const int s2 = t.component_major ? deg : 1;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) unsafeAtomicAdd(row + j * s2, K[i][j]);
+        }
+      }
+    }
+    if constexpr (WANT_B) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        // a Dirichlet row takes its value once (from the first half), never the lifting
+        const double v = (BC && (bcrow >> i & 1u)) ? (half == 0 ? growv[i] : 0.0) : be[i];
+        unsafeAtomicAdd(sbacc + 3 * o + i, v);
+      }
+    }
+  }
+  __syncthreads();
+
+  // write-out: each owned node's 3 rows are contiguous CSR ranges
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int i = wave; i < 3 * no; i += kBT / 64) {
+    const int o = i / 3, comp = i % 3;
+    const uint32_t info = sinfo[o];
+    const int off = static_cast<int>(info & 0xffffu) + comp * 3 * static_cast<int>(info >> 16);
+    const int len = 3 * static_cast<int>(info >> 16);
+    const int32_t row = srows[i];
+    if (WANT_A) {
+      const int64_t base = t.row_ptr[row];
+      for (int j = lane; j < len; j += 64) p.values[base + j] = svals[off + j];
+    }
+    if (WANT_B && lane == 0) p.b[row] = sbacc[i];
+  }
+}
+
+template <bool A, bool B, bool BC>
+hipError_t launch_blocked_t(const VP2BlockTopo& t, const VP2Pass& p, hipStream_t s) {
+  const VP2Lds L = vp2_lds(t.max_verts, t.max_owned, t.max_vals, B);
+  auto kern = vp2_blocked_kernel<A, B, BC>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, L.total);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(t.num_blocks), dim3(kBT), L.total, s, t, p, L);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t vp2_blocked_lds_bytes(const VP2BlockTopo& t, bool want_b) {
+  return static_cast<size_t>(vp2_lds(t.max_verts, t.max_owned, t.max_vals, want_b).total);
+}
+
+hipError_t launch_vp2_blocked(const VP2BlockTopo& t, const VP2Pass& p, hipStream_t s) {
+  if (t.num_blocks == 0) return hipSuccess;
+  if (p.bc) {
+    if (p.want_A && p.want_b) return launch_blocked_t<true, true, true>(t, p, s);
+    if (p.want_A) return launch_blocked_t<true, false, true>(t, p, s);
+    return launch_blocked_t<false, true, true>(t, p, s);
+  }
+  if (p.want_A && p.want_b) return launch_blocked_t<true, true, false>(t, p, s);
+  if (p.want_A) return launch_blocked_t<true, false, false>(t, p, s);
+  if (p.want_b) return launch_blocked_t<false, true, false>(t, p, s);
   return hipErrorInvalidValue;
 }
 

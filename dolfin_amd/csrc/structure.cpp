@@ -380,3 +380,263 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
 }
 
 }  // namespace dfa
+
+namespace dfa {
+
+// Vector P2 rank table. Nodes are identified by the cell-local node index a (0..9) of the
+// component-0 dof; the CSR row of dof (i, a) lists, for each column component j, the
+// neighbour nodes in ascending order, either component-major (UFC numbering, stride s1 = 1,
+// component stride = deg) or node-major (block-interleaved numbering, s1 = 3, stride 1).
+bool build_vp2_ranks(int64_t nc, const int32_t* cd, int64_t ndofs, const HostCSR& csr,
+                     std::vector<uint8_t>& rank) {
+  // Node id of a dof: dof of component 0 at the same cell-local node.
+  std::vector<int32_t> node0(ndofs, -1), comp(ndofs, -1);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int i = 0; i < 3; ++i)
+      for (int a = 0; a < 10; ++a) {
+        const int32_t d = cd[c * 30 + i * 10 + a];
+        const int32_t n0 = cd[c * 30 + a];
+        if ((node0[d] >= 0 && node0[d] != n0) || (comp[d] >= 0 && comp[d] != i))
+          throw std::runtime_error("vector P2 dofmap is not node-blocked");
+        node0[d] = n0;
+        comp[d] = i;
+      }
+  // Determine the layout from the first row with more than one neighbour.
+  int layout = 0;  // 1 = component-major, 2 = node-major
+  for (int64_t r = 0; r < ndofs && !layout; ++r) {
+    const int32_t b = csr.row_ptr[r], e = csr.row_ptr[r + 1];
+    if (e - b < 6) continue;
+    layout = (comp[csr.col[b]] == comp[csr.col[b + 1]]) ? 1 : 2;
+  }
+  if (!layout) layout = 1;
+
+  rank.assign(static_cast<size_t>(nc) * 100, 0);
+  bool bad = false;
+  for (int64_t c = 0; c < nc && !bad; ++c)
+    for (int a = 0; a < 10; ++a) {
+      const int32_t ra = cd[c * 30 + a];  // component-0 row of node a
+      const int32_t rb0 = csr.row_ptr[ra], re0 = csr.row_ptr[ra + 1];
+      const int32_t len = re0 - rb0;
+      if (len % 3) { bad = true; break; }
+      const int32_t deg = len / 3;
+      for (int b = 0; b < 10; ++b) {
+        // locate column (component 0, node b) in row ra
+        const int32_t colb = cd[c * 30 + b];
+        const int32_t* p = std::lower_bound(csr.col.data() + rb0, csr.col.data() + re0, colb);
+        if (p == csr.col.data() + re0 || *p != colb) { bad = true; break; }
+        const int64_t pos = p - (csr.col.data() + rb0);
+        const int64_t rk = (layout == 1) ? pos : pos / 3;
+        if (rk > 255 || rk >= deg) { bad = true; break; }
+        rank[c * 100 + a * 10 + b] = static_cast<uint8_t>(rk);
+        // verify all 9 component combinations land on the expected columns
+        for (int i = 0; i < 3 && !bad; ++i) {
+          const int32_t row = cd[c * 30 + i * 10 + a];
+          if (csr.row_ptr[row + 1] - csr.row_ptr[row] != len) { bad = true; break; }
+          for (int j = 0; j < 3; ++j) {
+            const int64_t slot = csr.row_ptr[row] + ((layout == 1) ? (rk + j * deg) : (rk * 3 + j));
+            if (csr.col[slot] != cd[c * 30 + j * 10 + b]) { bad = true; break; }
+          }
+        }
+      }
+    }
+  if (bad) throw std::runtime_error("vector P2 sparsity is not node-blocked (unsupported dofmap)");
+  return layout == 1;
+}
+
+
+// Vector P2 blocked partition.
+//  1. nodes are identified by their component-0 dof; node coordinates are vertices and edge
+//     midpoints; recursive coordinate bisection groups them so that each block's rows
+//     (3 rows of 3*deg entries per node) fit in max_block_vals doubles of LDS;
+//  2. every (cell, local node a) pair becomes one work item of the block owning node a;
+//  3. per block: the cells' vertices (local ids), the owned nodes' rows and LDS offsets, and
+//     per work item the 4 local vertex ids, the owned-node index, a, and the 10 ranks of the
+//     cell's nodes b among node a's neighbours (the same ranks the atomic kernel uses).
+// Task words: w0 = 4 x 16-bit local vertex ids; w1 = owned index (16) | a (8) << 16 |
+// ranks b=0..4 (8 bits each) << 24; w2 = ranks b=5..9 (8 bits each).
+void build_vp2_block_plan(int64_t nc, const int32_t* cells, const int32_t* cd, const int32_t* kd,
+                          const double* x, const HostCSR& csr, const std::vector<uint8_t>& rank,
+                          bool component_major, int32_t max_block_vals, VP2BlockPlan& P) {
+  static const int edge[6][2] = {{2, 3}, {1, 3}, {1, 2}, {0, 3}, {0, 2}, {0, 1}};
+  const int64_t ndofs = static_cast<int64_t>(csr.row_ptr.size()) - 1;
+  P = VP2BlockPlan();
+  P.component_major = component_major;
+  // node table (indexed by component-0 dof)
+  std::vector<float> nx(3 * ndofs, 0.f);
+  std::vector<uint8_t> isnode(ndofs, 0);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int a = 0; a < 10; ++a) {
+      const int32_t n = cd[c * 30 + a];
+      if (isnode[n]) continue;
+      isnode[n] = 1;
+      for (int q = 0; q < 3; ++q) {
+        double v;
+        if (a < 4) v = x[static_cast<int64_t>(cells[c * 4 + a]) * 3 + q];
+        else v = 0.5 * (x[static_cast<int64_t>(cells[c * 4 + edge[a - 4][0]]) * 3 + q] +
+                        x[static_cast<int64_t>(cells[c * 4 + edge[a - 4][1]]) * 3 + q]);
+        nx[3 * n + q] = static_cast<float>(v);
+      }
+    }
+  std::vector<int32_t> ids;
+  for (int64_t n = 0; n < ndofs; ++n)
+    if (isnode[n]) ids.push_back(static_cast<int32_t>(n));
+  auto vals_of = [&](int32_t n) { return 3 * (csr.row_ptr[n + 1] - csr.row_ptr[n]); };
+  std::vector<std::pair<int64_t, int64_t>> leaves;
+  std::function<void(int64_t, int64_t)> rcb = [&](int64_t lo, int64_t hi) {
+    int64_t tot = 0;
+    for (int64_t i = lo; i < hi; ++i) tot += vals_of(ids[i]);
+    if (tot <= max_block_vals || hi - lo == 1) {
+      if (tot > max_block_vals) throw std::runtime_error("vector P2 node rows exceed the LDS block");
+      leaves.emplace_back(lo, hi);
+      return;
+    }
+    float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+    for (int64_t i = lo; i < hi; ++i)
+      for (int q = 0; q < 3; ++q) {
+        mn[q] = std::min(mn[q], nx[3 * ids[i] + q]);
+        mx[q] = std::max(mx[q], nx[3 * ids[i] + q]);
+      }
+    int ax = 0;
+    for (int q = 1; q < 3; ++q)
+      if (mx[q] - mn[q] > mx[ax] - mn[ax]) ax = q;
+    const int64_t nleaf = (tot + max_block_vals - 1) / max_block_vals;
+    int64_t mid = lo + (hi - lo) * (nleaf / 2) / nleaf;
+    if (mid <= lo) mid = lo + 1;
+    std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi,
+                     [&](int32_t a, int32_t b) {
+                       const float pa = nx[3 * a + ax], pb = nx[3 * b + ax];
+                       return pa < pb || (pa == pb && a < b);
+                     });
+    rcb(lo, mid);
+    rcb(mid, hi);
+  };
+  rcb(0, static_cast<int64_t>(ids.size()));
+  const int32_t nb = static_cast<int32_t>(leaves.size());
+  P.num_blocks = nb;
+  std::vector<int32_t> owner(ndofs, -1), own_index(ndofs, -1);
+  for (int32_t b = 0; b < nb; ++b) {
+    std::sort(ids.begin() + leaves[b].first, ids.begin() + leaves[b].second);
+    for (int64_t i = leaves[b].first; i < leaves[b].second; ++i) {
+      owner[ids[i]] = b;
+      own_index[ids[i]] = static_cast<int32_t>(i - leaves[b].first);
+    }
+  }
+  // work items per block (cell order)
+  P.task_ptr.assign(nb + 1, 0);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int a = 0; a < 10; ++a) P.task_ptr[owner[cd[c * 30 + a]] + 1]++;
+  for (int32_t b = 0; b < nb; ++b) P.task_ptr[b + 1] += P.task_ptr[b];
+  const int64_t ntask = P.task_ptr[nb];
+  std::vector<int64_t> titem(ntask);  // cell * 10 + a
+  {
+    std::vector<int32_t> pos(P.task_ptr.begin(), P.task_ptr.end() - 1);
+    for (int64_t c = 0; c < nc; ++c)
+      for (int a = 0; a < 10; ++a) titem[pos[owner[cd[c * 30 + a]]]++] = c * 10 + a;
+  }
+  P.task.resize(3 * ntask);
+  P.task_cell.resize(ntask);
+  P.blk_vals.assign(nb, 0);
+  std::vector<std::vector<int32_t>> bverts(nb);
+  std::vector<std::vector<uint32_t>> binfo(nb);
+  std::vector<std::vector<int32_t>> brows(nb);
+  std::atomic<int> failed{0};
+  parallel_for(nb, [&](int64_t b0, int64_t b1, int) {
+    std::vector<int64_t> order;
+    for (int64_t b = b0; b < b1; ++b) {
+      const int32_t t0 = P.task_ptr[b], t1 = P.task_ptr[b + 1];
+      auto& verts = bverts[b];
+      for (int32_t k = t0; k < t1; ++k) {
+        const int64_t c = titem[k] / 10;
+        for (int v = 0; v < 4; ++v) verts.push_back(cells[c * 4 + v]);
+      }
+      std::sort(verts.begin(), verts.end());
+      verts.erase(std::unique(verts.begin(), verts.end()), verts.end());
+      if (verts.size() > 65535) { failed = 1; continue; }
+      // owned nodes: LDS offsets and rows
+      uint32_t off = 0;
+      for (int64_t i = leaves[b].first; i < leaves[b].second; ++i) {
+        const int32_t n = ids[i];
+        const int32_t len = csr.row_ptr[n + 1] - csr.row_ptr[n];
+        binfo[b].push_back(off | (static_cast<uint32_t>(len / 3) << 16));
+        off += 3 * len;
+      }
+      P.blk_vals[b] = static_cast<int32_t>(off);
+      // rows of each owned node: taken from any cell that contains it
+      brows[b].assign(3 * (leaves[b].second - leaves[b].first), -1);
+      // interleave work items over owned nodes so that a wavefront's lanes add to different rows
+      order.assign(titem.begin() + t0, titem.begin() + t1);
+      std::vector<int32_t> seen(leaves[b].second - leaves[b].first, 0);
+      std::vector<std::pair<int64_t, int64_t>> key(order.size());
+      for (size_t q = 0; q < order.size(); ++q) {
+        const int64_t c = order[q] / 10;
+        const int a = static_cast<int>(order[q] % 10);
+        const int32_t o = own_index[cd[c * 30 + a]];
+        key[q] = {static_cast<int64_t>(seen[o]++) * 65536 + o, order[q]};
+        for (int i = 0; i < 3; ++i) brows[b][3 * o + i] = cd[c * 30 + i * 10 + a];
+      }
+      std::sort(key.begin(), key.end());
+      for (size_t q = 0; q < key.size(); ++q) {
+        const int64_t item = key[q].second;
+        const int64_t c = item / 10;
+        const int a = static_cast<int>(item % 10);
+        const int32_t o = own_index[cd[c * 30 + a]];
+        uint64_t w0 = 0, w1 = 0, w2 = 0;
+        for (int v = 0; v < 4; ++v) {
+          const uint64_t lv = static_cast<uint64_t>(
+              std::lower_bound(verts.begin(), verts.end(), cells[c * 4 + v]) - verts.begin());
+          w0 |= lv << (16 * v);
+        }
+        w1 = static_cast<uint64_t>(o) | (static_cast<uint64_t>(a) << 16);
+        for (int bb = 0; bb < 10; ++bb) {
+          const uint64_t r = rank[c * 100 + a * 10 + bb];
+          if (bb < 5) w1 |= r << (24 + 8 * bb);
+          else w2 |= r << (8 * (bb - 5));
+        }
+        const int64_t k = t0 + static_cast<int64_t>(q);
+        P.task[3 * k] = w0;
+        P.task[3 * k + 1] = w1;
+        P.task[3 * k + 2] = w2;
+        P.task_cell[k] = static_cast<int32_t>(c);
+      }
+    }
+  });
+  if (failed) throw std::runtime_error("vector P2 plan: too many vertices in a block");
+  P.vert_ptr.assign(nb + 1, 0);
+  P.own_ptr.assign(nb + 1, 0);
+  for (int32_t b = 0; b < nb; ++b) {
+    P.vert_ptr[b + 1] = P.vert_ptr[b] + static_cast<int32_t>(bverts[b].size());
+    P.own_ptr[b + 1] = P.own_ptr[b] + static_cast<int32_t>(binfo[b].size());
+    P.max_verts = std::max<int32_t>(P.max_verts, static_cast<int32_t>(bverts[b].size()));
+    P.max_owned = std::max<int32_t>(P.max_owned, static_cast<int32_t>(binfo[b].size()));
+    P.max_vals = std::max(P.max_vals, P.blk_vals[b]);
+    P.max_tasks = std::max(P.max_tasks, P.task_ptr[b + 1] - P.task_ptr[b]);
+  }
+  // vertex -> coefficient dof (must be a function of the vertex)
+  std::vector<int32_t> vk;
+  {
+    int32_t nvmax = 0;
+    for (int64_t i = 0; i < nc * 4; ++i) nvmax = std::max(nvmax, cells[i] + 1);
+    vk.assign(nvmax, -1);
+    for (int64_t i = 0; i < nc * 4; ++i) {
+      if (vk[cells[i]] >= 0 && vk[cells[i]] != kd[i])
+        throw std::runtime_error("coefficient dofmap is not a function of the vertex");
+      vk[cells[i]] = kd[i];
+    }
+  }
+  P.vert_id.resize(P.vert_ptr[nb]);
+  P.vert_kdof.resize(P.vert_ptr[nb]);
+  P.own_info.resize(P.own_ptr[nb]);
+  P.own_rows.resize(3 * static_cast<size_t>(P.own_ptr[nb]));
+  for (int32_t b = 0; b < nb; ++b) {
+    int32_t o = P.vert_ptr[b];
+    for (int32_t v : bverts[b]) {
+      P.vert_id[o] = v;
+      P.vert_kdof[o] = vk[v];
+      ++o;
+    }
+    std::copy(binfo[b].begin(), binfo[b].end(), P.own_info.begin() + P.own_ptr[b]);
+    std::copy(brows[b].begin(), brows[b].end(), P.own_rows.begin() + 3 * static_cast<size_t>(P.own_ptr[b]));
+  }
+}
+
+}  // namespace dfa

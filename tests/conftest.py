@@ -51,7 +51,8 @@ def plan_lib():
         os.makedirs(out, exist_ok=True)
         so = os.path.join(out, "libplan_check.so")
         srcs = [os.path.join(ROOT, "tests", "plan_check.cpp"),
-                os.path.join(ROOT, "dolfin_amd", "csrc", "structure.cpp")]
+                os.path.join(ROOT, "dolfin_amd", "csrc", "structure.cpp"),
+                os.path.join(ROOT, "oracle", "oracle.cpp")]
         if not os.path.exists(so) or any(os.path.getmtime(s) > os.path.getmtime(so) for s in srcs):
             subprocess.check_call(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread",
                                    "-o", so] + srcs)
@@ -61,6 +62,9 @@ def plan_lib():
                                     ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp, vp,
                                     ctypes.c_int64, vp]
         L.plan_check_p1.restype = ctypes.c_int
+        L.plan_check_vp2.argtypes = [ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int64, vp, vp,
+                                     ctypes.c_int32, vp, ctypes.c_int64, vp]
+        L.plan_check_vp2.restype = ctypes.c_int
         L.csr_host.argtypes = [ctypes.c_int64, ctypes.c_int32, vp, ctypes.c_int64, ctypes.c_int32,
                                vp, vp, ctypes.c_int64]
         L.csr_host.restype = ctypes.c_int64

@@ -125,3 +125,97 @@ extern "C" int64_t csr_host(int64_t nc, int32_t nd, const int32_t* cd, int64_t n
   for (int64_t i = 0; i < nnz; ++i) col[i] = csr.col[i];
   return nnz;
 }
+
+// Vector P2 blocked partition walked like vp2_blocked_kernel: every index is checked against
+// the block's LDS sizes and the oracle's element matrices are accumulated through the plan
+// (task words, ranks, owned-row offsets, write-out rows).
+extern "C" void orc_vp2_elasticity(const double* xc, const double* mu, const double* lmbda, double* A);
+
+extern "C" int plan_check_vp2(int64_t nc, const int32_t* cells, const int32_t* cd, const int32_t* kd,
+                              const double* x, int64_t ndofs, const double* mu, const double* lmbda,
+                              int32_t block_vals, double* values, int64_t nnz_cap, int32_t* stats) {
+  try {
+    dfa::HostCSR csr;
+    dfa::build_csr(nc, 30, cd, ndofs, false, csr);
+    if (static_cast<int64_t>(csr.col.size()) > nnz_cap) return -10;
+    std::vector<uint8_t> rank;
+    const bool cm = dfa::build_vp2_ranks(nc, cd, ndofs, csr, rank);
+    dfa::VP2BlockPlan P;
+    dfa::build_vp2_block_plan(nc, cells, cd, kd, x, csr, rank, cm, block_vals, P);
+    std::vector<int> seen(nc * 10, 0);
+    std::vector<int> rowseen(ndofs, 0);
+    for (int b = 0; b < P.num_blocks; ++b) {
+      const int v0 = P.vert_ptr[b], nv = P.vert_ptr[b + 1] - v0;
+      const int o0 = P.own_ptr[b], no = P.own_ptr[b + 1] - o0;
+      const int nvals = P.blk_vals[b];
+      if (nv > P.max_verts || no > P.max_owned || nvals > P.max_vals || nvals > block_vals) return -1;
+      std::vector<double> vals(nvals, 0.0);
+      for (int o = 0; o < no; ++o) {
+        const uint32_t info = P.own_info[o0 + o];
+        const int off = info & 0xffff, deg = info >> 16;
+        if (off + 9 * deg > nvals) return -2;
+        for (int i = 0; i < 3; ++i) {
+          const int32_t row = P.own_rows[3 * (o0 + o) + i];
+          if (row < 0 || row >= ndofs) return -3;
+          if (csr.row_ptr[row + 1] - csr.row_ptr[row] != 3 * deg) return -4;
+          rowseen[row]++;
+        }
+      }
+      for (int k = P.task_ptr[b]; k < P.task_ptr[b + 1]; ++k) {
+        const uint64_t w0 = P.task[3 * k], w1 = P.task[3 * k + 1], w2 = P.task[3 * k + 2];
+        const int o = static_cast<int>(w1 & 0xffffu), a = static_cast<int>((w1 >> 16) & 0xffu);
+        if (o >= no || a >= 10) return -5;
+        const int64_t c = P.task_cell[k];
+        seen[c * 10 + a]++;
+        double xc[12], m[4], l[4];
+        for (int v = 0; v < 4; ++v) {
+          const int lv = static_cast<int>((w0 >> (16 * v)) & 0xffffu);
+          if (lv >= nv) return -6;
+          const int32_t gv = P.vert_id[v0 + lv];
+          if (gv != cells[c * 4 + v]) return -7;
+          for (int q = 0; q < 3; ++q) xc[3 * v + q] = x[static_cast<int64_t>(gv) * 3 + q];
+          m[v] = mu[P.vert_kdof[v0 + lv]];
+          l[v] = lmbda[P.vert_kdof[v0 + lv]];
+        }
+        // the owned node's rows must be the cell's dofs at local node a
+        for (int i = 0; i < 3; ++i)
+          if (P.own_rows[3 * (o0 + o) + i] != cd[c * 30 + i * 10 + a]) return -8;
+        double A[900];
+        orc_vp2_elasticity(xc, m, l, A);
+        const uint32_t info = P.own_info[o0 + o];
+        const int off = info & 0xffff, deg = info >> 16;
+        for (int bb = 0; bb < 10; ++bb) {
+          const int rk = static_cast<int>(bb < 5 ? (w1 >> (24 + 8 * bb)) & 0xffu : (w2 >> (8 * (bb - 5))) & 0xffu);
+          if (rk >= deg) return -9;
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+              const int slot = off + i * 3 * deg + (cm ? rk + j * deg : 3 * rk + j);
+              if (slot >= nvals) return -11;
+              vals[slot] += A[(i * 10 + a) * 30 + j * 10 + bb];
+            }
+        }
+      }
+      for (int o = 0; o < no; ++o) {
+        const uint32_t info = P.own_info[o0 + o];
+        const int off = info & 0xffff, deg = info >> 16;
+        for (int i = 0; i < 3; ++i) {
+          const int32_t row = P.own_rows[3 * (o0 + o) + i];
+          for (int j = 0; j < 3 * deg; ++j) values[csr.row_ptr[row] + j] = vals[off + i * 3 * deg + j];
+        }
+      }
+    }
+    for (int64_t i = 0; i < nc * 10; ++i)
+      if (seen[i] != 1) return -12;
+    for (int64_t r = 0; r < ndofs; ++r)
+      if (rowseen[r] != 1) return -13;
+    stats[0] = P.num_blocks;
+    stats[1] = P.max_verts;
+    stats[2] = P.max_owned;
+    stats[3] = P.max_vals;
+    stats[4] = static_cast<int32_t>(P.task_cell.size());
+    return 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "plan_check_vp2: %s\n", e.what());
+    return -100;
+  }
+}

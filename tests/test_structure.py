@@ -164,3 +164,33 @@ def test_workload_sizes():
     # algorithmic bytes of the 2D parity config (12 B connectivity/cell)
     assert workloads.p1_algorithmic_bytes(8192, 3, 4225, 2, 29057, 4225) == 565292
     assert workloads.p1_algorithmic_bytes(12582912, 4, 2146689, 3, 31802497, 2146689) == 660237364
+
+
+@pytest.mark.parametrize("n,block,vals", [(2, False, 2048), (3, False, 6144), (3, True, 4096),
+                                          (4, False, 6144)])
+def test_vp2_blocked_plan_walk(oracle, plan_lib, n, block, vals):
+    rng = np.random.default_rng(n)
+    mesh = dfa.UnitCubeMesh(n, n, n)
+    dfa.jitter_interior(mesh, 0.15 / n, rng)
+    perm = None
+    if block:
+        perm = rng.permutation(mesh.num_vertices() + mesh.cell_edges()[1])
+    V = dfa.VectorFunctionSpace(mesh, "P", 2, node_permutation=perm)
+    Q = V.coefficient_space()
+    mu = rng.uniform(0.5, 1.5, Q.dim())
+    la = rng.uniform(1, 10, Q.dim())
+    cells = np.ascontiguousarray(mesh.cells(), np.int32)
+    cd = np.ascontiguousarray(V.cell_dofs(), np.int32)
+    kd = np.ascontiguousarray(Q.cell_dofs(), np.int32)
+    x = np.ascontiguousarray(mesh.coordinates())
+    cap = cells.shape[0] * 900
+    out = np.zeros(cap)
+    stats = np.zeros(8, np.int32)
+    rc = plan_lib.plan_check_vp2(cells.shape[0], p(cells), p(cd), p(kd), p(x), V.dim(), p(mu), p(la),
+                                 vals, p(out), cap, p(stats))
+    assert rc == 0, f"vector P2 plan check failed with code {rc}"
+    rp, col = oracle.sparsity(cd, V.dim())
+    ref = oracle.assemble_matrix(oracle.VP2_ELASTICITY, x, cells, cd, rp, col, coeff_dofs=kd, c0=mu, c1=la)
+    got = out[: col.shape[0]]
+    assert np.linalg.norm(got - ref) <= 1e-13 * np.linalg.norm(ref)
+    assert stats[3] <= vals
