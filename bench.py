@@ -188,13 +188,16 @@ def main():
             t2_first = time.perf_counter() - t
             log(f"vector P2 first assembly {t2_first:.1f} s")
             e2 = dfa.engine_for(V2)
+            info2 = e2.plan_info()
+            sc2 = "blocked" if info2["num_blocks"] > 0 else "atomic"
             w2, ev2 = timed_steps(lambda: asm2.assemble(A2, b2), max(5, args.steps // 5), 2)
             nc2 = m2.num_cells()
             ms2 = float(np.mean(ev2))
             alg2 = workloads.vp2_algorithmic_bytes(nc2, m2.num_vertices(), e2.nnz, V2.dim())
             sec["vp2_elasticity"] = {
                 "workload": f"vector P2 elasticity + body force, UnitCubeMesh {args.p2_n}^3 "
-                            f"({nc2} cells, {V2.dim()} dofs, nnz {e2.nnz}), fused, atomic scatter",
+                            f"({nc2} cells, {V2.dim()} dofs, nnz {e2.nnz}), fused, {sc2} scatter",
+                "plan": info2,
                 "ms": ms2, "cells_per_s": nc2 / (ms2 * 1e-3),
                 "hbm_achieved_GBs": alg2 / (ms2 * 1e-3) / 1e9,
                 "hbm_frac": alg2 / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS,

@@ -23,17 +23,18 @@ def _forms(V, mu, lmbda, f3):
     return a, L
 
 
-@pytest.mark.parametrize("n,block", [(2, False), (3, False), (3, True)])
-def test_vp2_vs_oracle(oracle, n, block):
-    print(f"case vp2 n={n} block={block}", flush=True)
+@pytest.mark.parametrize("scatter", [dfa.SCATTER_ATOMIC, dfa.SCATTER_BLOCKED])
+@pytest.mark.parametrize("n,block", [(2, False), (3, False), (3, True), (5, False)])
+def test_vp2_vs_oracle(oracle, n, block, scatter):
+    print(f"case vp2 n={n} block={block} scatter={scatter}", flush=True)
     perm = None
     if block:
         m = dfa.UnitCubeMesh(n, n, n)
         perm = np.random.default_rng(n).permutation(m.num_vertices() + m.cell_edges()[1])
     mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(n, seed=n, node_permutation=perm)
     a, L = _forms(V, mu, lmbda, f3)
-    A = dfa.assemble(a)
-    b = dfa.assemble(L)
+    A = dfa.assemble(a, scatter=scatter)
+    b = dfa.assemble(L, scatter=scatter)
     rp, col, vals = A.data()
     orp, ocol = oracle.sparsity(V.cell_dofs(), V.dim())
     np.testing.assert_array_equal(rp, orp)
@@ -47,11 +48,12 @@ def test_vp2_vs_oracle(oracle, n, block):
     assert rel(b.get_local(), bref) < TOL
 
 
-def test_vp2_system_bc_vs_oracle(oracle):
-    mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(2, seed=8)
+@pytest.mark.parametrize("scatter", [dfa.SCATTER_ATOMIC, dfa.SCATTER_BLOCKED])
+def test_vp2_system_bc_vs_oracle(oracle, scatter):
+    mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(3, seed=8)
     a, L = _forms(V, mu, lmbda, f3)
     bc = dfa.DirichletBC(V, 0.25)
-    A, b = dfa.assemble_system(a, L, bc)
+    A, b = dfa.assemble_system(a, L, bc, scatter=scatter)
     rp, col, vals = A.data()
     mask = np.zeros(V.dim(), np.uint8)
     mask[bc.dofs] = 1
@@ -71,6 +73,8 @@ def test_vp2_32_rigid_modes():
     print("case vp2_32 rigid modes", flush=True)
     a, L = _forms(V, mu, lmbda, f3)
     A = dfa.assemble(a)
+    Aa = dfa.assemble(a, scatter=dfa.SCATTER_ATOMIC)
+    assert (torch.linalg.norm(A.values - Aa.values) / torch.linalg.norm(Aa.values)).item() < TOL
     rp, col = A.structure()
     rows = torch.repeat_interleave(torch.arange(V.dim(), device="cuda"), (rp[1:] - rp[:-1]).long())
     nv = mesh.num_vertices()
