@@ -115,7 +115,7 @@ int default_p2_block_vals() {
     const int v = std::atoi(e);
     if (v >= 1024 && v <= 16384) return v;
   }
-  return 6144;
+  return 8192;
 }
 
 int default_block_rows(int gdim) {
