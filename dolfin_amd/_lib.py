@@ -30,7 +30,8 @@ ADD_VALUES = 1
 
 EXPORTS = ["dfa_last_error", "dfa_version", "dfa_assembler_create", "dfa_assembler_destroy",
            "dfa_csr_dims", "dfa_csr_structure", "dfa_assemble_matrix", "dfa_assemble_vector",
-           "dfa_assemble_system", "dfa_adjoint", "dfa_plan_info"]
+           "dfa_assemble_system", "dfa_adjoint", "dfa_plan_info", "dfa_set_plan_defaults",
+           "dfa_batch_create", "dfa_batch_launch", "dfa_batch_info", "dfa_batch_destroy"]
 
 _lib = None
 
@@ -61,6 +62,14 @@ def load():
     L.dfa_adjoint.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.dfa_plan_info.argtypes = [_vp, _i32, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i64), ctypes.POINTER(_i32)]
+    L.dfa_set_plan_defaults.argtypes = [_i32, _i32, _i32]
+    _pp = ctypes.POINTER(ctypes.c_void_p)
+    L.dfa_batch_create.argtypes = [ctypes.POINTER(_vp), _i32, _pp, ctypes.POINTER(_i32), _pp, _pp, _pp,
+                                   _pp, _i32, _vp]
+    L.dfa_batch_launch.argtypes = [_vp, _vp]
+    L.dfa_batch_info.argtypes = [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                                 ctypes.POINTER(_i64)]
+    L.dfa_batch_destroy.argtypes = [_vp]
     for name in EXPORTS:
         if name not in ("dfa_last_error", "dfa_version"):
             getattr(L, name).restype = ctypes.c_int

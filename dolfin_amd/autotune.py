@@ -1,0 +1,84 @@
+"""Autotuner for the blocked-scatter plans and the batched launch.
+
+The tunables are the plan granularity (P1 rows per block, P1 workgroup size, vector-P2 LDS
+values per block -- i.e. cells per workgroup) and the number of resident workgroups per CU
+of the batched persistent kernel. Candidates are timed with HIP events on a sample of the
+workload and the fastest is returned; results are cached per (kind, size class).
+"""
+import ctypes
+import itertools
+import json
+import os
+import time
+
+import torch
+
+from . import _lib
+
+_CACHE = {}
+
+P1_ROWS = (128, 192, 256)
+P1_THREADS = (256, 512)
+P2_VALS = (4096, 6144, 8192, 10240)
+
+
+def set_plan_defaults(p1_rows=0, p1_threads=0, p2_vals=0):
+    _lib.check(_lib.load().dfa_set_plan_defaults(int(p1_rows), int(p1_threads), int(p2_vals)))
+
+
+def _time(fn, reps=5, warmup=2):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def tune_forms(make_forms, candidates=None, reps=5):
+    """make_forms() -> list of forms built from fresh spaces (so plans are rebuilt with the
+    current defaults). Each candidate (p1_rows, p1_threads, p2_vals) is applied with
+    set_plan_defaults, the forms are assembled one by one and timed; returns
+    (best candidate, {candidate: ms})."""
+    from .fem import Assembler
+    from .la import Matrix
+    if candidates is None:
+        candidates = list(itertools.product(P1_ROWS, P1_THREADS, P2_VALS))
+    times = {}
+    for cand in candidates:
+        set_plan_defaults(*cand)
+        try:
+            forms = make_forms()
+            tensors = [Matrix() for _ in forms]
+            asm = Assembler()
+
+            def run():
+                for f, A in zip(forms, tensors):
+                    asm.assemble(A, f)
+            times[cand] = _time(run, reps)
+        finally:
+            set_plan_defaults(0, 0, 0)
+    best = min(times, key=times.get)
+    return best, times
+
+
+def tune_batch_grid(make_batch, grids=(1, 2, 4, 0), reps=5):
+    """Time the batched launch for several resident-workgroups-per-CU limits."""
+    times = {}
+    for g in grids:
+        b = make_batch(g)
+        times[g] = _time(b.assemble, reps)
+        del b
+    best = min(times, key=times.get)
+    return best, times
+
+
+def cached(key, compute):
+    if key not in _CACHE:
+        _CACHE[key] = compute()
+    return _CACHE[key]

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "common.h"
+#include "assembler.h"
 #include "kernels.h"
 
 namespace dfa {
@@ -20,66 +21,7 @@ thread_local std::string g_error;
 }
 void set_error(const std::string& msg) { g_error = msg; }
 const char* last_error() { return g_error.c_str(); }
-
-struct HipError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-
-inline void hip_check(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-struct DeviceArena {
-  std::vector<void*> ptrs;
-  int64_t bytes = 0;
-  template <class T>
-  T* upload(const std::vector<T>& v, hipStream_t s) {
-    return upload(v.data(), v.size(), s);
-  }
-  template <class T>
-  T* upload(const T* h, size_t n, hipStream_t s) {
-    void* d = nullptr;
-    const size_t nb = std::max<size_t>(n * sizeof(T), 16);
-    hip_check(hipMalloc(&d, nb), "hipMalloc");
-    ptrs.push_back(d);
-    bytes += static_cast<int64_t>(n * sizeof(T));
-    if (n) hip_check(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
-    return static_cast<T*>(d);
-  }
-  void release() {
-    for (void* p : ptrs) (void)hipFree(p);
-    ptrs.clear();
-    bytes = 0;
-  }
-};
-
 }  // namespace dfa
-
-struct dfa_assembler_s {
-  int32_t gdim = 0, element = 0, nd = 0, nvc = 0;
-  int64_t nv = 0, nc = 0, ndofs = 0, nkdofs = 0, nnz = 0;
-  bool rows_covered = true;   // every CSR row belongs to a dof of some cell
-  int32_t* d_cells = nullptr;
-  int32_t* d_cdofs = nullptr;
-  int32_t* d_kdofs = nullptr;
-  int32_t* d_row_ptr = nullptr;
-  int32_t* d_col = nullptr;
-  dfa::DeviceArena base;
-  // blocked P1 plan
-  bool has_plan = false;
-  std::string plan_error;
-  dfa::P1BlockTopo btopo;
-  dfa::DeviceArena plan;
-  int64_t appearances = 0;
-  bool kdof_injective = false;
-  // vector P2: per cell, rank of node b among the neighbours of node a (100 bytes) and the
-  // CSR strides (node stride s1, component stride s2 per row)
-  uint8_t* d_vp2_rank = nullptr;
-  int32_t vp2_s1 = 0;
-  bool vp2_component_major = true;
-  bool has_vp2_plan = false;
-  dfa::VP2BlockTopo vtopo;
-};
 
 namespace {
 
@@ -102,7 +44,12 @@ int guarded(F&& f) {
   }
 }
 
+// Plan parameters for assemblers created afterwards by this thread (0 = built-in default);
+// set by dfa_set_plan_defaults, e.g. from an autotuner.
+thread_local int tl_p1_rows = 0, tl_p1_threads = 0, tl_p2_vals = 0;
+
 int default_block_threads() {
+  if (tl_p1_threads) return tl_p1_threads;
   if (const char* e = std::getenv("DFA_BLOCK_THREADS")) {
     const int v = std::atoi(e);
     if (v == 256 || v == 512) return v;
@@ -111,6 +58,7 @@ int default_block_threads() {
 }
 
 int default_p2_block_vals() {
+  if (tl_p2_vals) return tl_p2_vals;
   if (const char* e = std::getenv("DFA_P2_BLOCK_VALS")) {
     const int v = std::atoi(e);
     if (v >= 1024 && v <= 16384) return v;
@@ -119,6 +67,7 @@ int default_p2_block_vals() {
 }
 
 int default_block_rows(int gdim) {
+  if (tl_p1_rows) return tl_p1_rows;
   if (const char* e = std::getenv("DFA_BLOCK_ROWS")) {
     const int v = std::atoi(e);
     if (v >= 8 && v <= 4096) return v;
@@ -132,6 +81,18 @@ extern "C" {
 
 const char* dfa_last_error(void) { return dfa::last_error(); }
 const char* dfa_version(void) { return "dolfin_amd 0.1 (gfx950)"; }
+
+int dfa_set_plan_defaults(int32_t p1_block_rows, int32_t p1_threads, int32_t p2_block_vals) {
+  if ((p1_block_rows && (p1_block_rows < 8 || p1_block_rows > 4096)) ||
+      (p1_threads && p1_threads != 256 && p1_threads != 512) ||
+      (p2_block_vals && (p2_block_vals < 1024 || p2_block_vals > 16384)))
+    return fail("plan defaults out of range (rows 8..4096, threads 256/512, P2 values 1024..16384)",
+                DFA_ERR_ARG);
+  tl_p1_rows = p1_block_rows;
+  tl_p1_threads = p1_threads;
+  tl_p2_vals = p2_block_vals;
+  return DFA_OK;
+}
 
 int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t nc,
                          const int32_t* h_cells, int32_t element, int32_t nd,

@@ -1,0 +1,74 @@
+// Internal state of a dfa_assembler handle (shared by the C ABI and the batched launcher).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace dfa {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct DeviceArena {
+  std::vector<void*> ptrs;
+  int64_t bytes = 0;
+  template <class T>
+  T* upload(const std::vector<T>& v, hipStream_t s) {
+    return upload(v.data(), v.size(), s);
+  }
+  template <class T>
+  T* upload(const T* h, size_t n, hipStream_t s) {
+    void* d = nullptr;
+    const size_t nb = std::max<size_t>(n * sizeof(T), 16);
+    hip_check(hipMalloc(&d, nb), "hipMalloc");
+    ptrs.push_back(d);
+    bytes += static_cast<int64_t>(n * sizeof(T));
+    if (n) hip_check(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    return static_cast<T*>(d);
+  }
+  void release() {
+    for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+    bytes = 0;
+  }
+};
+
+}  // namespace dfa
+
+struct dfa_assembler_s {
+  int32_t gdim = 0, element = 0, nd = 0, nvc = 0;
+  int64_t nv = 0, nc = 0, ndofs = 0, nkdofs = 0, nnz = 0;
+  bool rows_covered = true;   // every CSR row belongs to a dof of some cell
+  int32_t* d_cells = nullptr;
+  int32_t* d_cdofs = nullptr;
+  int32_t* d_kdofs = nullptr;
+  int32_t* d_row_ptr = nullptr;
+  int32_t* d_col = nullptr;
+  dfa::DeviceArena base;
+  // blocked P1 plan
+  bool has_plan = false;
+  std::string plan_error;
+  dfa::P1BlockTopo btopo;
+  dfa::DeviceArena plan;
+  int64_t appearances = 0;
+  bool kdof_injective = false;
+  // vector P2: per cell, rank of node b among the neighbours of node a (100 bytes) and the
+  // CSR strides (node stride s1, component stride s2 per row)
+  uint8_t* d_vp2_rank = nullptr;
+  int32_t vp2_s1 = 0;
+  bool vp2_component_major = true;
+  bool has_vp2_plan = false;
+  dfa::VP2BlockTopo vtopo;
+};

@@ -19,148 +19,13 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "p1_device.h"
 
 namespace dfa {
 
 namespace {
 
-constexpr int kThreads = 256;
-
-__device__ __forceinline__ void atomic_add_global(double* p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void atomic_add_lds(double* p, double v) { unsafeAtomicAdd(p, v); }
-
-// Cofactor gradients g[a] = det(J) * grad(lambda_a) and |det J| of an affine simplex.
-template <int D>
-__device__ __forceinline__ void p1_geometry(const double (&p)[D + 1][D], double (&g)[D + 1][D],
-                                            double& absdet) {
-  if constexpr (D == 3) {
-    double e1[3], e2[3], e3[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      e1[i] = p[1][i] - p[0][i];
-      e2[i] = p[2][i] - p[0][i];
-      e3[i] = p[3][i] - p[0][i];
-    }
-    // rows of adj(J) for J = [e1 e2 e3]: e2 x e3, e3 x e1, e1 x e2
-    g[1][0] = e2[1] * e3[2] - e2[2] * e3[1];
-    g[1][1] = e2[2] * e3[0] - e2[0] * e3[2];
-    g[1][2] = e2[0] * e3[1] - e2[1] * e3[0];
-    g[2][0] = e3[1] * e1[2] - e3[2] * e1[1];
-    g[2][1] = e3[2] * e1[0] - e3[0] * e1[2];
-    g[2][2] = e3[0] * e1[1] - e3[1] * e1[0];
-    g[3][0] = e1[1] * e2[2] - e1[2] * e2[1];
-    g[3][1] = e1[2] * e2[0] - e1[0] * e2[2];
-    g[3][2] = e1[0] * e2[1] - e1[1] * e2[0];
-    const double det = e1[0] * g[1][0] + e1[1] * g[1][1] + e1[2] * g[1][2];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) g[0][i] = -(g[1][i] + g[2][i] + g[3][i]);
-    absdet = fabs(det);
-  } else if constexpr (D == 2) {
-    const double e1x = p[1][0] - p[0][0], e1y = p[1][1] - p[0][1];
-    const double e2x = p[2][0] - p[0][0], e2y = p[2][1] - p[0][1];
-    g[1][0] = e2y;
-    g[1][1] = -e2x;
-    g[2][0] = -e1y;
-    g[2][1] = e1x;
-    g[0][0] = -(g[1][0] + g[2][0]);
-    g[0][1] = -(g[1][1] + g[2][1]);
-    absdet = fabs(e1x * e2y - e1y * e2x);
-  } else {
-    const double e = p[1][0] - p[0][0];
-    g[1][0] = 1.0;
-    g[0][0] = -1.0;
-    absdet = fabs(e);
-  }
-}
-
-template <int D>
-struct Fact {
-  static constexpr double value = D == 1 ? 1.0 : (D == 2 ? 2.0 : 6.0);
-};
-
-// 1/d to full FP64 accuracy without the IEEE division sequence: hardware reciprocal
-// estimate refined by two Newton steps (d is a nonzero, finite |det J|).
-__device__ __forceinline__ double recip(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
-  return fma(r, e, r);
-}
-
-// Element matrix (row-major, N x N) and vector of one P1 cell.
-template <int D, int AFORM, bool WANT_B>
-__device__ __forceinline__ void p1_element(const double (&p)[D + 1][D], const double (&kap)[D + 1],
-                                           const double (&fv)[D + 1], double (&A)[D + 1][D + 1],
-                                           double (&be)[D + 1]) {
-  constexpr int N = D + 1;
-  double g[N][D];
-  double absdet;
-  p1_geometry<D>(p, g, absdet);
-  if constexpr (AFORM == DFA_FORM_P1_POISSON) {
-    double ks = kap[0];
-#pragma unroll
-    for (int a = 1; a < N; ++a) ks += kap[a];
-    constexpr double c = 1.0 / (N * Fact<D>::value);
-    const double w = ks * c * recip(absdet);
-#pragma unroll
-    for (int a = 0; a < N; ++a)
-#pragma unroll
-      for (int b = a; b < N; ++b) {
-        double s = g[a][0] * g[b][0];
-#pragma unroll
-        for (int i = 1; i < D; ++i) s = fma(g[a][i], g[b][i], s);
-        A[a][b] = w * s;
-        A[b][a] = w * s;
-      }
-  } else if constexpr (AFORM == DFA_FORM_P1_MASS) {
-    constexpr double c = 1.0 / (Fact<D>::value * (D + 1.0) * (D + 2.0));
-    const double w = absdet * c;
-#pragma unroll
-    for (int a = 0; a < N; ++a)
-#pragma unroll
-      for (int b = 0; b < N; ++b) A[a][b] = (a == b) ? 2.0 * w : w;
-  }
-  if constexpr (WANT_B) {
-    double fs = fv[0];
-#pragma unroll
-    for (int a = 1; a < N; ++a) fs += fv[a];
-    constexpr double c = 1.0 / (Fact<D>::value * (D + 1.0) * (D + 2.0));
-    const double w = absdet * c;
-#pragma unroll
-    for (int a = 0; a < N; ++a) be[a] = w * (fv[a] + fs);
-  }
-}
-
-// SystemAssembler::apply_bc for a square form (dolfin/fem/SystemAssembler.cpp:1051-1075),
-// applied to the element tensors in local column order.
-template <int N>
-__device__ __forceinline__ void apply_bc(double (&A)[N][N], double (&be)[N], const bool (&isbc)[N],
-                                         const double (&g)[N]) {
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    if (!isbc[i]) continue;
-#pragma unroll
-    for (int j = 0; j < N; ++j) A[i][j] = 0.0;
-#pragma unroll
-    for (int r = 0; r < N; ++r) be[r] -= A[r][i] * g[i];
-#pragma unroll
-    for (int r = 0; r < N; ++r) A[r][i] = 0.0;
-    be[i] = g[i];
-    A[i][i] = 1.0;
-  }
-}
-
-__device__ __forceinline__ int find_slot(const int32_t* __restrict__ row_ptr,
-                                         const int32_t* __restrict__ col, int32_t r, int32_t c) {
-  int lo = row_ptr[r], hi = row_ptr[r + 1];
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (col[mid] <= c) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
+using namespace p1dev;
 
 // ---------------------------------------------------------------------------------------
 // Atomic scatter: thread per cell.
@@ -209,264 +74,10 @@ __global__ __launch_bounds__(kThreads) void p1_atomic_kernel(P1AtomicTopo t, P1P
   }
 }
 
-// ---------------------------------------------------------------------------------------
-// Blocked (owner-computes) scatter: workgroup per row block.
-// ---------------------------------------------------------------------------------------
-struct LdsLayout {
-  int node, f, bcv, bcf, own, vals, bv, rdof, total;
-};
-
-__host__ __device__ inline int align16(int n) { return (n + 15) & ~15; }
-
-// Node record in LDS, read with 128-bit loads. Small: coordinates and kappa (32 bytes for
-// triangles/tets; f and the Dirichlet value in separate arrays). Wide: coordinates, kappa, f,
-// Dirichlet value (48 bytes): a 12-dword stride starts consecutive records on 8 different
-// 4-bank groups, while a 32-byte stride reuses only 4 of them.
-__host__ __device__ constexpr int node_stride(int D, bool wide = false) {
-  return wide ? (D == 1 ? 4 : 6) : (D == 1 ? 2 : 4);
-}
-
-// LDS layout of one block. Strided: owned row i occupies values [i*kRowStride, +16), ranks
-// index it directly. Compact: rows packed back to back (any row length), located through a
-// per-node table.
-__host__ __device__ inline LdsLayout lds_layout(int D, int maxN, int maxR, int maxNnz, bool f,
-                                                bool bc, bool strided, bool wide) {
-  LdsLayout L;
-  int o = 0;
-  L.node = o; o += align16(maxN * node_stride(D, wide) * 8);
-  L.f = o;    o += (f && !wide) ? align16(maxN * 8) : 0;
-  L.bcv = o;  o += (bc && !wide) ? align16(maxN * 8) : 0;
-  L.bcf = o;  o += bc ? align16(maxN) : 0;
-  L.own = o;  o += strided ? 0 : align16(maxN * 4);
-  L.vals = o; o += align16((strided ? maxR * kRowStride : maxNnz) * 8);
-  L.bv = o;   o += f ? align16(maxR * 8) : 0;
-  L.rdof = o; o += align16(maxR * 4);
-  L.total = o;
-  return L;
-}
-
-template <int D>
-__device__ __forceinline__ void load_node(const double* __restrict__ rec, double (&pc)[D], double& kap) {
-  if constexpr (D == 1) {
-    const double2 r = *reinterpret_cast<const double2*>(rec);
-    pc[0] = r.x;
-    kap = r.y;
-  } else {
-    const double2 r0 = *reinterpret_cast<const double2*>(rec);
-    const double2 r1 = *reinterpret_cast<const double2*>(rec + 2);
-    pc[0] = r0.x;
-    pc[1] = r0.y;
-    if constexpr (D == 3) {
-      pc[2] = r1.x;
-      kap = r1.y;
-    } else {
-      kap = r1.x;
-    }
-  }
-}
-
-// XCD-aware block order: workgroup i runs on XCD i % 8; each XCD gets a contiguous range of
-// (spatially adjacent) blocks so that shared ghost nodes hit the same L2.
-__device__ __forceinline__ int xcd_block(int nb) {
-  const int q = nb >> 3, r = nb & 7;
-  const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
-  return xcd * q + min(xcd, r) + j;
-}
-
-// Prologue shared by the assembly and adjoint kernels: gather the block's nodes into LDS.
-template <int D, int T, bool NEED_K, bool NEED_F, bool BC, bool OWN, bool WIDE>
-__device__ __forceinline__ void gather_nodes(const P1BlockTopo& t, const double* __restrict__ x,
-                                             const double* __restrict__ kappa,
-                                             const double* __restrict__ f,
-                                             const uint8_t* __restrict__ bcm,
-                                             const double* __restrict__ bcv, int blk, double* snode,
-                                             double* sf, double* sbcv, uint8_t* sbcf, int32_t* sown) {
-  constexpr int NS = node_stride(D, WIDE);
-  const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
-  for (int i = threadIdx.x; i < nn; i += T) {
-    const int32_t v = t.node_vertex[n0 + i];
-    double rec[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) rec[k] = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; ++k) rec[k] = x[static_cast<int64_t>(v) * D + k];
-    if constexpr (NEED_K || NEED_F) {
-      const int32_t kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + i];
-      if constexpr (NEED_K) rec[D] = kappa[kd];
-      if constexpr (NEED_F) {
-        if constexpr (WIDE) rec[D + 1] = f[kd];
-        else sf[i] = f[kd];
-      }
-    }
-    if constexpr (BC) {
-      const int32_t dd = t.node_dof[n0 + i];
-      const uint8_t m = bcm[dd];
-      sbcf[i] = m;
-      const double g = m ? bcv[dd] : 0.0;
-      if constexpr (WIDE) rec[D + 2] = g;
-      else sbcv[i] = g;
-    }
-#pragma unroll
-    for (int k = 0; k < NS; k += 2)
-      *reinterpret_cast<double2*>(snode + i * NS + k) = make_double2(rec[k], rec[k + 1]);
-    if constexpr (OWN) sown[i] = -1;
-  }
-}
-
-template <int NS>
-__device__ __forceinline__ void load_rec(const double* __restrict__ p, double (&rec)[NS]) {
-#pragma unroll
-  for (int k = 0; k < NS; k += 2) {
-    const double2 r = *reinterpret_cast<const double2*>(p + k);
-    rec[k] = r.x;
-    rec[k + 1] = r.y;
-  }
-}
-
 template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE>
 __global__ __launch_bounds__(T) void p1_blocked_kernel(P1BlockTopo t, P1Pass p, LdsLayout L) {
-  constexpr int N = D + 1;
-  constexpr int NS = node_stride(D, WIDE);
-  constexpr bool WANT_A = AFORM >= 0;
-  constexpr bool NEED_K = AFORM == DFA_FORM_P1_POISSON;
   extern __shared__ __align__(16) unsigned char lds[];
-  double* snode = reinterpret_cast<double*>(lds + L.node);
-  double* sf = reinterpret_cast<double*>(lds + L.f);
-  double* sbcv = reinterpret_cast<double*>(lds + L.bcv);
-  uint8_t* sbcf = lds + L.bcf;
-  int32_t* sown = reinterpret_cast<int32_t*>(lds + L.own);
-  double* svals = reinterpret_cast<double*>(lds + L.vals);
-  double* sbv = reinterpret_cast<double*>(lds + L.bv);
-  int32_t* srdof = reinterpret_cast<int32_t*>(lds + L.rdof);
-
-  const int blk = xcd_block(t.num_blocks);
-  const int tid = threadIdx.x;
-  const bool do_A = WANT_A && p.want_A;
-
-  gather_nodes<D, T, NEED_K, WANT_B, BC, !STRIDED, WIDE>(t, p.x, p.kappa, p.f, p.bc_mask,
-                                                         p.bc_value, blk, snode, sf, sbcv, sbcf,
-                                                         sown);
-  const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
-  if (do_A) {
-    const int nz = STRIDED ? nr * kRowStride : t.blk_nnz[blk];
-    for (int i = tid; i < (nz + 1) / 2; i += T)
-      reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
-  }
-  for (int i = tid; i < nr; i += T) {
-    srdof[i] = t.row_dof[r0 + i];
-    if constexpr (WANT_B) sbv[i] = 0.0;
-  }
-  if constexpr (!STRIDED) {
-    __syncthreads();
-    for (int i = tid; i < nr; i += T) {
-      const uint32_t info = t.row_info[r0 + i];
-      sown[info & 0xffffu] = static_cast<int>(info >> 16) | (i << 16);
-    }
-  }
-  __syncthreads();
-
-  const int c0 = t.cell_ptr[blk], ncb = t.cell_ptr[blk + 1] - c0;
-  // cell records are prefetched one iteration ahead
-  uint64_t ids_n = 0, rk_n = 0;
-  if (tid < ncb) {
-    ids_n = t.cell_ids[c0 + tid];
-    if (do_A) rk_n = t.cell_rank[c0 + tid];
-  }
-  for (int k = tid; k < ncb; k += T) {
-    const uint64_t ids = ids_n, rk = rk_n;
-    if (k + T < ncb) {
-      ids_n = t.cell_ids[c0 + k + T];
-      if (do_A) rk_n = t.cell_rank[c0 + k + T];
-    }
-    int l[N];
-    double pc[N][D], kap[N], fv[N], A[N][N], be[N];
-    double gv[N];
-#pragma unroll
-    for (int a = 0; a < N; ++a) {
-      l[a] = static_cast<int>((ids >> (16 * a)) & 0xffffu);
-      if constexpr (WIDE) {
-        double rec[NS];
-        load_rec<NS>(snode + l[a] * NS, rec);
-#pragma unroll
-        for (int i = 0; i < D; ++i) pc[a][i] = rec[i];
-        kap[a] = rec[D];
-        fv[a] = WANT_B ? rec[D + 1] : 0.0;
-        gv[a] = BC ? rec[D + 2] : 0.0;
-      } else {
-        load_node<D>(snode + l[a] * NS, pc[a], kap[a]);
-        fv[a] = WANT_B ? sf[l[a]] : 0.0;
-        gv[a] = BC ? sbcv[l[a]] : 0.0;
-      }
-      be[a] = 0.0;
-    }
-    p1_element<D, AFORM, WANT_B>(pc, kap, fv, A, be);
-    if constexpr (BC) {
-      bool isbc[N];
-#pragma unroll
-      for (int a = 0; a < N; ++a) isbc[a] = sbcf[l[a]] != 0;
-      apply_bc<N>(A, be, isbc, gv);
-    }
-#pragma unroll
-    for (int a = 0; a < N; ++a) {
-      int off, row;
-      if constexpr (STRIDED) {
-        if (l[a] >= nr) continue;
-        row = l[a];
-        off = l[a] * kRowStride;
-      } else {
-        const int own = sown[l[a]];
-        if (own < 0) continue;
-        off = own & 0xffff;
-        row = own >> 16;
-      }
-      if (do_A) {
-        const uint32_t ra = static_cast<uint32_t>(rk >> (16 * a));
-#pragma unroll
-        for (int b = 0; b < N; ++b)
-          atomic_add_lds(svals + off + static_cast<int>((ra >> (4 * b)) & 15u), A[a][b]);
-      }
-      if constexpr (WANT_B) atomic_add_lds(sbv + row, be[a]);
-    }
-  }
-  __syncthreads();
-
-  if (do_A) {
-    if constexpr (STRIDED) {
-      // 16 lanes per row; consecutive rows are consecutive dofs, so a wave stores 4 adjacent
-      // CSR rows.
-      const int grp = tid >> 4, ln = tid & 15;
-      for (int i = grp; i < nr; i += T / 16) {
-        const int32_t dof = srdof[i];
-        const int base = t.csr_row_ptr[dof];
-        const int len = t.csr_row_ptr[dof + 1] - base;
-        if (ln < len) {
-          double v = svals[i * kRowStride + ln];
-          if (p.add) v += p.values[base + ln];
-          p.values[base + ln] = v;
-        }
-      }
-    } else {
-      // runs of consecutive rows: each run is one contiguous CSR range
-      const int q0 = t.run_ptr[blk], nq = t.run_ptr[blk + 1] - q0;
-      const int wave = tid >> 6, lane = tid & 63;
-      for (int q = wave; q < nq; q += T / 64) {
-        const uint64_t run = t.runs[q0 + q];
-        const int64_t gstart = static_cast<int64_t>(static_cast<uint32_t>(run));
-        const int loff = static_cast<int>((run >> 32) & 0xffffu);
-        const int len = static_cast<int>(run >> 48);
-        for (int jj = lane; jj < len; jj += 64) {
-          double v = svals[loff + jj];
-          if (p.add) v += p.values[gstart + jj];
-          p.values[gstart + jj] = v;
-        }
-      }
-    }
-  }
-  if constexpr (WANT_B)
-    for (int i = tid; i < nr; i += T) {
-      const int32_t dof = srdof[i];
-      p.b[dof] = p.add ? p.b[dof] + sbv[i] : sbv[i];
-    }
+  p1_block_body<D, AFORM, WANT_B, BC, STRIDED, T, WIDE>(t, p, L, lds, xcd_block(t.num_blocks));
 }
 
 // Adjoint of J = u^T K(kappa, x) u - F(f, x)^T u on the same blocks: every owned vertex

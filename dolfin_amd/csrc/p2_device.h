@@ -1,0 +1,427 @@
+// Device-side building blocks of the vector-P2 elasticity kernels (exact element weights,
+// node-pair contraction, the blocked-scatter block body), shared by the standalone kernels
+// and the batched persistent kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace dfa {
+namespace {
+constexpr int kThreads = 256;
+constexpr int kCells = 16;  // cells per workgroup iteration
+
+__constant__ int kEdge[6][2] = {{2, 3}, {1, 3}, {1, 2}, {0, 3}, {0, 2}, {0, 1}};
+
+// int lambda_n lambda_m lambda_r / |T| on a tetrahedron.
+__device__ __forceinline__ double bary3(int n, int m, int r) {
+  if (n == m && m == r) return 1.0 / 20.0;
+  if (n == m || m == r || n == r) return 1.0 / 60.0;
+  return 1.0 / 120.0;
+}
+
+struct CellData {
+  double D[4][3];      // grad lambda
+  double Wm[4][4], Wl[4][4];
+  double Rm[4], Rl[4]; // row sums of W
+  double Tm, Tl;       // total sums
+  double vol;
+  int32_t dof[30];
+  int32_t rbase[30];   // row_ptr[dof]
+  int32_t deg[10];     // neighbours per node (row length / 3)
+  uint8_t rank[100];
+  uint8_t bcf[30];
+  double bcv[30];
+  double be[30];
+};
+
+// Geometry and exact coefficient weights of one cell (held in LDS by the atomic kernel, in
+// registers by the blocked one).
+struct CellGeom {
+  double D[4][3];
+  double Wm[4][4], Wl[4][4];
+  double Rm[4], Rl[4];
+  double Tm, Tl;
+  double vol;
+};
+
+template <class C>
+__device__ __forceinline__ void vp2_cell_setup(const double (&p)[4][3], const double (&mu)[4],
+                                               const double (&la)[4], C& c) {
+  double e1[3], e2[3], e3[3];
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = p[1][i] - p[0][i];
+    e2[i] = p[2][i] - p[0][i];
+    e3[i] = p[3][i] - p[0][i];
+  }
+  double g[4][3];
+  g[1][0] = e2[1] * e3[2] - e2[2] * e3[1];
+  g[1][1] = e2[2] * e3[0] - e2[0] * e3[2];
+  g[1][2] = e2[0] * e3[1] - e2[1] * e3[0];
+  g[2][0] = e3[1] * e1[2] - e3[2] * e1[1];
+  g[2][1] = e3[2] * e1[0] - e3[0] * e1[2];
+  g[2][2] = e3[0] * e1[1] - e3[1] * e1[0];
+  g[3][0] = e1[1] * e2[2] - e1[2] * e2[1];
+  g[3][1] = e1[2] * e2[0] - e1[0] * e2[2];
+  g[3][2] = e1[0] * e2[1] - e1[1] * e2[0];
+  const double det = e1[0] * g[1][0] + e1[1] * g[1][1] + e1[2] * g[1][2];
+  const double inv = 1.0 / det;
+  for (int i = 0; i < 3; ++i) g[0][i] = -(g[1][i] + g[2][i] + g[3][i]);
+  for (int k = 0; k < 4; ++k)
+    for (int i = 0; i < 3; ++i) c.D[k][i] = g[k][i] * inv;
+  c.vol = fabs(det) / 6.0;
+  c.Tm = c.Tl = 0.0;
+  for (int m = 0; m < 4; ++m) {
+    c.Rm[m] = c.Rl[m] = 0.0;
+    for (int r = 0; r < 4; ++r) {
+      double wm = 0.0, wl = 0.0;
+      for (int n = 0; n < 4; ++n) {
+        const double w = bary3(n, m, r);
+        wm += mu[n] * w;
+        wl += la[n] * w;
+      }
+      c.Wm[m][r] = c.vol * wm;
+      c.Wl[m][r] = c.vol * wl;
+      c.Rm[m] += c.vol * wm;
+      c.Rl[m] += c.vol * wl;
+    }
+    c.Tm += c.Rm[m];
+    c.Tl += c.Rl[m];
+  }
+}
+
+// Factor of grad phi_a: terms (kind, index, gradient index). kind 0: u = 4 e_k - 1;
+// kind 1: u = 4 e_k.
+struct Factor {
+  int n;
+  int kind[2], idx[2], grad[2];
+};
+
+__device__ __forceinline__ Factor factor_of(int a) {
+  Factor f;
+  if (a < 4) {
+    f.n = 1;
+    f.kind[0] = 0;
+    f.idx[0] = a;
+    f.grad[0] = a;
+  } else {
+    const int k = kEdge[a - 4][0], l = kEdge[a - 4][1];
+    f.n = 2;
+    f.kind[0] = 1; f.idx[0] = l; f.grad[0] = k;
+    f.kind[1] = 1; f.idx[1] = k; f.grad[1] = l;
+  }
+  return f;
+}
+
+// u^T W u' for the two factor kinds.
+__device__ __forceinline__ double alpha(int k1, int i1, int k2, int i2, const double (&W)[4][4],
+                                        const double (&R)[4], double T) {
+  if (k1 == 0 && k2 == 0) return 16.0 * W[i1][i2] - 4.0 * R[i1] - 4.0 * R[i2] + T;
+  if (k1 == 0) return 4.0 * (4.0 * W[i1][i2] - R[i2]);
+  if (k2 == 0) return 4.0 * (4.0 * W[i1][i2] - R[i1]);
+  return 16.0 * W[i1][i2];
+}
+
+// 3x3 block K[(i,a),(j,b)] of the elasticity element matrix.
+template <class C>
+__device__ __forceinline__ void vp2_block(const C& c, int a, int b, double (&K)[3][3]) {
+  const Factor fa = factor_of(a), fb = factor_of(b);
+  double Sm[3][3] = {}, Sl[3][3] = {};
+  for (int t = 0; t < fa.n; ++t)
+    for (int s = 0; s < fb.n; ++s) {
+      const double am = alpha(fa.kind[t], fa.idx[t], fb.kind[s], fb.idx[s], c.Wm, c.Rm, c.Tm);
+      const double al = alpha(fa.kind[t], fa.idx[t], fb.kind[s], fb.idx[s], c.Wl, c.Rl, c.Tl);
+      const double* wa = c.D[fa.grad[t]];
+      const double* wb = c.D[fb.grad[s]];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const double o = wa[p] * wb[q];
+          Sm[p][q] += am * o;
+          Sl[p][q] += al * o;
+        }
+    }
+  const double tr = Sm[0][0] + Sm[1][1] + Sm[2][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) K[i][j] = (i == j ? tr : 0.0) + Sm[j][i] + Sl[i][j];
+}
+
+}  // namespace
+
+namespace p2dev {
+
+// Register-only evaluation of one node's row blocks (no runtime-indexed register arrays).
+// grad phi_a = sum_t (u_t . lambda) w_t with t = 0, 1:
+//   vertex a:     u_0 = 4 e_a - 1, w_0 = grad lambda_a, w_1 = 0;
+//   edge (k, l):  u_0 = 4 e_l, w_0 = grad lambda_k;  u_1 = 4 e_k, w_1 = grad lambda_l.
+// v_t = u_t^T W (4-vectors, per coefficient field) are formed once per work item; for a
+// compile-time node b the scalars alpha_ts = v_t . u'_s need only constant indices.
+// ---------------------------------------------------------------------------------------
+struct NodeFactors {
+  double w[2][3];
+  double vm[2][4], vl[2][4];
+  double sm[2], sl[2];
+};
+
+__device__ __forceinline__ double sel4(int i, double a0, double a1, double a2, double a3) {
+  return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
+}
+
+template <class C>
+__device__ __forceinline__ void node_factors(const C& c, int a, NodeFactors& f) {
+  // edge e = a - 4 joins (k, l): k = 2,1,1,0,0,0 and l = 3,3,2,3,2,1 packed two bits each
+  const int e = a - 4;
+  const bool vert = a < 4;
+  const int k = vert ? a : (22 >> (2 * e)) & 3;
+  const int l = vert ? 0 : (1775 >> (2 * e)) & 3;
+  const int g0 = k, g1 = l;            // gradient directions
+  const int i0 = vert ? a : l, i1 = k; // W rows
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    f.w[0][q] = sel4(g0, c.D[0][q], c.D[1][q], c.D[2][q], c.D[3][q]);
+    f.w[1][q] = vert ? 0.0 : sel4(g1, c.D[0][q], c.D[1][q], c.D[2][q], c.D[3][q]);
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const double wm0 = sel4(i0, c.Wm[0][m], c.Wm[1][m], c.Wm[2][m], c.Wm[3][m]);
+    const double wl0 = sel4(i0, c.Wl[0][m], c.Wl[1][m], c.Wl[2][m], c.Wl[3][m]);
+    const double wm1 = sel4(i1, c.Wm[0][m], c.Wm[1][m], c.Wm[2][m], c.Wm[3][m]);
+    const double wl1 = sel4(i1, c.Wl[0][m], c.Wl[1][m], c.Wl[2][m], c.Wl[3][m]);
+    f.vm[0][m] = 4.0 * wm0 - (vert ? c.Rm[m] : 0.0);
+    f.vl[0][m] = 4.0 * wl0 - (vert ? c.Rl[m] : 0.0);
+    f.vm[1][m] = 4.0 * wm1;
+    f.vl[1][m] = 4.0 * wl1;
+  }
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    f.sm[t] = f.vm[t][0] + f.vm[t][1] + f.vm[t][2] + f.vm[t][3];
+    f.sl[t] = f.vl[t][0] + f.vl[t][1] + f.vl[t][2] + f.vl[t][3];
+  }
+}
+
+// K[(i,a),(j,B)] for a compile-time node B.
+template <int B, class C>
+__device__ __forceinline__ void node_block(const C& c, const NodeFactors& f, double (&K)[3][3]) {
+  constexpr bool VB = B < 4;
+  constexpr int EK = VB ? B : (22 >> (2 * (B - 4))) & 3;
+  constexpr int EL = VB ? 0 : (1775 >> (2 * (B - 4))) & 3;
+  // factors of B: s = 0 (idx, grad) = VB ? (B, B) : (EL, EK); s = 1 (edge only): (EK, EL)
+  constexpr int NS = VB ? 1 : 2;
+  constexpr int IDX0 = VB ? B : EL, GRD0 = VB ? B : EK;
+  constexpr int IDX1 = EK, GRD1 = EL;
+  double Sm[3][3], Sl[3][3];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Sm[p][q] = Sl[p][q] = 0.0;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int idx = s == 0 ? IDX0 : IDX1;
+      const int grd = s == 0 ? GRD0 : GRD1;
+      const double am = VB ? 4.0 * f.vm[t][idx] - f.sm[t] : 4.0 * f.vm[t][idx];
+      const double al = VB ? 4.0 * f.vl[t][idx] - f.sl[t] : 4.0 * f.vl[t][idx];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const double o = f.w[t][p] * c.D[grd][q];
+          Sm[p][q] = fma(am, o, Sm[p][q]);
+          Sl[p][q] = fma(al, o, Sl[p][q]);
+        }
+    }
+  }
+  const double tr = Sm[0][0] + Sm[1][1] + Sm[2][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) K[i][j] = (i == j ? tr : 0.0) + Sm[j][i] + Sl[i][j];
+}
+
+constexpr int kBT = 256;
+
+struct VP2Lds {
+  int vert, vals, info, rows, bacc, total;
+};
+
+__host__ __device__ inline int al16(int n) { return (n + 15) & ~15; }
+
+__host__ __device__ inline VP2Lds vp2_lds(int maxV, int maxO, int maxVals, bool b) {
+  VP2Lds L;
+  int o = 0;
+  L.vert = o; o += al16(maxV * 6 * 8);
+  L.vals = o; o += al16(maxVals * 8);
+  L.info = o; o += al16(maxO * 4);
+  L.rows = o; o += al16(maxO * 3 * 4);
+  L.bacc = o; o += b ? al16(maxO * 3 * 8) : 0;
+  L.total = o;
+  return L;
+}
+
+
+// One node block of the blocked vector-P2 scatter (standalone kernel or batched work unit).
+template <bool WANT_A, bool WANT_B, bool BC>
+__device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2Pass& p,
+                                               const VP2Lds& L, unsigned char* lds, int blk) {
+  double* svert = reinterpret_cast<double*>(lds + L.vert);
+  double* svals = reinterpret_cast<double*>(lds + L.vals);
+  uint32_t* sinfo = reinterpret_cast<uint32_t*>(lds + L.info);
+  int32_t* srows = reinterpret_cast<int32_t*>(lds + L.rows);
+  double* sbacc = reinterpret_cast<double*>(lds + L.bacc);
+  const int tid = threadIdx.x;
+
+  const int v0 = t.vert_ptr[blk], nv = t.vert_ptr[blk + 1] - v0;
+  for (int i = tid; i < nv; i += kBT) {
+    const int64_t v = t.vert_id[v0 + i];
+    const int32_t kd = t.vert_kdof[v0 + i];
+    double* rec = svert + i * 6;
+    *reinterpret_cast<double2*>(rec) = make_double2(p.x[v * 3], p.x[v * 3 + 1]);
+    const bool need_c = WANT_A || BC;  // the body force alone needs no coefficients
+    *reinterpret_cast<double2*>(rec + 2) = make_double2(p.x[v * 3 + 2], need_c ? p.mu[kd] : 0.0);
+    *reinterpret_cast<double2*>(rec + 4) = make_double2(need_c ? p.lmbda[kd] : 0.0, 0.0);
+  }
+  const int o0 = t.own_ptr[blk], no = t.own_ptr[blk + 1] - o0;
+  for (int i = tid; i < no; i += kBT) {
+    sinfo[i] = t.own_info[o0 + i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) srows[3 * i + k] = t.own_rows[3 * (o0 + i) + k];
+  }
+  if (WANT_A) {
+    const int nvals = t.blk_vals[blk];
+    for (int i = tid; i < (nvals + 1) / 2; i += kBT)
+      reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
+  }
+  if (WANT_B)
+    for (int i = tid; i < 3 * no; i += kBT) sbacc[i] = 0.0;
+  __syncthreads();
+
+  // Each work item is split over two threads (node pairs b = 0..4 and b = 5..9): blocks hold
+  // ~130 items, so one item per thread would leave half of the workgroup idle.
+  const int t0 = t.task_ptr[blk], nt = t.task_ptr[blk + 1] - t0;
+  for (int kk = tid; kk < 2 * nt; kk += kBT) {
+    const int half = kk >= nt ? 1 : 0;  // wavefronts stay on one half
+    const int k = kk - half * nt;
+    const uint64_t w0 = t.task[3 * (t0 + k)];
+    const uint64_t w1 = t.task[3 * (t0 + k) + 1];
+    const uint64_t w2 = t.task[3 * (t0 + k) + 2];
+    const int o = static_cast<int>(w1 & 0xffffu);
+    const int a = static_cast<int>((w1 >> 16) & 0xffu);
+    double pc[4][3], mu[4], la[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const double* rec = svert + static_cast<int>((w0 >> (16 * v)) & 0xffffu) * 6;
+      const double2 r0 = *reinterpret_cast<const double2*>(rec);
+      const double2 r1 = *reinterpret_cast<const double2*>(rec + 2);
+      const double2 r2 = *reinterpret_cast<const double2*>(rec + 4);
+      pc[v][0] = r0.x;
+      pc[v][1] = r0.y;
+      pc[v][2] = r1.x;
+      mu[v] = r1.y;
+      la[v] = r2.x;
+    }
+    CellGeom c;
+    vp2_cell_setup(pc, mu, la, c);
+    const uint32_t info = sinfo[o];
+    const int off = static_cast<int>(info & 0xffffu);
+    const int deg = static_cast<int>(info >> 16);
+    double be[3] = {0.0, 0.0, 0.0};
+    if (WANT_B && half == 0) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) be[i] = p.f3[i] * (a < 4 ? -c.vol / 20.0 : c.vol / 5.0);
+    }
+    // Dirichlet data of the cell (rows of node a, columns of every node b)
+    uint32_t bcrow = 0, bccol = 0;  // bit i: row (i,a); bit j*10+b: column (j,b)
+    double growv[3] = {0, 0, 0};
+    const int32_t* cdofs = nullptr;
+    if constexpr (BC) {
+      cdofs = t.cell_dofs + static_cast<int64_t>(t.task_cell[t0 + k]) * 30;
+      for (int e = 0; e < 30; ++e)
+        if (p.bc_mask[cdofs[e]]) bccol |= 1u << e;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (bccol >> (i * 10 + a) & 1u) {
+          bcrow |= 1u << i;
+          growv[i] = p.bc_value[cdofs[i * 10 + a]];
+        }
+    }
+    if constexpr (WANT_A || BC) {
+      NodeFactors fa;
+      node_factors(c, a, fa);
+      auto one = [&](auto bconst) {
+        constexpr int b = decltype(bconst)::value;
+        double K[3][3];
+        node_block<b>(c, fa, K);
+        if constexpr (BC) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              const bool cb = (bccol >> (j * 10 + b)) & 1u;
+              if (WANT_B && cb && !(bcrow >> i & 1u))
+                be[i] -= K[i][j] * p.bc_value[cdofs[j * 10 + b]];
+              if (cb || (bcrow >> i & 1u)) K[i][j] = (i == j && a == b) ? 1.0 : 0.0;
+            }
+        }
+        if constexpr (WANT_A) {
+          const int rk = static_cast<int>(b < 5 ? (w1 >> (24 + 8 * b)) & 0xffu : (w2 >> (8 * (b - 5))) & 0xffu);
+          const int s2 = t.component_major ? deg : 1;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            double* row = svals + off + i * 3 * deg + (t.component_major ? rk : 3 * rk);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) unsafeAtomicAdd(row + j * s2, K[i][j]);
+          }
+        }
+      };
+      if (half == 0) {
+        one(std::integral_constant<int, 0>{});
+        one(std::integral_constant<int, 1>{});
+        one(std::integral_constant<int, 2>{});
+        one(std::integral_constant<int, 3>{});
+        one(std::integral_constant<int, 4>{});
+      } else {
+        one(std::integral_constant<int, 5>{});
+        one(std::integral_constant<int, 6>{});
+        one(std::integral_constant<int, 7>{});
+        one(std::integral_constant<int, 8>{});
+        one(std::integral_constant<int, 9>{});
+      }
+    }
+    if constexpr (WANT_B) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        // a Dirichlet row takes its value once (from the first half), never the lifting
+        const double v = (BC && (bcrow >> i & 1u)) ? (half == 0 ? growv[i] : 0.0) : be[i];
+        unsafeAtomicAdd(sbacc + 3 * o + i, v);
+      }
+    }
+  }
+  __syncthreads();
+
+  // write-out: each owned node's 3 rows are contiguous CSR ranges
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int i = wave; i < 3 * no; i += kBT / 64) {
+    const int o = i / 3, comp = i % 3;
+    const uint32_t info = sinfo[o];
+    const int off = static_cast<int>(info & 0xffffu) + comp * 3 * static_cast<int>(info >> 16);
+    const int len = 3 * static_cast<int>(info >> 16);
+    const int32_t row = srows[i];
+    if (WANT_A) {
+      const int64_t base = t.row_ptr[row];
+      for (int j = lane; j < len; j += 64) p.values[base + j] = svals[off + j];
+    }
+    if (WANT_B && lane == 0) p.b[row] = sbacc[i];
+  }
+}
+
+
+}  // namespace p2dev
+}  // namespace dfa

@@ -139,6 +139,30 @@ int dfa_adjoint(dfa_assembler h, int32_t a_form, const double* d_x, const double
                 const double* d_c1, const double* d_f, const double* h_f3, const double* d_u,
                 double* d_J, double* d_dJdc0, double* d_dJdc1, double* d_dJdx, void* stream);
 
+/* Plan parameters for assemblers created afterwards by the calling thread (0 = built-in
+   default): P1 rows per block, P1 workgroup size (256 or 512), vector-P2 LDS values per
+   block. Used by the autotuner; no counterpart in the reference. */
+int dfa_set_plan_defaults(int32_t p1_block_rows, int32_t p1_threads, int32_t p2_block_vals);
+
+/*
+ * Batched assembly (no counterpart in the reference, which assembles one form per
+ * Assembler::assemble call): n independent assemblers, each with one bilinear form
+ * (DFA_FORM_P1_POISSON on tetrahedra or DFA_FORM_VP2_ELASTICITY), assembled by ONE
+ * persistent kernel whose launch is captured once in a hipGraph. Every member needs its
+ * blocked plan (built at creation when coordinates are given). d_x, d_c0, d_c1 (lmbda, P2
+ * only) and d_values are per member and are read/written at every launch, so coefficients
+ * may change between launches. grid_per_cu limits resident workgroups per CU (0 = maximum).
+ */
+typedef struct dfa_batch_s* dfa_batch;
+int dfa_batch_create(dfa_batch* out, int32_t n, const dfa_assembler* hs, const int32_t* forms,
+                     const double* const* d_x, const double* const* d_c0,
+                     const double* const* d_c1, double* const* d_values, int32_t grid_per_cu,
+                     void* stream);
+/* Replay the captured launch on `stream`. */
+int dfa_batch_launch(dfa_batch b, void* stream);
+int dfa_batch_info(dfa_batch b, int64_t* units, int32_t* grid, int32_t* lds_bytes, int64_t* cells);
+int dfa_batch_destroy(dfa_batch b);
+
 /* Introspection for benchmarks: algorithmic bytes of one matrix assembly (the compulsory
    traffic: connectivity, coordinates, coefficients, CSR values written once, col/row_ptr
    read once) and the bytes the plan of the chosen strategy actually stores. */
