@@ -42,6 +42,23 @@ def _compile(src):
     return obj
 
 
+EXAMPLE_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "system_assembler_example.cpp")
+EXAMPLE_BIN = os.path.join(os.path.dirname(HERE), "tests", "_build", "cpp_system_assembler")
+
+
+def build_example():
+    """C++ host program against the header-only adapter, linked to the in-tree library."""
+    os.makedirs(os.path.dirname(EXAMPLE_BIN), exist_ok=True)
+    # host-only C++ against the HIP runtime API (the adapter launches nothing itself)
+    cmd = ["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           EXAMPLE_SRC, "-o", EXAMPLE_BIN, f"-L{HERE}", "-ldolfin_amd", "-L/opt/rocm/lib",
+           "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../dolfin_amd", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"example build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return EXAMPLE_BIN
+
+
 def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -53,8 +70,9 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_example()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", EXAMPLE_BIN)
     return LIB
 
 

@@ -1,0 +1,247 @@
+// C++ host adapter over the dolfin_amd C ABI (include/dolfin_amd.h), header-only.
+//
+// Mirrors the DOLFIN 2019.1 assembler interface for the cell-integral path so that a
+// maintainer can route dolfin::Assembler::assemble(GenericTensor&, const Form&)
+// (dolfin/fem/Assembler.h:66) and SystemAssembler::assemble(A, b) (SystemAssembler.h:63-84)
+// to the GPU:
+//   HipAssembler         add_values / finalize_tensor / keep_diagonal flags
+//                        (dolfin/fem/AssemblerBase.h:46-67), assemble(tensor, form)
+//   HipSystemAssembler   (a, L, Dirichlet values) -> assemble(A, b) / (A) / (b)
+//   DeviceCSRMatrix      the GenericMatrix subset the assembler uses: empty, size, nnz, zero,
+//                        apply, norm("frobenius"), data() (dolfin/la/EigenMatrix.cpp:77-105,
+//                        180-186, 341-355, 417-420)
+//   DeviceVector         GenericVector subset: size, zero, apply, norm("l2"), get_local
+// Errors from the library are rethrown as std::runtime_error with the library's message,
+// like dolfin_error (dolfin/log/Logger.cpp:142-185).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../dolfin_amd.h"
+
+namespace dolfin_amd {
+
+inline void check(int rc) {
+  if (rc != DFA_OK) throw std::runtime_error(std::string("dolfin_amd: ") + dfa_last_error());
+}
+
+inline void hip(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Owned device allocation.
+template <class T>
+class DeviceArray {
+ public:
+  DeviceArray() = default;
+  explicit DeviceArray(size_t n) { resize(n); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  ~DeviceArray() { release(); }
+  void resize(size_t n) {
+    release();
+    n_ = n;
+    if (n) hip(hipMalloc(&p_, n * sizeof(T)), "hipMalloc");
+  }
+  void upload(const T* h, size_t n, hipStream_t s = nullptr) {
+    if (n != n_) resize(n);
+    if (n) hip(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+  }
+  std::vector<T> download(hipStream_t s = nullptr) const {
+    std::vector<T> h(n_);
+    if (n_) hip(hipMemcpyAsync(h.data(), p_, n_ * sizeof(T), hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return h;
+  }
+  T* get() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// One (mesh, function space) pair: the engine handle with the CSR structure and plans.
+// The caller passes what DOLFIN's Mesh / DofMap hold: Mesh::cells(), DofMap::cell_dofs
+// (flattened), the coefficient space's cell dofs and the coordinates (used for the work
+// partition only).
+class Space {
+ public:
+  Space(int32_t gdim, int64_t num_vertices, const std::vector<int32_t>& cells, int32_t element,
+        int32_t ndofs_cell, const std::vector<int32_t>& cell_dofs, int64_t num_dofs,
+        const std::vector<int32_t>& coeff_dofs, int64_t num_coeff_dofs,
+        const std::vector<double>& x, bool keep_diagonal = false, hipStream_t s = nullptr)
+      : gdim_(gdim), element_(element), num_vertices_(num_vertices) {
+    const int64_t nc = static_cast<int64_t>(cells.size()) / (gdim + 1);
+    dfa_assembler h = nullptr;
+    check(dfa_assembler_create(&h, gdim, num_vertices, nc, cells.data(), element, ndofs_cell,
+                               cell_dofs.data(), num_dofs, coeff_dofs.data(), num_coeff_dofs,
+                               x.data(), keep_diagonal ? 1 : 0, s));
+    h_.reset(h, [](dfa_assembler p) { dfa_assembler_destroy(p); });
+    check(dfa_csr_dims(h, &num_rows_, &nnz_));
+  }
+  dfa_assembler handle() const { return h_.get(); }
+  int64_t num_rows() const { return num_rows_; }
+  int64_t nnz() const { return nnz_; }
+  int32_t gdim() const { return gdim_; }
+  int32_t element() const { return element_; }
+  int64_t num_vertices() const { return num_vertices_; }
+
+ private:
+  std::shared_ptr<dfa_assembler_s> h_;
+  int32_t gdim_, element_;
+  int64_t num_vertices_;
+  int64_t num_rows_ = 0, nnz_ = 0;
+};
+
+// A registered cell-integral form: its kind, space and device coefficient vectors (the
+// counterpart of dolfin::Form with an FFC-generated ufc::form, dolfin/fem/Form.h:85).
+struct Form {
+  std::shared_ptr<const Space> space;
+  int32_t kind = -1;              // DFA_FORM_*
+  int32_t rank = 2;               // 2 bilinear, 1 linear
+  const double* x = nullptr;      // device coordinates [num_vertices x gdim]
+  const double* c0 = nullptr;     // kappa / f / mu
+  const double* c1 = nullptr;     // lmbda
+  double f3[3] = {0, 0, 0};       // constant body force
+};
+
+class DeviceCSRMatrix {
+ public:
+  bool empty() const { return space_ == nullptr; }
+  void init(std::shared_ptr<const Space> space) {
+    space_ = std::move(space);
+    values_.resize(static_cast<size_t>(space_->nnz()));
+    zero();
+  }
+  const Space* space() const { return space_.get(); }
+  int64_t size(int) const { return space_ ? space_->num_rows() : 0; }
+  int64_t nnz() const { return space_ ? space_->nnz() : 0; }
+  void zero(hipStream_t s = nullptr) {
+    if (values_.size()) hip(hipMemsetAsync(values_.get(), 0, values_.size() * sizeof(double), s), "hipMemsetAsync");
+  }
+  void apply(const std::string&) {}
+  double* values() { return values_.get(); }
+  double norm(const std::string& type) const {
+    if (type != "frobenius") throw std::runtime_error("only the Frobenius norm is provided");
+    double s = 0.0;
+    for (double v : values_.download()) s += v * v;
+    return std::sqrt(s);
+  }
+  // CSR export (EigenMatrix::data): row_ptr, col, values on the host.
+  void data(std::vector<int32_t>& row_ptr, std::vector<int32_t>& col, std::vector<double>& vals) const {
+    DeviceArray<int32_t> rp(static_cast<size_t>(space_->num_rows() + 1)), cc(static_cast<size_t>(space_->nnz()));
+    check(dfa_csr_structure(space_->handle(), rp.get(), cc.get(), nullptr));
+    row_ptr = rp.download();
+    col = cc.download();
+    vals = values_.download();
+  }
+
+ private:
+  std::shared_ptr<const Space> space_;
+  DeviceArray<double> values_;
+};
+
+class DeviceVector {
+ public:
+  bool empty() const { return values_.size() == 0; }
+  void init(int64_t n) {
+    values_.resize(static_cast<size_t>(n));
+    zero();
+  }
+  int64_t size() const { return static_cast<int64_t>(values_.size()); }
+  void zero(hipStream_t s = nullptr) {
+    if (values_.size()) hip(hipMemsetAsync(values_.get(), 0, values_.size() * sizeof(double), s), "hipMemsetAsync");
+  }
+  void apply(const std::string&) {}
+  double* values() { return values_.get(); }
+  std::vector<double> get_local() const { return values_.download(); }
+  double norm(const std::string& type) const {
+    if (type != "l2") throw std::runtime_error("only the l2 norm is provided");
+    double s = 0.0;
+    for (double v : values_.download()) s += v * v;
+    return std::sqrt(s);
+  }
+
+ private:
+  DeviceArray<double> values_;
+};
+
+// dolfin::AssemblerBase flags (dolfin/fem/AssemblerBase.h:46-67).
+struct AssemblerFlags {
+  bool add_values = false;
+  bool finalize_tensor = true;
+  bool keep_diagonal = false;  // fixed when the Space is created
+};
+
+class HipAssembler : public AssemblerFlags {
+ public:
+  explicit HipAssembler(int32_t scatter = DFA_SCATTER_AUTO, hipStream_t s = nullptr)
+      : scatter_(scatter), stream_(s) {}
+
+  // Assembler::assemble for a rank-2 form: structure on first use, then zero + cell loop.
+  void assemble(DeviceCSRMatrix& A, const Form& a) {
+    if (a.rank != 2) throw std::runtime_error("matrix assembly needs a bilinear form");
+    if (A.empty() || A.space() != a.space.get()) A.init(a.space);
+    check(dfa_assemble_matrix(a.space->handle(), a.kind, a.x, a.c0, a.c1, A.values(), scatter_,
+                              add_values ? DFA_ADD_VALUES : 0, stream_));
+    if (finalize_tensor) A.apply("add");
+  }
+
+  void assemble(DeviceVector& b, const Form& L) {
+    if (L.rank != 1) throw std::runtime_error("vector assembly needs a linear form");
+    if (b.empty() || b.size() != L.space->num_rows()) b.init(L.space->num_rows());
+    check(dfa_assemble_vector(L.space->handle(), L.kind, L.x, L.c0, L.f3, b.values(), scatter_,
+                              add_values ? DFA_ADD_VALUES : 0, stream_));
+    if (finalize_tensor) b.apply("add");
+  }
+
+ private:
+  int32_t scatter_;
+  hipStream_t stream_;
+};
+
+// SystemAssembler with one set of Dirichlet values (bc_mask/bc_value on the device, one
+// entry per dof), applied per cell as in SystemAssembler::apply_bc.
+class HipSystemAssembler {
+ public:
+  HipSystemAssembler(Form a, Form L, const uint8_t* d_bc_mask = nullptr,
+                     const double* d_bc_value = nullptr, int32_t scatter = DFA_SCATTER_AUTO,
+                     hipStream_t s = nullptr)
+      : a_(std::move(a)), L_(std::move(L)), mask_(d_bc_mask), value_(d_bc_value),
+        scatter_(scatter), stream_(s) {
+    if (a_.space != L_.space) throw std::runtime_error("a and L must share the function space");
+  }
+
+  void assemble(DeviceCSRMatrix& A, DeviceVector& b) { run(&A, &b); }
+  void assemble(DeviceCSRMatrix& A) { run(&A, nullptr); }
+  void assemble(DeviceVector& b) { run(nullptr, &b); }
+
+ private:
+  void run(DeviceCSRMatrix* A, DeviceVector* b) {
+    if (A && (A->empty() || A->space() != a_.space.get())) A->init(a_.space);
+    if (b && (b->empty() || b->size() != a_.space->num_rows())) b->init(a_.space->num_rows());
+    check(dfa_assemble_system(a_.space->handle(), a_.kind, L_.kind, a_.x, a_.c0, a_.c1, L_.c0,
+                              L_.f3, mask_, value_, A ? A->values() : nullptr,
+                              b ? b->values() : nullptr, scatter_, 0, stream_));
+  }
+  Form a_, L_;
+  const uint8_t* mask_;
+  const double* value_;
+  int32_t scatter_;
+  hipStream_t stream_;
+};
+
+}  // namespace dolfin_amd

@@ -1,0 +1,105 @@
+// C++ host program using the header-only adapter (include/dolfin_amd/HipAssembler.hpp) the way
+// DOLFIN's own C++ would: build a mesh and dofmap on the host, create the Space once, then
+// SystemAssembler / Assembler calls. Checks the reference's known answers:
+//   P1 Poisson + DirichletBC(1) on UnitCubeMesh(4): |A|_F = 96.847818767384,
+//   |b|_2 = 96.564760289080 (python/test/unit/fem/test_system_assembler.py:73-91);
+//   P1 Poisson on UnitIntervalMesh(48): |A|_F = 811.75365721381274, |b| = 1.43583841167606474
+//   (python/test/unit/fem/test_assembler.py:39-55).
+// Exit code 0 when every check holds.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <vector>
+
+#include "../../include/dolfin_amd/HipAssembler.hpp"
+
+using namespace dolfin_amd;
+
+static void box_mesh(int n, std::vector<double>& x, std::vector<int32_t>& cells) {
+  for (int iz = 0; iz <= n; ++iz)
+    for (int iy = 0; iy <= n; ++iy)
+      for (int ix = 0; ix <= n; ++ix) {
+        x.push_back(static_cast<double>(ix) / n);
+        x.push_back(static_cast<double>(iy) / n);
+        x.push_back(static_cast<double>(iz) / n);
+      }
+  for (int iz = 0; iz < n; ++iz)
+    for (int iy = 0; iy < n; ++iy)
+      for (int ix = 0; ix < n; ++ix) {
+        const int v0 = iz * (n + 1) * (n + 1) + iy * (n + 1) + ix, v1 = v0 + 1, v2 = v0 + n + 1,
+                  v3 = v1 + n + 1, v4 = v0 + (n + 1) * (n + 1), v5 = v1 + (n + 1) * (n + 1),
+                  v6 = v2 + (n + 1) * (n + 1), v7 = v3 + (n + 1) * (n + 1);
+        const int t[6][4] = {{v0, v1, v3, v7}, {v0, v1, v7, v5}, {v0, v5, v7, v4},
+                             {v0, v3, v2, v7}, {v0, v6, v4, v7}, {v0, v2, v6, v7}};
+        for (auto& c : t) {
+          int s[4] = {c[0], c[1], c[2], c[3]};
+          std::sort(s, s + 4);
+          cells.insert(cells.end(), s, s + 4);
+        }
+      }
+}
+
+static bool close(double a, double b) { return std::fabs(a - b) < 1e-10; }
+
+int main() {
+  bool ok = true;
+  {
+    std::vector<double> x;
+    std::vector<int32_t> cells;
+    box_mesh(4, x, cells);
+    const int64_t nv = static_cast<int64_t>(x.size()) / 3;
+    auto V = std::make_shared<const Space>(3, nv, cells, DFA_ELEMENT_P1, 4, cells, nv, cells, nv, x);
+    DeviceArray<double> dx, one, ten, g;
+    DeviceArray<uint8_t> mask;
+    std::vector<double> h1(nv, 1.0), h10(nv, 10.0), hg(nv, 0.0);
+    std::vector<uint8_t> hm(nv, 0);
+    for (int64_t v = 0; v < nv; ++v)
+      for (int q = 0; q < 3; ++q)
+        if (x[3 * v + q] == 0.0 || x[3 * v + q] == 1.0) {
+          hm[v] = 1;
+          hg[v] = 1.0;
+        }
+    dx.upload(x.data(), x.size());
+    one.upload(h1.data(), h1.size());
+    ten.upload(h10.data(), h10.size());
+    g.upload(hg.data(), hg.size());
+    mask.upload(hm.data(), hm.size());
+    Form a{V, DFA_FORM_P1_POISSON, 2, dx.get(), one.get()};
+    Form L{V, DFA_FORM_P1_LOAD, 1, dx.get(), ten.get()};
+    DeviceCSRMatrix A;
+    DeviceVector b;
+    HipSystemAssembler sys(a, L, mask.get(), g.get());
+    sys.assemble(A, b);
+    const double nA = A.norm("frobenius"), nb = b.norm("l2");
+    std::printf("UnitCubeMesh(4) P1 + bc: |A| = %.12f |b| = %.12f\n", nA, nb);
+    ok &= close(nA, 96.847818767384) && close(nb, 96.564760289080);
+    sys.assemble(A);  // re-assembly of A only
+    ok &= close(A.norm("frobenius"), 96.847818767384);
+  }
+  {
+    const int n = 48;
+    std::vector<double> x;
+    std::vector<int32_t> cells;
+    for (int i = 0; i <= n; ++i) x.push_back(static_cast<double>(i) / n);
+    for (int i = 0; i < n; ++i) {
+      cells.push_back(i);
+      cells.push_back(i + 1);
+    }
+    auto V = std::make_shared<const Space>(1, n + 1, cells, DFA_ELEMENT_P1, 2, cells, n + 1, cells, n + 1, x);
+    DeviceArray<double> dx, one, ten;
+    std::vector<double> h1(n + 1, 1.0), h10(n + 1, 10.0);
+    dx.upload(x.data(), x.size());
+    one.upload(h1.data(), h1.size());
+    ten.upload(h10.data(), h10.size());
+    HipAssembler assembler;
+    DeviceCSRMatrix A;
+    DeviceVector b;
+    assembler.assemble(A, Form{V, DFA_FORM_P1_POISSON, 2, dx.get(), one.get()});
+    assembler.assemble(b, Form{V, DFA_FORM_P1_LOAD, 1, dx.get(), ten.get()});
+    std::printf("UnitIntervalMesh(48) P1: |A| = %.14f |b| = %.17f\n", A.norm("frobenius"), b.norm("l2"));
+    ok &= close(A.norm("frobenius"), 811.75365721381274) && close(b.norm("l2"), 1.43583841167606474);
+  }
+  std::printf(ok ? "adapter checks passed\n" : "adapter checks FAILED\n");
+  return ok ? 0 : 1;
+}

@@ -194,3 +194,9 @@ def test_vp2_blocked_plan_walk(oracle, plan_lib, n, block, vals):
     got = out[: col.shape[0]]
     assert np.linalg.norm(got - ref) <= 1e-13 * np.linalg.norm(ref)
     assert stats[3] <= vals
+
+
+def test_cpp_adapter_example_is_built():
+    # built by dolfin_amd.build (host C++ against include/dolfin_amd/HipAssembler.hpp)
+    path = os.path.join(ROOT, "tests", "_build", "cpp_system_assembler")
+    assert os.path.exists(path) and os.access(path, os.X_OK)
