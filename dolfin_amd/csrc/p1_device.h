@@ -266,7 +266,7 @@ __device__ __forceinline__ void load_rec(const double* __restrict__ p, double (&
 
 // One row block of the blocked scatter (the workgroup's whole job in the standalone kernel,
 // one work unit of the batched persistent kernel).
-template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE>
+template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE, int U = 1>
 __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass& p,
                                               const LdsLayout& L, unsigned char* lds, int blk) {
   constexpr int N = D + 1;
@@ -308,66 +308,82 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
   __syncthreads();
 
   const int c0 = t.cell_ptr[blk], ncb = t.cell_ptr[blk + 1] - c0;
-  // cell records are prefetched one iteration ahead
-  uint64_t ids_n = 0, rk_n = 0;
-  if (tid < ncb) {
-    ids_n = t.cell_ids[c0 + tid];
-    if (do_A) rk_n = t.cell_rank[c0 + tid];
+  // U cell appearances per thread and iteration: all their LDS loads are issued before any
+  // arithmetic (latency hiding at low occupancy); records are prefetched one iteration ahead.
+  uint64_t ids_n[U], rk_n[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int kk = tid + u * T;
+    ids_n[u] = kk < ncb ? t.cell_ids[c0 + kk] : 0;
+    rk_n[u] = (do_A && kk < ncb) ? t.cell_rank[c0 + kk] : 0;
   }
-  for (int k = tid; k < ncb; k += T) {
-    const uint64_t ids = ids_n, rk = rk_n;
-    if (k + T < ncb) {
-      ids_n = t.cell_ids[c0 + k + T];
-      if (do_A) rk_n = t.cell_rank[c0 + k + T];
-    }
-    int l[N];
-    double pc[N][D], kap[N], fv[N], A[N][N], be[N];
-    double gv[N];
+  for (int k = tid; k < ncb; k += U * T) {
+    uint64_t ids[U], rk[U];
 #pragma unroll
-    for (int a = 0; a < N; ++a) {
-      l[a] = static_cast<int>((ids >> (16 * a)) & 0xffffu);
-      if constexpr (WIDE) {
-        double rec[NS];
-        load_rec<NS>(snode + l[a] * NS, rec);
-#pragma unroll
-        for (int i = 0; i < D; ++i) pc[a][i] = rec[i];
-        kap[a] = rec[D];
-        fv[a] = WANT_B ? rec[D + 1] : 0.0;
-        gv[a] = BC ? rec[D + 2] : 0.0;
-      } else {
-        load_node<D>(snode + l[a] * NS, pc[a], kap[a]);
-        fv[a] = WANT_B ? sf[l[a]] : 0.0;
-        gv[a] = BC ? sbcv[l[a]] : 0.0;
+    for (int u = 0; u < U; ++u) {
+      ids[u] = ids_n[u];
+      rk[u] = rk_n[u];
+      const int kn = k + (U + u) * T;
+      if (kn < ncb) {
+        ids_n[u] = t.cell_ids[c0 + kn];
+        if (do_A) rk_n[u] = t.cell_rank[c0 + kn];
       }
-      be[a] = 0.0;
     }
-    p1_element<D, AFORM, WANT_B>(pc, kap, fv, A, be);
-    if constexpr (BC) {
-      bool isbc[N];
+    int l[U][N];
+    double pc[U][N][D], kap[U][N], fv[U][N], gv[U][N];
 #pragma unroll
-      for (int a = 0; a < N; ++a) isbc[a] = sbcf[l[a]] != 0;
-      apply_bc<N>(A, be, isbc, gv);
-    }
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-    for (int a = 0; a < N; ++a) {
-      int off, row;
-      if constexpr (STRIDED) {
-        if (l[a] >= nr) continue;
-        row = l[a];
-        off = l[a] * kRowStride;
-      } else {
-        const int own = sown[l[a]];
-        if (own < 0) continue;
-        off = own & 0xffff;
-        row = own >> 16;
+      for (int a = 0; a < N; ++a) {
+        l[u][a] = static_cast<int>((ids[u] >> (16 * a)) & 0xffffu);
+        if constexpr (WIDE) {
+          double rec[NS];
+          load_rec<NS>(snode + l[u][a] * NS, rec);
+#pragma unroll
+          for (int i = 0; i < D; ++i) pc[u][a][i] = rec[i];
+          kap[u][a] = rec[D];
+          fv[u][a] = WANT_B ? rec[D + 1] : 0.0;
+          gv[u][a] = BC ? rec[D + 2] : 0.0;
+        } else {
+          load_node<D>(snode + l[u][a] * NS, pc[u][a], kap[u][a]);
+          fv[u][a] = WANT_B ? sf[l[u][a]] : 0.0;
+          gv[u][a] = BC ? sbcv[l[u][a]] : 0.0;
+        }
       }
-      if (do_A) {
-        const uint32_t ra = static_cast<uint32_t>(rk >> (16 * a));
 #pragma unroll
-        for (int b = 0; b < N; ++b)
-          atomic_add_lds(svals + off + static_cast<int>((ra >> (4 * b)) & 15u), A[a][b]);
+    for (int u = 0; u < U; ++u) {
+      if (k + u * T >= ncb) continue;
+      double A[N][N], be[N];
+#pragma unroll
+      for (int a = 0; a < N; ++a) be[a] = 0.0;
+      p1_element<D, AFORM, WANT_B>(pc[u], kap[u], fv[u], A, be);
+      if constexpr (BC) {
+        bool isbc[N];
+#pragma unroll
+        for (int a = 0; a < N; ++a) isbc[a] = sbcf[l[u][a]] != 0;
+        apply_bc<N>(A, be, isbc, gv[u]);
       }
-      if constexpr (WANT_B) atomic_add_lds(sbv + row, be[a]);
+#pragma unroll
+      for (int a = 0; a < N; ++a) {
+        int off, row;
+        if constexpr (STRIDED) {
+          if (l[u][a] >= nr) continue;
+          row = l[u][a];
+          off = l[u][a] * kRowStride;
+        } else {
+          const int own = sown[l[u][a]];
+          if (own < 0) continue;
+          off = own & 0xffff;
+          row = own >> 16;
+        }
+        if (do_A) {
+          const uint32_t ra = static_cast<uint32_t>(rk[u] >> (16 * a));
+#pragma unroll
+          for (int b = 0; b < N; ++b)
+            atomic_add_lds(svals + off + static_cast<int>((ra >> (4 * b)) & 15u), A[a][b]);
+        }
+        if constexpr (WANT_B) atomic_add_lds(sbv + row, be[a]);
+      }
     }
   }
   __syncthreads();
