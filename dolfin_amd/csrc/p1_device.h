@@ -154,7 +154,7 @@ __device__ __forceinline__ int find_slot(const int32_t* __restrict__ row_ptr,
 // Blocked (owner-computes) scatter: workgroup per row block.
 // ---------------------------------------------------------------------------------------
 struct LdsLayout {
-  int node, f, bcv, bcf, own, vals, bv, rdof, total;
+  int node, f, bcv, bcf, own, vals, bv, rdof, rbase, total;
 };
 
 __host__ __device__ inline int align16(int n) { return (n + 15) & ~15; }
@@ -182,6 +182,7 @@ __host__ __device__ inline LdsLayout lds_layout(int D, int maxN, int maxR, int m
   L.vals = o; o += align16((strided ? maxR * kRowStride : maxNnz) * 8);
   L.bv = o;   o += f ? align16(maxR * 8) : 0;
   L.rdof = o; o += align16(maxR * 4);
+  L.rbase = o; o += strided ? align16(2 * maxR * 4) : 0;
   L.total = o;
   return L;
 }
@@ -281,6 +282,7 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
   double* svals = reinterpret_cast<double*>(lds + L.vals);
   double* sbv = reinterpret_cast<double*>(lds + L.bv);
   int32_t* srdof = reinterpret_cast<int32_t*>(lds + L.rdof);
+  int32_t* srbase = reinterpret_cast<int32_t*>(lds + L.rbase);
 
   const int tid = threadIdx.x;
   const bool do_A = WANT_A && p.want_A;
@@ -295,7 +297,17 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
       reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
   }
   for (int i = tid; i < nr; i += T) {
-    srdof[i] = t.row_dof[r0 + i];
+    const int32_t dof = t.row_dof[r0 + i];
+    srdof[i] = dof;
+    if constexpr (STRIDED) {
+      // CSR start of each owned row, fetched now so that the write-out issues no dependent
+      // global loads (the row length follows from the next row's start or the dof after)
+      if (do_A) {
+        const int32_t b0 = t.csr_row_ptr[dof];
+        srbase[i] = b0;
+        srbase[nr + i] = t.csr_row_ptr[dof + 1] - b0;
+      }
+    }
     if constexpr (WANT_B) sbv[i] = 0.0;
   }
   if constexpr (!STRIDED) {
@@ -394,9 +406,8 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
       // CSR rows.
       const int grp = tid >> 4, ln = tid & 15;
       for (int i = grp; i < nr; i += T / 16) {
-        const int32_t dof = srdof[i];
-        const int base = t.csr_row_ptr[dof];
-        const int len = t.csr_row_ptr[dof + 1] - base;
+        const int base = srbase[i];
+        const int len = srbase[nr + i];
         if (ln < len) {
           double v = svals[i * kRowStride + ln];
           if (p.add) v += p.values[base + ln];
