@@ -11,7 +11,7 @@ import os
 import torch  # noqa: F401  (load torch's HIP runtime before the library)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libdolfin_amd.so")
+LIB_PATH = os.environ.get("DFA_LIBRARY") or os.path.join(_HERE, "libdolfin_amd.so")
 
 ELEMENT_P1 = 1
 ELEMENT_VECTOR_P2 = 2

@@ -163,8 +163,14 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
             t.node_kdof = h->plan.upload(P.node_kdof, s);
             t.node_dof = h->plan.upload(P.node_dof, s);
             t.cell_ptr = h->plan.upload(P.cell_ptr, s);
-            t.cell_ids = h->plan.upload(P.cell_ids, s);
-            t.cell_rank = h->plan.upload(P.cell_rank, s);
+            {
+              std::vector<uint64_t> rec(2 * P.cell_ids.size());
+              for (size_t k = 0; k < P.cell_ids.size(); ++k) {
+                rec[2 * k] = P.cell_ids[k];
+                rec[2 * k + 1] = P.cell_rank[k];
+              }
+              t.cell_rec = h->plan.upload(rec, s);
+            }
             t.row_ptr_b = h->plan.upload(P.row_ptr, s);
             t.row_dof = h->plan.upload(P.row_dof, s);
             t.row_info = h->plan.upload(P.row_info, s);

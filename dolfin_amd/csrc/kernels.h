@@ -44,8 +44,8 @@ struct P1BlockTopo {
   const int32_t* node_kdof = nullptr;
   const int32_t* node_dof = nullptr;
   const int32_t* cell_ptr = nullptr;
-  const uint64_t* cell_ids = nullptr;
-  const uint64_t* cell_rank = nullptr;
+  // per cell appearance, interleaved: [2k] 4 x 16-bit local node ids, [2k+1] 16 x 4-bit ranks
+  const uint64_t* cell_rec = nullptr;
   const int32_t* row_ptr_b = nullptr;
   const int32_t* row_dof = nullptr;
   const uint32_t* row_info = nullptr;

@@ -154,7 +154,7 @@ __device__ __forceinline__ int find_slot(const int32_t* __restrict__ row_ptr,
 // Blocked (owner-computes) scatter: workgroup per row block.
 // ---------------------------------------------------------------------------------------
 struct LdsLayout {
-  int node, f, bcv, bcf, own, vals, bv, rdof, rbase, total;
+  int node, f, bcv, bcf, own, vals, bv, rdof, rbase, diag, total;
 };
 
 __host__ __device__ inline int align16(int n) { return (n + 15) & ~15; }
@@ -183,6 +183,7 @@ __host__ __device__ inline LdsLayout lds_layout(int D, int maxN, int maxR, int m
   L.bv = o;   o += f ? align16(maxR * 8) : 0;
   L.rdof = o; o += align16(maxR * 4);
   L.rbase = o; o += strided ? align16(2 * maxR * 4) : 0;
+  L.diag = o;  o += strided ? align16(maxR) : 0;
   L.total = o;
   return L;
 }
@@ -286,6 +287,8 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
 
   const int tid = threadIdx.x;
   const bool do_A = WANT_A && p.want_A;
+  constexpr bool ROWSUM_DIAG = STRIDED && AFORM == DFA_FORM_P1_POISSON && !BC;
+  int8_t* sdiag = reinterpret_cast<int8_t*>(lds + L.diag);
 
   gather_nodes<D, T, NEED_K, WANT_B, BC, !STRIDED, WIDE>(t, p.x, p.kappa, p.f, p.bc_mask,
                                                          p.bc_value, blk, snode, sf, sbcv, sbcf,
@@ -322,12 +325,14 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
   const int c0 = t.cell_ptr[blk], ncb = t.cell_ptr[blk + 1] - c0;
   // U cell appearances per thread and iteration: all their LDS loads are issued before any
   // arithmetic (latency hiding at low occupancy); records are prefetched one iteration ahead.
+  const ulonglong2* __restrict__ crec = reinterpret_cast<const ulonglong2*>(t.cell_rec) + c0;
   uint64_t ids_n[U], rk_n[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int kk = tid + u * T;
-    ids_n[u] = kk < ncb ? t.cell_ids[c0 + kk] : 0;
-    rk_n[u] = (do_A && kk < ncb) ? t.cell_rank[c0 + kk] : 0;
+    const ulonglong2 r = kk < ncb ? crec[kk] : make_ulonglong2(0, 0);
+    ids_n[u] = r.x;
+    rk_n[u] = r.y;
   }
   for (int k = tid; k < ncb; k += U * T) {
     uint64_t ids[U], rk[U];
@@ -337,8 +342,9 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
       rk[u] = rk_n[u];
       const int kn = k + (U + u) * T;
       if (kn < ncb) {
-        ids_n[u] = t.cell_ids[c0 + kn];
-        if (do_A) rk_n[u] = t.cell_rank[c0 + kn];
+        const ulonglong2 r = crec[kn];
+        ids_n[u] = r.x;
+        rk_n[u] = r.y;
       }
     }
     int l[U][N];
@@ -348,7 +354,18 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
 #pragma unroll
       for (int a = 0; a < N; ++a) {
         l[u][a] = static_cast<int>((ids[u] >> (16 * a)) & 0xffffu);
-        if constexpr (WIDE) {
+        if constexpr (WIDE && !BC && D == 3) {
+          // x, y | z, kappa as two 128-bit reads, f alone as a 64-bit read
+          const double* rp = snode + l[u][a] * NS;
+          const double2 r0 = *reinterpret_cast<const double2*>(rp);
+          const double2 r1 = *reinterpret_cast<const double2*>(rp + 2);
+          pc[u][a][0] = r0.x;
+          pc[u][a][1] = r0.y;
+          pc[u][a][2] = r1.x;
+          kap[u][a] = r1.y;
+          fv[u][a] = WANT_B ? rp[4] : 0.0;
+          gv[u][a] = 0.0;
+        } else if constexpr (WIDE) {
           double rec[NS];
           load_rec<NS>(snode + l[u][a] * NS, rec);
 #pragma unroll
@@ -391,8 +408,18 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
         if (do_A) {
           const uint32_t ra = static_cast<uint32_t>(rk[u] >> (16 * a));
 #pragma unroll
-          for (int b = 0; b < N; ++b)
-            atomic_add_lds(svals + off + static_cast<int>((ra >> (4 * b)) & 15u), A[a][b]);
+          for (int b = 0; b < N; ++b) {
+            const int slot = static_cast<int>((ra >> (4 * b)) & 15u);
+            if constexpr (ROWSUM_DIAG) {
+              // the diagonal is recovered from the row sum in the write-out; only its
+              // position is recorded (every writer stores the same value)
+              if (b == a) {
+                sdiag[row] = static_cast<int8_t>(slot);
+                continue;
+              }
+            }
+            atomic_add_lds(svals + off + slot, A[a][b]);
+          }
         }
         if constexpr (WANT_B) atomic_add_lds(sbv + row, be[a]);
       }
@@ -408,7 +435,18 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
       for (int i = grp; i < nr; i += T / 16) {
         const int base = srbase[i];
         const int len = srbase[nr + i];
-        if (ln < len) {
+        if constexpr (ROWSUM_DIAG) {
+          // kappa-Poisson element rows sum to zero: K_aa = -sum_{b != a} K_ab
+          double v = ln < len ? svals[i * kRowStride + ln] : 0.0;
+          double rs = v;
+#pragma unroll
+          for (int m = 8; m > 0; m >>= 1) rs += __shfl_xor(rs, m, 16);
+          if (ln == sdiag[i]) v = -rs;
+          if (ln < len) {
+            if (p.add) v += p.values[base + ln];
+            p.values[base + ln] = v;
+          }
+        } else if (ln < len) {
           double v = svals[i * kRowStride + ln];
           if (p.add) v += p.values[base + ln];
           p.values[base + ln] = v;

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTop
   double Jl = 0.0;
   const int c0 = t.cell_ptr[blk], ncb = t.cell_ptr[blk + 1] - c0;
   for (int k = tid; k < ncb; k += kThreads) {
-    const uint64_t ids = t.cell_ids[c0 + k];
+    const uint64_t ids = t.cell_rec[2 * (c0 + k)];
     int l[N];
     double pc[N][D], kap[N];
 #pragma unroll
