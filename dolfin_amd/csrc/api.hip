@@ -156,7 +156,7 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
             t.max_row_len = P.max_row_len;
             t.threads = default_block_threads();
             if (const char* e = std::getenv("DFA_NODE_RECORD")) t.wide_nodes = std::atoi(e) != 32;
-            if (const char* e = std::getenv("DFA_ILP")) t.ilp = std::atoi(e) == 2 ? 2 : 1;
+            t.max_cells = P.max_cells;
             t.kdof_is_vertex = P.kdof_is_vertex;
             t.node_ptr = h->plan.upload(P.node_ptr, s);
             t.node_vertex = h->plan.upload(P.node_vertex, s);

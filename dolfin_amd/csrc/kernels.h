@@ -37,7 +37,7 @@ struct P1BlockTopo {
   int32_t max_nodes = 0, max_rows = 0, max_nnz = 0, max_row_len = 0;
   int32_t threads = 256;               // workgroup size of the blocked kernels (256 or 512)
   bool wide_nodes = true;              // 48-byte LDS node records (kappa, f, Dirichlet value inline)
-  int32_t ilp = 1;                     // cell appearances per thread and loop iteration (1 or 2)
+  int32_t max_cells = 0;               // largest cell-appearance list of a block
   bool kdof_is_vertex = false;
   const int32_t* node_ptr = nullptr;
   const int32_t* node_vertex = nullptr;

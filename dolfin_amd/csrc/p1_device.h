@@ -268,7 +268,7 @@ __device__ __forceinline__ void load_rec(const double* __restrict__ p, double (&
 
 // One row block of the blocked scatter (the workgroup's whole job in the standalone kernel,
 // one work unit of the batched persistent kernel).
-template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE, int U = 1>
+template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE, int RR = 0>
 __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass& p,
                                               const LdsLayout& L, unsigned char* lds, int blk) {
   constexpr int N = D + 1;
@@ -289,6 +289,15 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
   const bool do_A = WANT_A && p.want_A;
   constexpr bool ROWSUM_DIAG = STRIDED && AFORM == DFA_FORM_P1_POISSON && !BC;
   int8_t* sdiag = reinterpret_cast<int8_t*>(lds + L.diag);
+  // cell records: issued first so that their latency overlaps the node gather
+  const int c0 = t.cell_ptr[blk], ncb = t.cell_ptr[blk + 1] - c0;
+  const ulonglong2* __restrict__ crec = reinterpret_cast<const ulonglong2*>(t.cell_rec) + c0;
+  ulonglong2 rreg[RR > 0 ? RR : 1];
+  if constexpr (RR > 0) {
+#pragma unroll
+    for (int j = 0; j < RR; ++j)
+      rreg[j] = (tid + j * T < ncb) ? crec[tid + j * T] : make_ulonglong2(0, 0);
+  }
 
   gather_nodes<D, T, NEED_K, WANT_B, BC, !STRIDED, WIDE>(t, p.x, p.kappa, p.f, p.bc_mask,
                                                          p.bc_value, blk, snode, sf, sbcv, sbcf,
@@ -322,107 +331,91 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
   }
   __syncthreads();
 
-  const int c0 = t.cell_ptr[blk], ncb = t.cell_ptr[blk + 1] - c0;
-  // U cell appearances per thread and iteration: all their LDS loads are issued before any
-  // arithmetic (latency hiding at low occupancy); records are prefetched one iteration ahead.
-  const ulonglong2* __restrict__ crec = reinterpret_cast<const ulonglong2*>(t.cell_rec) + c0;
-  uint64_t ids_n[U], rk_n[U];
+  auto process = [&](uint64_t ids, uint64_t rk) {
+    int l[N];
+    double pc[N][D], kap[N], fv[N], gv[N];
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const int kk = tid + u * T;
-    const ulonglong2 r = kk < ncb ? crec[kk] : make_ulonglong2(0, 0);
-    ids_n[u] = r.x;
-    rk_n[u] = r.y;
-  }
-  for (int k = tid; k < ncb; k += U * T) {
-    uint64_t ids[U], rk[U];
+    for (int a = 0; a < N; ++a) {
+      l[a] = static_cast<int>((ids >> (16 * a)) & 0xffffu);
+      if constexpr (WIDE && !BC && D == 3) {
+        // x, y | z, kappa as two 128-bit reads, f alone as a 64-bit read
+        const double* rp = snode + l[a] * NS;
+        const double2 r0 = *reinterpret_cast<const double2*>(rp);
+        const double2 r1 = *reinterpret_cast<const double2*>(rp + 2);
+        pc[a][0] = r0.x;
+        pc[a][1] = r0.y;
+        pc[a][2] = r1.x;
+        kap[a] = r1.y;
+        fv[a] = WANT_B ? rp[4] : 0.0;
+        gv[a] = 0.0;
+      } else if constexpr (WIDE) {
+        double rec[NS];
+        load_rec<NS>(snode + l[a] * NS, rec);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      ids[u] = ids_n[u];
-      rk[u] = rk_n[u];
-      const int kn = k + (U + u) * T;
-      if (kn < ncb) {
-        const ulonglong2 r = crec[kn];
-        ids_n[u] = r.x;
-        rk_n[u] = r.y;
+        for (int q = 0; q < D; ++q) pc[a][q] = rec[q];
+        kap[a] = rec[D];
+        fv[a] = WANT_B ? rec[D + 1] : 0.0;
+        gv[a] = BC ? rec[D + 2] : 0.0;
+      } else {
+        load_node<D>(snode + l[a] * NS, pc[a], kap[a]);
+        fv[a] = WANT_B ? sf[l[a]] : 0.0;
+        gv[a] = BC ? sbcv[l[a]] : 0.0;
       }
     }
-    int l[U][N];
-    double pc[U][N][D], kap[U][N], fv[U][N], gv[U][N];
+    double A[N][N], be[N];
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+    for (int a = 0; a < N; ++a) be[a] = 0.0;
+    p1_element<D, AFORM, WANT_B>(pc, kap, fv, A, be);
+    if constexpr (BC) {
+      bool isbc[N];
 #pragma unroll
-      for (int a = 0; a < N; ++a) {
-        l[u][a] = static_cast<int>((ids[u] >> (16 * a)) & 0xffffu);
-        if constexpr (WIDE && !BC && D == 3) {
-          // x, y | z, kappa as two 128-bit reads, f alone as a 64-bit read
-          const double* rp = snode + l[u][a] * NS;
-          const double2 r0 = *reinterpret_cast<const double2*>(rp);
-          const double2 r1 = *reinterpret_cast<const double2*>(rp + 2);
-          pc[u][a][0] = r0.x;
-          pc[u][a][1] = r0.y;
-          pc[u][a][2] = r1.x;
-          kap[u][a] = r1.y;
-          fv[u][a] = WANT_B ? rp[4] : 0.0;
-          gv[u][a] = 0.0;
-        } else if constexpr (WIDE) {
-          double rec[NS];
-          load_rec<NS>(snode + l[u][a] * NS, rec);
+      for (int a = 0; a < N; ++a) isbc[a] = sbcf[l[a]] != 0;
+      apply_bc<N>(A, be, isbc, gv);
+    }
 #pragma unroll
-          for (int i = 0; i < D; ++i) pc[u][a][i] = rec[i];
-          kap[u][a] = rec[D];
-          fv[u][a] = WANT_B ? rec[D + 1] : 0.0;
-          gv[u][a] = BC ? rec[D + 2] : 0.0;
-        } else {
-          load_node<D>(snode + l[u][a] * NS, pc[u][a], kap[u][a]);
-          fv[u][a] = WANT_B ? sf[l[u][a]] : 0.0;
-          gv[u][a] = BC ? sbcv[l[u][a]] : 0.0;
-        }
+    for (int a = 0; a < N; ++a) {
+      int off, row;
+      if constexpr (STRIDED) {
+        if (l[a] >= nr) continue;
+        row = l[a];
+        off = l[a] * kRowStride;
+      } else {
+        const int own = sown[l[a]];
+        if (own < 0) continue;
+        off = own & 0xffff;
+        row = own >> 16;
       }
+      if (do_A) {
+        const uint32_t ra = static_cast<uint32_t>(rk >> (16 * a));
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (k + u * T >= ncb) continue;
-      double A[N][N], be[N];
-#pragma unroll
-      for (int a = 0; a < N; ++a) be[a] = 0.0;
-      p1_element<D, AFORM, WANT_B>(pc[u], kap[u], fv[u], A, be);
-      if constexpr (BC) {
-        bool isbc[N];
-#pragma unroll
-        for (int a = 0; a < N; ++a) isbc[a] = sbcf[l[u][a]] != 0;
-        apply_bc<N>(A, be, isbc, gv[u]);
-      }
-#pragma unroll
-      for (int a = 0; a < N; ++a) {
-        int off, row;
-        if constexpr (STRIDED) {
-          if (l[u][a] >= nr) continue;
-          row = l[u][a];
-          off = l[u][a] * kRowStride;
-        } else {
-          const int own = sown[l[u][a]];
-          if (own < 0) continue;
-          off = own & 0xffff;
-          row = own >> 16;
-        }
-        if (do_A) {
-          const uint32_t ra = static_cast<uint32_t>(rk[u] >> (16 * a));
-#pragma unroll
-          for (int b = 0; b < N; ++b) {
-            const int slot = static_cast<int>((ra >> (4 * b)) & 15u);
-            if constexpr (ROWSUM_DIAG) {
-              // the diagonal is recovered from the row sum in the write-out; only its
-              // position is recorded (every writer stores the same value)
-              if (b == a) {
-                sdiag[row] = static_cast<int8_t>(slot);
-                continue;
-              }
+        for (int b = 0; b < N; ++b) {
+          const int slot = static_cast<int>((ra >> (4 * b)) & 15u);
+          if constexpr (ROWSUM_DIAG) {
+            // the diagonal is recovered from the row sum in the write-out; only its
+            // position is recorded (every writer stores the same value)
+            if (b == a) {
+              sdiag[row] = static_cast<int8_t>(slot);
+              continue;
             }
-            atomic_add_lds(svals + off + slot, A[a][b]);
           }
+          atomic_add_lds(svals + off + slot, A[a][b]);
         }
-        if constexpr (WANT_B) atomic_add_lds(sbv + row, be[a]);
       }
+      if constexpr (WANT_B) atomic_add_lds(sbv + row, be[a]);
+    }
+  };
+  if constexpr (RR > 0) {
+    // all of this thread's cell records were loaded into registers in the prologue
+#pragma unroll
+    for (int j = 0; j < RR; ++j)
+      if (tid + j * T < ncb) process(rreg[j].x, rreg[j].y);
+  } else {
+    // streaming: records prefetched one iteration ahead
+    ulonglong2 nxt = tid < ncb ? crec[tid] : make_ulonglong2(0, 0);
+    for (int k = tid; k < ncb; k += T) {
+      const ulonglong2 cur = nxt;
+      if (k + T < ncb) nxt = crec[k + T];
+      process(cur.x, cur.y);
     }
   }
   __syncthreads();

@@ -74,10 +74,10 @@ __global__ __launch_bounds__(kThreads) void p1_atomic_kernel(P1AtomicTopo t, P1P
   }
 }
 
-template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE, int U>
+template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE, int RR>
 __global__ __launch_bounds__(T) void p1_blocked_kernel(P1BlockTopo t, P1Pass p, LdsLayout L) {
   extern __shared__ __align__(16) unsigned char lds[];
-  p1_block_body<D, AFORM, WANT_B, BC, STRIDED, T, WIDE, U>(t, p, L, lds, xcd_block(t.num_blocks));
+  p1_block_body<D, AFORM, WANT_B, BC, STRIDED, T, WIDE, RR>(t, p, L, lds, xcd_block(t.num_blocks));
 }
 
 // Adjoint of J = u^T K(kappa, x) u - F(f, x)^T u on the same blocks: every owned vertex
@@ -212,10 +212,10 @@ hipError_t launch_atomic_t(const P1AtomicTopo& t, const P1Pass& p, hipStream_t s
   return hipGetLastError();
 }
 
-template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE, int U>
+template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE, int RR>
 hipError_t launch_blocked_tt(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
   const LdsLayout L = lds_layout(D, t.max_nodes, t.max_rows, t.max_nnz, WANT_B, BC, STRIDED, WIDE);
-  auto kern = p1_blocked_kernel<D, AFORM, WANT_B, BC, STRIDED, T, WIDE, U>;
+  auto kern = p1_blocked_kernel<D, AFORM, WANT_B, BC, STRIDED, T, WIDE, RR>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, L.total);
   if (e != hipSuccess) return e;
@@ -223,14 +223,15 @@ hipError_t launch_blocked_tt(const P1BlockTopo& t, const P1Pass& p, hipStream_t 
   return hipGetLastError();
 }
 
+// Records in registers when every block's cell list fits 4 per thread.
 template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, bool WIDE>
 hipError_t launch_blocked_tu(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
   if (t.threads == 512) {
-    if (t.ilp == 2) return launch_blocked_tt<D, AFORM, WANT_B, BC, STRIDED, 512, WIDE, 2>(t, p, s);
-    return launch_blocked_tt<D, AFORM, WANT_B, BC, STRIDED, 512, WIDE, 1>(t, p, s);
+    if (t.max_cells <= 4 * 512) return launch_blocked_tt<D, AFORM, WANT_B, BC, STRIDED, 512, WIDE, 4>(t, p, s);
+    return launch_blocked_tt<D, AFORM, WANT_B, BC, STRIDED, 512, WIDE, 0>(t, p, s);
   }
-  if (t.ilp == 2) return launch_blocked_tt<D, AFORM, WANT_B, BC, STRIDED, 256, WIDE, 2>(t, p, s);
-  return launch_blocked_tt<D, AFORM, WANT_B, BC, STRIDED, 256, WIDE, 1>(t, p, s);
+  if (t.max_cells <= 4 * 256) return launch_blocked_tt<D, AFORM, WANT_B, BC, STRIDED, 256, WIDE, 4>(t, p, s);
+  return launch_blocked_tt<D, AFORM, WANT_B, BC, STRIDED, 256, WIDE, 0>(t, p, s);
 }
 
 template <int D, int AFORM, bool WANT_B, bool BC>
@@ -241,7 +242,7 @@ hipError_t launch_blocked_t(const P1BlockTopo& t, const P1Pass& p, hipStream_t s
     if (t.wide_nodes) return launch_blocked_tu<D, AFORM, WANT_B, BC, true, true>(t, p, s);
     return launch_blocked_tu<D, AFORM, WANT_B, BC, true, false>(t, p, s);
   }
-  return launch_blocked_tt<D, AFORM, WANT_B, BC, false, 256, false, 1>(t, p, s);
+  return launch_blocked_tt<D, AFORM, WANT_B, BC, false, 256, false, 0>(t, p, s);
 }
 
 template <int D>
