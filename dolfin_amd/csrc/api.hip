@@ -174,6 +174,7 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
             t.row_ptr_b = h->plan.upload(P.row_ptr, s);
             t.row_dof = h->plan.upload(P.row_dof, s);
             t.row_info = h->plan.upload(P.row_info, s);
+            t.row_diag = h->plan.upload(P.row_diag, s);
             t.blk_nnz = h->plan.upload(P.blk_nnz, s);
             t.run_ptr = h->plan.upload(P.run_ptr, s);
             t.runs = h->plan.upload(P.runs, s);

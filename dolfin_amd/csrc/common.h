@@ -44,6 +44,7 @@ struct P1BlockPlan {
   std::vector<int32_t> row_ptr;         // [nb+1] owned rows per block
   std::vector<int32_t> row_dof;         // global row (dof) of each owned row
   std::vector<uint32_t> row_info;       // (local node id) | (LDS value offset << 16)
+  std::vector<uint8_t> row_diag;        // position of the diagonal entry in each owned row
   std::vector<int32_t> blk_nnz;         // [nb] values per block
   // Write-out runs: maximal sequences of owned rows with consecutive dofs are one contiguous
   // CSR range. Packed as csr_start | local_offset << 32 | length << 48.

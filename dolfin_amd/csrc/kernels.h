@@ -49,6 +49,7 @@ struct P1BlockTopo {
   const int32_t* row_ptr_b = nullptr;
   const int32_t* row_dof = nullptr;
   const uint32_t* row_info = nullptr;
+  const uint8_t* row_diag = nullptr;
   const int32_t* blk_nnz = nullptr;
   const int32_t* run_ptr = nullptr;
   const uint64_t* runs = nullptr;

@@ -266,6 +266,40 @@ __device__ __forceinline__ void load_rec(const double* __restrict__ p, double (&
   }
 }
 
+// Sum over each 16-lane row of the wavefront, result in every lane, with DPP moves only
+// (VALU; no LDS-pipe permute): xor 1 and xor 2 within quads, then rotate the row by 4 and 8.
+__device__ __forceinline__ double dpp_move(double v, int ctrl) {
+  const int2 a = *reinterpret_cast<int2*>(&v);
+  int2 r;
+  switch (ctrl) {
+    case 0xB1:
+      r.x = __builtin_amdgcn_update_dpp(0, a.x, 0xB1, 0xF, 0xF, false);
+      r.y = __builtin_amdgcn_update_dpp(0, a.y, 0xB1, 0xF, 0xF, false);
+      break;
+    case 0x4E:
+      r.x = __builtin_amdgcn_update_dpp(0, a.x, 0x4E, 0xF, 0xF, false);
+      r.y = __builtin_amdgcn_update_dpp(0, a.y, 0x4E, 0xF, 0xF, false);
+      break;
+    case 0x124:
+      r.x = __builtin_amdgcn_update_dpp(0, a.x, 0x124, 0xF, 0xF, false);
+      r.y = __builtin_amdgcn_update_dpp(0, a.y, 0x124, 0xF, 0xF, false);
+      break;
+    default:
+      r.x = __builtin_amdgcn_update_dpp(0, a.x, 0x128, 0xF, 0xF, false);
+      r.y = __builtin_amdgcn_update_dpp(0, a.y, 0x128, 0xF, 0xF, false);
+      break;
+  }
+  return *reinterpret_cast<double*>(&r);
+}
+
+__device__ __forceinline__ double row16_sum(double v) {
+  v += dpp_move(v, 0xB1);   // quad_perm [1,0,3,2]
+  v += dpp_move(v, 0x4E);   // quad_perm [2,3,0,1]
+  v += dpp_move(v, 0x124);  // row_ror 4
+  v += dpp_move(v, 0x128);  // row_ror 8
+  return v;
+}
+
 // One row block of the blocked scatter (the workgroup's whole job in the standalone kernel,
 // one work unit of the batched persistent kernel).
 template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE, int RR = 0>
@@ -321,6 +355,7 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
       }
     }
     if constexpr (WANT_B) sbv[i] = 0.0;
+    if constexpr (ROWSUM_DIAG) sdiag[i] = static_cast<int8_t>(t.row_diag[r0 + i]);
   }
   if constexpr (!STRIDED) {
     __syncthreads();
@@ -391,12 +426,7 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
         for (int b = 0; b < N; ++b) {
           const int slot = static_cast<int>((ra >> (4 * b)) & 15u);
           if constexpr (ROWSUM_DIAG) {
-            // the diagonal is recovered from the row sum in the write-out; only its
-            // position is recorded (every writer stores the same value)
-            if (b == a) {
-              sdiag[row] = static_cast<int8_t>(slot);
-              continue;
-            }
+            if (b == a) continue;  // recovered from the row sum in the write-out
           }
           atomic_add_lds(svals + off + slot, A[a][b]);
         }
@@ -431,9 +461,7 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
         if constexpr (ROWSUM_DIAG) {
           // kappa-Poisson element rows sum to zero: K_aa = -sum_{b != a} K_ab
           double v = ln < len ? svals[i * kRowStride + ln] : 0.0;
-          double rs = v;
-#pragma unroll
-          for (int m = 8; m > 0; m >>= 1) rs += __shfl_xor(rs, m, 16);
+          const double rs = row16_sum(v);
           if (ln == sdiag[i]) v = -rs;
           if (ln < len) {
             if (p.add) v += p.values[base + ln];

@@ -223,6 +223,7 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   std::vector<std::vector<int32_t>> brows(nb);
   std::vector<std::vector<uint32_t>> binfo(nb);
   std::vector<std::vector<uint64_t>> bruns(nb);
+  std::vector<std::vector<uint8_t>> bdiag(nb);
   P.cell_ids.resize(app.size());
   P.cell_rank.resize(app.size());
   P.blk_nnz.assign(nb, 0);
@@ -261,6 +262,11 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
       uint32_t off = 0;
       for (int32_t d : blk_dofs[b]) {
         const uint32_t ln = local(dofv[d]);
+        {
+          const int32_t* rb = csr.col.data() + csr.row_ptr[d];
+          const int32_t* re = csr.col.data() + csr.row_ptr[d + 1];
+          bdiag[b].push_back(static_cast<uint8_t>(std::min<int64_t>(std::lower_bound(rb, re, d) - rb, 255)));
+        }
         brows[b].push_back(d);
         binfo[b].push_back(ln | (off << 16));
         row_off_of_node[ln] = static_cast<int32_t>(off);
@@ -306,7 +312,11 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
             if (!(busy >> q & 1)) { colour = q; break; }
           for (int a = 0; a < nvc; ++a)
             if (row_off_of_node[ln[a]] >= 0) used[ln[a]] |= (1ull << colour);
-          col[k - kb0] = {colour, static_cast<int32_t>(c)};
+          // secondary key: the cell's smallest local node, so that the lanes of a wavefront
+          // read nearby node records (different LDS banks) within a colour
+          uint32_t lmin = ln[0];
+          for (int a = 1; a < nvc; ++a) lmin = std::min(lmin, ln[a]);
+          col[k - kb0] = {colour * 65536 + static_cast<int>(lmin), static_cast<int32_t>(c)};
         }
         std::stable_sort(col.begin(), col.end(),
                          [](const std::pair<int, int32_t>& x, const std::pair<int, int32_t>& y) {
@@ -364,6 +374,7 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   P.node_dof.resize(P.node_ptr[nb]);
   P.row_dof.resize(P.row_ptr[nb]);
   P.row_info.resize(P.row_ptr[nb]);
+  P.row_diag.resize(P.row_ptr[nb]);
   parallel_for(nb, [&](int64_t b0, int64_t b1, int) {
     for (int64_t b = b0; b < b1; ++b) {
       int32_t o = P.node_ptr[b];
@@ -375,6 +386,7 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
       }
       std::copy(brows[b].begin(), brows[b].end(), P.row_dof.begin() + P.row_ptr[b]);
       std::copy(binfo[b].begin(), binfo[b].end(), P.row_info.begin() + P.row_ptr[b]);
+      std::copy(bdiag[b].begin(), bdiag[b].end(), P.row_diag.begin() + P.row_ptr[b]);
     }
   });
 }
