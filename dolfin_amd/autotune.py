@@ -17,7 +17,7 @@ from . import _lib
 
 _CACHE = {}
 
-P1_ROWS = (128, 192, 256)
+P1_ROWS = (128, 160, 192, 256)
 P1_THREADS = (256, 512)
 P2_VALS = (4096, 6144, 8192, 10240)
 

@@ -63,7 +63,7 @@ int default_p2_block_vals() {
     const int v = std::atoi(e);
     if (v >= 1024 && v <= 16384) return v;
   }
-  return 8192;
+  return 6144;
 }
 
 int default_block_rows(int gdim) {
@@ -72,7 +72,7 @@ int default_block_rows(int gdim) {
     const int v = std::atoi(e);
     if (v >= 8 && v <= 4096) return v;
   }
-  return gdim == 3 ? 192 : 512;
+  return gdim == 3 ? 160 : 512;
 }
 
 }  // namespace

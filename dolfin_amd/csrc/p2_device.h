@@ -246,6 +246,160 @@ __device__ __forceinline__ void node_block(const C& c, const NodeFactors& f, dou
     for (int j = 0; j < 3; ++j) K[i][j] = (i == j ? tr : 0.0) + Sm[j][i] + Sl[i][j];
 }
 
+// ---------------------------------------------------------------------------------------
+// Lean per-cell data for the blocked kernel. With c P1 (nodal values c_m, S = sum c_m) the
+// exact weights have the closed form
+//   W_mr = int c lambda_m lambda_r = vol (1 + delta_mr)(S + c_m + c_r) / 120,
+//   R_m  = sum_r W_mr = vol (S + c_m) / 20,
+// so u_t^T W needs only c at the factor's index. Vertex and edge nodes get separate code
+// paths (one and two gradient factors), selected per work item.
+// ---------------------------------------------------------------------------------------
+struct LeanGeom {
+  double D[4][3];
+  double cm[4], cl[4];
+  double Sm, Sl, vol;
+};
+
+__device__ __forceinline__ void lean_setup(const double (&p)[4][3], const double (&mu)[4],
+                                           const double (&la)[4], LeanGeom& g) {
+  double e1[3], e2[3], e3[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = p[1][i] - p[0][i];
+    e2[i] = p[2][i] - p[0][i];
+    e3[i] = p[3][i] - p[0][i];
+  }
+  double c[4][3];
+  c[1][0] = e2[1] * e3[2] - e2[2] * e3[1];
+  c[1][1] = e2[2] * e3[0] - e2[0] * e3[2];
+  c[1][2] = e2[0] * e3[1] - e2[1] * e3[0];
+  c[2][0] = e3[1] * e1[2] - e3[2] * e1[1];
+  c[2][1] = e3[2] * e1[0] - e3[0] * e1[2];
+  c[2][2] = e3[0] * e1[1] - e3[1] * e1[0];
+  c[3][0] = e1[1] * e2[2] - e1[2] * e2[1];
+  c[3][1] = e1[2] * e2[0] - e1[0] * e2[2];
+  c[3][2] = e1[0] * e2[1] - e1[1] * e2[0];
+  const double det = e1[0] * c[1][0] + e1[1] * c[1][1] + e1[2] * c[1][2];
+  double inv = __builtin_amdgcn_rcp(det);
+  double er = fma(-det, inv, 1.0);
+  inv = fma(inv, er, inv);
+  er = fma(-det, inv, 1.0);
+  inv = fma(inv, er, inv);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) c[0][i] = -(c[1][i] + c[2][i] + c[3][i]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g.D[k][i] = c[k][i] * inv;
+  g.vol = fabs(det) * (1.0 / 6.0);
+  g.Sm = mu[0] + mu[1] + mu[2] + mu[3];
+  g.Sl = la[0] + la[1] + la[2] + la[3];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    g.cm[k] = mu[k];
+    g.cl[k] = la[k];
+  }
+}
+
+struct NodeV {
+  double w[2][3];
+  double vm[2][4], vl[2][4];
+  double sm[2], sl[2];
+};
+
+template <bool VERT>
+__device__ __forceinline__ void node_v(const LeanGeom& g, int a, NodeV& f) {
+  const double s120 = g.vol * (1.0 / 120.0);
+  if constexpr (VERT) {
+    const int i = a;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) f.w[0][q] = sel4(i, g.D[0][q], g.D[1][q], g.D[2][q], g.D[3][q]);
+    const double cim = sel4(i, g.cm[0], g.cm[1], g.cm[2], g.cm[3]);
+    const double cil = sel4(i, g.cl[0], g.cl[1], g.cl[2], g.cl[3]);
+    // v = 4 W[i][.] - R
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const double dd = (m == i) ? 2.0 : 1.0;
+      f.vm[0][m] = s120 * (4.0 * dd * (g.Sm + cim + g.cm[m]) - 6.0 * (g.Sm + g.cm[m]));
+      f.vl[0][m] = s120 * (4.0 * dd * (g.Sl + cil + g.cl[m]) - 6.0 * (g.Sl + g.cl[m]));
+    }
+    f.sm[0] = f.vm[0][0] + f.vm[0][1] + f.vm[0][2] + f.vm[0][3];
+    f.sl[0] = f.vl[0][0] + f.vl[0][1] + f.vl[0][2] + f.vl[0][3];
+  } else {
+    const int e = a - 4;
+    const int k = (22 >> (2 * e)) & 3, l = (1775 >> (2 * e)) & 3;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      f.w[0][q] = sel4(k, g.D[0][q], g.D[1][q], g.D[2][q], g.D[3][q]);
+      f.w[1][q] = sel4(l, g.D[0][q], g.D[1][q], g.D[2][q], g.D[3][q]);
+    }
+    const double clm = sel4(l, g.cm[0], g.cm[1], g.cm[2], g.cm[3]);
+    const double cll = sel4(l, g.cl[0], g.cl[1], g.cl[2], g.cl[3]);
+    const double ckm = sel4(k, g.cm[0], g.cm[1], g.cm[2], g.cm[3]);
+    const double ckl = sel4(k, g.cl[0], g.cl[1], g.cl[2], g.cl[3]);
+    // u_0 = 4 e_l (gradient lambda_k), u_1 = 4 e_k (gradient lambda_l): v_t = 4 W[idx][.]
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const double d0 = (m == l) ? 8.0 : 4.0, d1 = (m == k) ? 8.0 : 4.0;
+      f.vm[0][m] = s120 * d0 * (g.Sm + clm + g.cm[m]);
+      f.vl[0][m] = s120 * d0 * (g.Sl + cll + g.cl[m]);
+      f.vm[1][m] = s120 * d1 * (g.Sm + ckm + g.cm[m]);
+      f.vl[1][m] = s120 * d1 * (g.Sl + ckl + g.cl[m]);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      f.sm[t] = f.vm[t][0] + f.vm[t][1] + f.vm[t][2] + f.vm[t][3];
+      f.sl[t] = f.vl[t][0] + f.vl[t][1] + f.vl[t][2] + f.vl[t][3];
+    }
+  }
+}
+
+// K[(i,a),(j,B)] for a compile-time node B and node kind of a.
+template <int B, bool VERT>
+__device__ __forceinline__ void node_block_lean(const LeanGeom& g, const NodeV& f, double (&K)[3][3]) {
+  constexpr bool VB = B < 4;
+  constexpr int EK = VB ? B : (22 >> (2 * (B - 4))) & 3;
+  constexpr int EL = VB ? 0 : (1775 >> (2 * (B - 4))) & 3;
+  constexpr int NS = VB ? 1 : 2;
+  constexpr int NT = VERT ? 1 : 2;
+  constexpr int IDX0 = VB ? B : EL, GRD0 = VB ? B : EK;
+  constexpr int IDX1 = EK, GRD1 = EL;
+  double Sm[3][3], Sl[3][3];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Sm[p][q] = Sl[p][q] = 0.0;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    // z = sum_s alpha_ts w'_s, then S += w_t (x) z
+    double zm[3] = {0.0, 0.0, 0.0}, zl[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int idx = s == 0 ? IDX0 : IDX1;
+      const int grd = s == 0 ? GRD0 : GRD1;
+      const double am = VB ? 4.0 * f.vm[t][idx] - f.sm[t] : 4.0 * f.vm[t][idx];
+      const double al = VB ? 4.0 * f.vl[t][idx] - f.sl[t] : 4.0 * f.vl[t][idx];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        zm[q] = fma(am, g.D[grd][q], zm[q]);
+        zl[q] = fma(al, g.D[grd][q], zl[q]);
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        Sm[p][q] = fma(f.w[t][p], zm[q], Sm[p][q]);
+        Sl[p][q] = fma(f.w[t][p], zl[q], Sl[p][q]);
+      }
+  }
+  const double tr = Sm[0][0] + Sm[1][1] + Sm[2][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) K[i][j] = (i == j ? tr : 0.0) + Sm[j][i] + Sl[i][j];
+}
+
 constexpr int kBT = 256;
 
 struct VP2Lds {
@@ -327,8 +481,8 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
       mu[v] = r1.y;
       la[v] = r2.x;
     }
-    CellGeom c;
-    vp2_cell_setup(pc, mu, la, c);
+    LeanGeom c;
+    lean_setup(pc, mu, la, c);
     const uint32_t info = sinfo[o];
     const int off = static_cast<int>(info & 0xffffu);
     const int deg = static_cast<int>(info >> 16);
@@ -353,47 +507,53 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
         }
     }
     if constexpr (WANT_A || BC) {
-      NodeFactors fa;
-      node_factors(c, a, fa);
-      auto one = [&](auto bconst) {
-        constexpr int b = decltype(bconst)::value;
-        double K[3][3];
-        node_block<b>(c, fa, K);
-        if constexpr (BC) {
+      auto run = [&](auto vconst) {
+        constexpr bool VERT = decltype(vconst)::value;
+        NodeV fa;
+        node_v<VERT>(c, a, fa);
+        auto one = [&](auto bconst) {
+          constexpr int b = decltype(bconst)::value;
+          double K[3][3];
+          node_block_lean<b, VERT>(c, fa, K);
+          if constexpr (BC) {
 #pragma unroll
-          for (int i = 0; i < 3; ++i)
+            for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-              const bool cb = (bccol >> (j * 10 + b)) & 1u;
-              if (WANT_B && cb && !(bcrow >> i & 1u))
-                be[i] -= K[i][j] * p.bc_value[cdofs[j * 10 + b]];
-              if (cb || (bcrow >> i & 1u)) K[i][j] = (i == j && a == b) ? 1.0 : 0.0;
-            }
-        }
-        if constexpr (WANT_A) {
-          const int rk = static_cast<int>(b < 5 ? (w1 >> (24 + 8 * b)) & 0xffu : (w2 >> (8 * (b - 5))) & 0xffu);
-          const int s2 = t.component_major ? deg : 1;
-#pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            double* row = svals + off + i * 3 * deg + (t.component_major ? rk : 3 * rk);
-#pragma unroll
-            for (int j = 0; j < 3; ++j) unsafeAtomicAdd(row + j * s2, K[i][j]);
+              for (int j = 0; j < 3; ++j) {
+                const bool cb = (bccol >> (j * 10 + b)) & 1u;
+                if (WANT_B && cb && !(bcrow >> i & 1u))
+                  be[i] -= K[i][j] * p.bc_value[cdofs[j * 10 + b]];
+                if (cb || (bcrow >> i & 1u)) K[i][j] = (i == j && a == b) ? 1.0 : 0.0;
+              }
           }
+          if constexpr (WANT_A) {
+            const int rk = static_cast<int>(b < 5 ? (w1 >> (24 + 8 * b)) & 0xffu : (w2 >> (8 * (b - 5))) & 0xffu);
+            const int s2 = t.component_major ? deg : 1;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              double* row = svals + off + i * 3 * deg + (t.component_major ? rk : 3 * rk);
+#pragma unroll
+              for (int j = 0; j < 3; ++j) unsafeAtomicAdd(row + j * s2, K[i][j]);
+            }
+          }
+        };
+        if (half == 0) {
+          one(std::integral_constant<int, 0>{});
+          one(std::integral_constant<int, 1>{});
+          one(std::integral_constant<int, 2>{});
+          one(std::integral_constant<int, 3>{});
+          one(std::integral_constant<int, 4>{});
+        } else {
+          one(std::integral_constant<int, 5>{});
+          one(std::integral_constant<int, 6>{});
+          one(std::integral_constant<int, 7>{});
+          one(std::integral_constant<int, 8>{});
+          one(std::integral_constant<int, 9>{});
         }
       };
-      if (half == 0) {
-        one(std::integral_constant<int, 0>{});
-        one(std::integral_constant<int, 1>{});
-        one(std::integral_constant<int, 2>{});
-        one(std::integral_constant<int, 3>{});
-        one(std::integral_constant<int, 4>{});
-      } else {
-        one(std::integral_constant<int, 5>{});
-        one(std::integral_constant<int, 6>{});
-        one(std::integral_constant<int, 7>{});
-        one(std::integral_constant<int, 8>{});
-        one(std::integral_constant<int, 9>{});
-      }
+      // work items are ordered vertex nodes first, so a wavefront mostly takes one branch
+      if (a < 4) run(std::integral_constant<bool, true>{});
+      else run(std::integral_constant<bool, false>{});
     }
     if constexpr (WANT_B) {
 #pragma unroll

@@ -583,7 +583,9 @@ void build_vp2_block_plan(int64_t nc, const int32_t* cells, const int32_t* cd, c
         const int64_t c = order[q] / 10;
         const int a = static_cast<int>(order[q] % 10);
         const int32_t o = own_index[cd[c * 30 + a]];
-        key[q] = {static_cast<int64_t>(seen[o]++) * 65536 + o, order[q]};
+        // vertex nodes first (one gradient factor), then edge nodes; within each kind the
+        // items are interleaved over the owned nodes
+        key[q] = {(a >= 4 ? (int64_t{1} << 40) : 0) + static_cast<int64_t>(seen[o]++) * 65536 + o, order[q]};
         for (int i = 0; i < 3; ++i) brows[b][3 * o + i] = cd[c * 30 + i * 10 + a];
       }
       std::sort(key.begin(), key.end());
