@@ -66,6 +66,14 @@ int default_p2_block_vals() {
   return 6144;
 }
 
+int default_chain_threads() {
+  if (const char* e = std::getenv("DFA_CHAIN_THREADS")) {
+    const int v = std::atoi(e);
+    if (v == 256 || v == 512) return v;
+  }
+  return 512;
+}
+
 int default_block_rows(int gdim) {
   if (tl_p1_rows) return tl_p1_rows;
   if (const char* e = std::getenv("DFA_BLOCK_ROWS")) {
@@ -163,13 +171,21 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
             t.node_kdof = h->plan.upload(P.node_kdof, s);
             t.node_dof = h->plan.upload(P.node_dof, s);
             t.cell_ptr = h->plan.upload(P.cell_ptr, s);
-            {
-              std::vector<uint64_t> rec(2 * P.cell_ids.size());
-              for (size_t k = 0; k < P.cell_ids.size(); ++k) {
-                rec[2 * k] = P.cell_ids[k];
-                rec[2 * k + 1] = P.cell_rank[k];
-              }
-              t.cell_rec = h->plan.upload(rec, s);
+            // host arrays stay alive until the stream synchronisation below
+            std::vector<uint64_t> rec(2 * P.cell_ids.size());
+            for (size_t k = 0; k < P.cell_ids.size(); ++k) {
+              rec[2 * k] = P.cell_ids[k];
+              rec[2 * k + 1] = P.cell_rank[k];
+            }
+            t.cell_rec = h->plan.upload(rec, s);
+            dfa::P1ChainPlan C;
+            const char* ce = std::getenv("DFA_P1_CHAINS");
+            if (gdim == 3 && !(ce && std::atoi(ce) == 0)) {
+              dfa::build_p1_chain_plan(P, C);
+              t.max_chains = C.max_chains;
+              t.chain_threads = default_chain_threads();
+              t.chain_ptr = h->plan.upload(C.chain_ptr, s);
+              t.chain_rec = h->plan.upload(C.chain_rec, s);
             }
             t.row_ptr_b = h->plan.upload(P.row_ptr, s);
             t.row_dof = h->plan.upload(P.row_dof, s);

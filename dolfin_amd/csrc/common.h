@@ -52,6 +52,24 @@ struct P1BlockPlan {
   std::vector<uint64_t> runs;
 };
 
+// Edge chains of a tetrahedral block plan. The cell appearances of each block are grouped
+// into chains around an edge (p, q): cell k of a chain is (p, q, r_k, r_{k+1}), so
+// consecutive cells share the face (p, q, r_{k+1}) and a chain of m cells touches m + 3
+// nodes. One lane walks one chain: it reads every node once and sums the contributions to
+// (p, q), (p, r_k), (q, r_k) and the load vector in registers before adding them to LDS.
+// Record (12 x uint32): [0] p | q << 16, [1..3] r_0 .. r_5 as 16-bit pairs, [4] r_6 | m << 16,
+// [5..10] one byte per emitted pair, rank(u -> v) | rank(v -> u) << 4, in the order
+// (p, r_k), (q, r_k), (r_k, r_{k+1}) for k = 0..5, then (p, r_m), (q, r_m), (p, q) at
+// bytes 18, 19, 20; [11] unused. Chains are sorted by length (longest first) per block.
+constexpr int kChainLen = 6;
+constexpr int kChainWords = 12;
+struct P1ChainPlan {
+  int32_t max_chains = 0;
+  std::vector<int32_t> chain_ptr;   // [nb+1]
+  std::vector<uint32_t> chain_rec;  // kChainWords per chain
+};
+void build_p1_chain_plan(const P1BlockPlan& plan, P1ChainPlan& chains);
+
 // Throws std::runtime_error on inconsistent input or exceeded limits.
 void build_p1_block_plan(int32_t gdim, int64_t num_vertices, int64_t num_cells,
                          const int32_t* cells, const int32_t* cell_dofs,

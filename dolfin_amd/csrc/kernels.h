@@ -38,6 +38,11 @@ struct P1BlockTopo {
   int32_t threads = 256;               // workgroup size of the blocked kernels (256 or 512)
   bool wide_nodes = true;              // 48-byte LDS node records (kappa, f, Dirichlet value inline)
   int32_t max_cells = 0;               // largest cell-appearance list of a block
+  // edge chains (tetrahedra): kChainWords uint32 per chain, chain_ptr[nb + 1]
+  int32_t max_chains = 0;
+  int32_t chain_threads = 256;
+  const int32_t* chain_ptr = nullptr;
+  const uint32_t* chain_rec = nullptr;
   bool kdof_is_vertex = false;
   const int32_t* node_ptr = nullptr;
   const int32_t* node_vertex = nullptr;

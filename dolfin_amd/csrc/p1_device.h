@@ -498,5 +498,211 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
 }
 
 
+// ---------------------------------------------------------------------------------------
+// Edge-chain scatter for the tetrahedral kappa-Poisson operator (+ load vector) without
+// Dirichlet rows, strided LDS rows, 48-byte node records. A lane walks one chain of cells
+// (p, q, r_k, r_{k+1}), k < m, around the edge (p, q): it reads each of the m + 3 nodes once,
+// keeps the sums for (p, q), (p, r_k), (q, r_k) and the load vector in registers until they
+// are complete and adds every finished entry to each owned row at the recorded ranks. The
+// diagonal is minus the row sum at write-out (the element rows sum to zero).
+// ---------------------------------------------------------------------------------------
+// The matrix walk of one chain (see above). K(p, q), K(p, r_k), K(q, r_k) are summed in
+// registers while r_k slides; every finished entry is added once per owned row of the pair.
+__device__ __forceinline__ void chain_walk_matrix(const uint32_t (&d)[kChainWords],
+                                                  const double* snode, double* svals, uint32_t nr) {
+  const uint32_t lp = d[0] & 0xffffu, lq = d[0] >> 16;
+  const int m = static_cast<int>(d[4] >> 16);
+  auto rid = [&](int k) { return (d[1 + k / 2] >> (16 * (k & 1))) & 0xffffu; };
+  auto rank_byte = [&](int j) { return (d[5 + j / 4] >> (8 * (j % 4))) & 0xffu; };
+  // the finished entry (u, v) = (v, u) with value w, into every owned row of the pair
+  auto emit = [&](uint32_t u, uint32_t v, uint32_t byte, double w) {
+    if (u < nr) atomic_add_lds(svals + u * kRowStride + (byte & 15u), w);
+    if (v < nr) atomic_add_lds(svals + v * kRowStride + (byte >> 4), w);
+  };
+  // node record: x, y | z, kappa | f
+  auto xy = [&](uint32_t l) { return *reinterpret_cast<const double2*>(snode + l * 6); };
+  auto zk = [&](uint32_t l) { return *reinterpret_cast<const double2*>(snode + l * 6 + 2); };
+
+  // fixed over the chain: p, e1 = q - p, kappa_p + kappa_q
+  const double2 pxy = xy(lp), pzk = zk(lp), qxy = xy(lq), qzk = zk(lq);
+  const double px = pxy.x, py = pxy.y, pz = pzk.x;
+  const double e1x = qxy.x - px, e1y = qxy.y - py, e1z = qzk.x - pz;
+  const double kpq = pzk.y + qzk.y;
+  // sliding: e2 = r_k - p and kappa of r_k
+  uint32_t cur = rid(0);
+  double e2x, e2y, e2z, kr;
+  {
+    const double2 rxy = xy(cur), rzk = zk(cur);
+    e2x = rxy.x - px;
+    e2y = rxy.y - py;
+    e2z = rzk.x - pz;
+    kr = rzk.y;
+  }
+  double apq = 0.0, cp = 0.0, cq = 0.0;
+#pragma unroll
+  for (int k = 0; k < kChainLen; ++k) {
+    if (k < m) {
+      const uint32_t ln = rid(k + 1);
+      const double2 nxy = xy(ln), nzk = zk(ln);
+      const double e3x = nxy.x - px, e3y = nxy.y - py, e3z = nzk.x - pz;
+      // cofactor gradients of q, r_k, r_{k+1}: g1 = e2 x e3, g2 = e3 x e1, g3 = e1 x e2
+      const double g1x = e2y * e3z - e2z * e3y, g1y = e2z * e3x - e2x * e3z, g1z = e2x * e3y - e2y * e3x;
+      const double g2x = e3y * e1z - e3z * e1y, g2y = e3z * e1x - e3x * e1z, g2z = e3x * e1y - e3y * e1x;
+      const double g3x = e1y * e2z - e1z * e2y, g3y = e1z * e2x - e1x * e2z, g3z = e1x * e2y - e1y * e2x;
+      const double det = e1x * g1x + e1y * g1y + e1z * g1z;
+      // kappa-Poisson: K_ab = mean(kappa) / (6 |det|) g_a . g_b, g_p = -(g1 + g2 + g3)
+      const double w = (kpq + kr + nzk.y) * (1.0 / 24.0) * recip(fabs(det));
+      const double g11 = fma(g1x, g1x, fma(g1y, g1y, g1z * g1z));
+      const double g22 = fma(g2x, g2x, fma(g2y, g2y, g2z * g2z));
+      const double g33 = fma(g3x, g3x, fma(g3y, g3y, g3z * g3z));
+      const double g12 = fma(g1x, g2x, fma(g1y, g2y, g1z * g2z));
+      const double g13 = fma(g1x, g3x, fma(g1y, g3y, g1z * g3z));
+      const double g23 = fma(g2x, g3x, fma(g2y, g3y, g2z * g3z));
+      apq -= w * (g11 + g12 + g13);
+      cp -= w * (g12 + g22 + g23);
+      cq += w * g12;
+      // r_k is in no later cell of the chain: its entries with p and q are complete
+      emit(lp, cur, rank_byte(3 * k), cp);
+      emit(lq, cur, rank_byte(3 * k + 1), cq);
+      emit(cur, ln, rank_byte(3 * k + 2), w * g23);
+      cp = -w * (g13 + g23 + g33);
+      cq = w * g13;
+      e2x = e3x;
+      e2y = e3y;
+      e2z = e3z;
+      kr = nzk.y;
+      cur = ln;
+    }
+  }
+  emit(lp, cur, rank_byte(18), cp);
+  emit(lq, cur, rank_byte(19), cq);
+  emit(lp, lq, rank_byte(20), apq);
+}
+
+// The load-vector walk of one chain: b_a += |det| / 120 (f_a + sum of the cell's f). The
+// contributions to p and q are kept as sum(w F) + f_{p,q} sum(w) until the chain ends; those
+// to r_k are complete when r_k leaves the window. A second, light pass over the same chain
+// keeps the matrix walk's register footprint (and so the workgroups per CU) unchanged.
+__device__ __forceinline__ void chain_walk_vector(const uint32_t (&d)[kChainWords],
+                                                  const double* snode, double* sbv, uint32_t nr) {
+  const uint32_t lp = d[0] & 0xffffu, lq = d[0] >> 16;
+  const int m = static_cast<int>(d[4] >> 16);
+  auto rid = [&](int k) { return (d[1 + k / 2] >> (16 * (k & 1))) & 0xffffu; };
+  auto emit_b = [&](uint32_t u, double w) {
+    if (u < nr) atomic_add_lds(sbv + u, w);
+  };
+  auto xy = [&](uint32_t l) { return *reinterpret_cast<const double2*>(snode + l * 6); };
+  auto zc = [&](uint32_t l) { return snode[l * 6 + 2]; };
+  auto fv = [&](uint32_t l) { return snode[l * 6 + 4]; };
+  const double2 pxy = xy(lp), qxy = xy(lq);
+  const double px = pxy.x, py = pxy.y, pz = zc(lp);
+  const double e1x = qxy.x - px, e1y = qxy.y - py, e1z = zc(lq) - pz;
+  const double fp = fv(lp), fq = fv(lq), fpq = fp + fq;
+  uint32_t cur = rid(0);
+  double e2x, e2y, e2z, fr;
+  {
+    const double2 rxy = xy(cur);
+    e2x = rxy.x - px;
+    e2y = rxy.y - py;
+    e2z = zc(cur) - pz;
+    fr = fv(cur);
+  }
+  double swf = 0.0, sw = 0.0, cb = 0.0;
+#pragma unroll
+  for (int k = 0; k < kChainLen; ++k) {
+    if (k < m) {
+      const uint32_t ln = rid(k + 1);
+      const double2 nxy = xy(ln);
+      const double e3x = nxy.x - px, e3y = nxy.y - py, e3z = zc(ln) - pz;
+      const double fn = fv(ln);
+      const double det = e1x * (e2y * e3z - e2z * e3y) + e1y * (e2z * e3x - e2x * e3z) +
+                         e1z * (e2x * e3y - e2y * e3x);
+      const double wb = fabs(det) * (1.0 / 120.0);
+      const double F = fpq + fr + fn;
+      swf = fma(wb, F, swf);
+      sw += wb;
+      emit_b(cur, fma(wb, F + fr, cb));
+      cb = wb * (F + fn);
+      e2x = e3x;
+      e2y = e3y;
+      e2z = e3z;
+      fr = fn;
+      cur = ln;
+    }
+  }
+  emit_b(cur, cb);
+  emit_b(lp, fma(fp, sw, swf));
+  emit_b(lq, fma(fq, sw, swf));
+}
+
+template <bool WANT_B, int T, int RR>
+__device__ __forceinline__ void p1_chain_body(const P1BlockTopo& t, const P1Pass& p,
+                                              const LdsLayout& L, unsigned char* lds, int blk) {
+  double* snode = reinterpret_cast<double*>(lds + L.node);
+  double* svals = reinterpret_cast<double*>(lds + L.vals);
+  double* sbv = reinterpret_cast<double*>(lds + L.bv);
+  int32_t* srdof = reinterpret_cast<int32_t*>(lds + L.rdof);
+  int32_t* srbase = reinterpret_cast<int32_t*>(lds + L.rbase);
+  int8_t* sdiag = reinterpret_cast<int8_t*>(lds + L.diag);
+  const int tid = threadIdx.x;
+  // chain records first, so that their latency overlaps the node gather
+  const int c0 = t.chain_ptr[blk], nch = t.chain_ptr[blk + 1] - c0;
+  const uint4* __restrict__ crec = reinterpret_cast<const uint4*>(t.chain_rec) + 3 * static_cast<int64_t>(c0);
+  uint32_t d[RR][kChainWords];
+#pragma unroll
+  for (int j = 0; j < RR; ++j) {
+    const int i = tid + j * T;
+#pragma unroll
+    for (int h = 0; h < 3; ++h) {
+      const uint4 w = i < nch ? crec[3 * i + h] : make_uint4(0, 0, 0, 0);
+      d[j][4 * h] = w.x;
+      d[j][4 * h + 1] = w.y;
+      d[j][4 * h + 2] = w.z;
+      d[j][4 * h + 3] = w.w;
+    }
+  }
+  gather_nodes<3, T, true, WANT_B, false, false, true>(t, p.x, p.kappa, p.f, nullptr, nullptr, blk,
+                                                       snode, nullptr, nullptr, nullptr, nullptr);
+  const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
+  for (int i = tid; i < (nr * kRowStride + 1) / 2; i += T)
+    reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
+  for (int i = tid; i < nr; i += T) {
+    const int32_t dof = t.row_dof[r0 + i];
+    srdof[i] = dof;
+    const int32_t b0 = t.csr_row_ptr[dof];
+    srbase[i] = b0;
+    srbase[nr + i] = t.csr_row_ptr[dof + 1] - b0;
+    sdiag[i] = static_cast<int8_t>(t.row_diag[r0 + i]);
+    if constexpr (WANT_B) sbv[i] = 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < RR; ++j)
+    if (tid + j * T < nch) {
+      chain_walk_matrix(d[j], snode, svals, static_cast<uint32_t>(nr));
+      if constexpr (WANT_B) chain_walk_vector(d[j], snode, sbv, static_cast<uint32_t>(nr));
+    }
+  __syncthreads();
+
+  // write-out: 16 lanes per row, diagonal = minus the row sum
+  const int grp = tid >> 4, ln = tid & 15;
+  for (int i = grp; i < nr; i += T / 16) {
+    const int base = srbase[i];
+    const int len = srbase[nr + i];
+    double v = ln < len ? svals[i * kRowStride + ln] : 0.0;
+    const double rs = row16_sum(v);
+    if (ln == sdiag[i]) v = -rs;
+    if (ln < len) {
+      if (p.add) v += p.values[base + ln];
+      p.values[base + ln] = v;
+    }
+  }
+  if constexpr (WANT_B)
+    for (int i = tid; i < nr; i += T) {
+      const int32_t dof = srdof[i];
+      p.b[dof] = p.add ? p.b[dof] + sbv[i] : sbv[i];
+    }
+}
+
 }  // namespace p1dev
 }  // namespace dfa

@@ -17,6 +17,8 @@
 //             value is written to HBM exactly once, coalesced.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 #include "p1_device.h"
@@ -78,6 +80,15 @@ template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE
 __global__ __launch_bounds__(T) void p1_blocked_kernel(P1BlockTopo t, P1Pass p, LdsLayout L) {
   extern __shared__ __align__(16) unsigned char lds[];
   p1_block_body<D, AFORM, WANT_B, BC, STRIDED, T, WIDE, RR>(t, p, L, lds, xcd_block(t.num_blocks));
+}
+
+// 512 threads, one chain each: six waves per SIMD, i.e. three resident workgroups per CU (at
+// most 80 VGPRs; the load vector takes a second, light walk for that reason).
+template <bool WANT_B, int T, int RR>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(RR == 1 && T == 512 ? 6 : 1)))
+void p1_chain_kernel(P1BlockTopo t, P1Pass p, LdsLayout L) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  p1_chain_body<WANT_B, T, RR>(t, p, L, lds, xcd_block(t.num_blocks));
 }
 
 // Adjoint of J = u^T K(kappa, x) u - F(f, x)^T u on the same blocks: every owned vertex
@@ -234,9 +245,41 @@ hipError_t launch_blocked_tu(const P1BlockTopo& t, const P1Pass& p, hipStream_t 
   return launch_blocked_tt<D, AFORM, WANT_B, BC, STRIDED, 256, WIDE, 0>(t, p, s);
 }
 
+// The edge-chain kernel applies to tetrahedral kappa-Poisson (+ load) without Dirichlet rows
+// when the plan has chains and every row fits the strided LDS layout.
+bool chains_fit(const P1BlockTopo& t, const P1Pass& p) {
+  return t.chain_ptr && t.chain_rec && p.gdim == 3 && p.a_form == DFA_FORM_P1_POISSON && !p.bc &&
+         t.max_row_len <= kMaxRank && t.wide_nodes && t.max_chains <= 1024;
+}
+
+template <bool WANT_B, int T, int RR>
+hipError_t launch_chain_tt(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
+  const LdsLayout L = lds_layout(3, t.max_nodes, t.max_rows, t.max_nnz, WANT_B, false, true, true);
+  auto kern = p1_chain_kernel<WANT_B, T, RR>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, L.total);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(t.num_blocks), dim3(T), L.total, s, t, p, L);
+  return hipGetLastError();
+}
+
+template <bool WANT_B>
+hipError_t launch_chain_t(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
+  if (t.chain_threads == 512) {
+    if (t.max_chains <= 512) return launch_chain_tt<WANT_B, 512, 1>(t, p, s);
+    return launch_chain_tt<WANT_B, 512, 2>(t, p, s);
+  }
+  if (t.max_chains <= 256) return launch_chain_tt<WANT_B, 256, 1>(t, p, s);
+  if (t.max_chains <= 512) return launch_chain_tt<WANT_B, 256, 2>(t, p, s);
+  return launch_chain_tt<WANT_B, 256, 4>(t, p, s);
+}
+
 template <int D, int AFORM, bool WANT_B, bool BC>
 hipError_t launch_blocked_t(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
   if (t.num_blocks == 0) return hipSuccess;
+  if constexpr (D == 3 && AFORM == DFA_FORM_P1_POISSON && !BC) {
+    if (chains_fit(t, p)) return launch_chain_t<WANT_B>(t, p, s);
+  }
   const bool strided = t.max_row_len <= kMaxRank;
   if (strided) {
     if (t.wide_nodes) return launch_blocked_tu<D, AFORM, WANT_B, BC, true, true>(t, p, s);

@@ -2,6 +2,7 @@
 // Runs once per (mesh, dofmap), like the first call of AssemblerBase::init_global_tensor
 // (dolfin/fem/AssemblerBase.cpp:137-285).
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cstring>
 #include <functional>
@@ -389,6 +390,162 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
       std::copy(bdiag[b].begin(), bdiag[b].end(), P.row_diag.begin() + P.row_ptr[b]);
     }
   });
+}
+
+// Chains are grown greedily: an unassigned cell seeds one chain per edge, walking across the
+// faces that contain the edge in both directions through unassigned cells of the block, and
+// the longest (capped at kChainLen) is kept. Seeds follow the block's cell order.
+void build_p1_chain_plan(const P1BlockPlan& P, P1ChainPlan& C) {
+  if (P.nvc != 4) throw std::runtime_error("edge chains need tetrahedra");
+  const int32_t nb = P.num_blocks;
+  std::vector<std::vector<uint32_t>> brec(nb);
+  std::atomic<int> failed{0};
+  parallel_for(nb, [&](int64_t b0, int64_t b1, int) {
+    for (int64_t b = b0; b < b1; ++b) {
+      const int32_t c0 = P.cell_ptr[b], ncb = P.cell_ptr[b + 1] - c0;
+      std::vector<std::array<uint32_t, 4>> cl(ncb);
+      for (int32_t k = 0; k < ncb; ++k)
+        for (int a = 0; a < 4; ++a) cl[k][a] = static_cast<uint32_t>((P.cell_ids[c0 + k] >> (16 * a)) & 0xffffu);
+      // faces: sorted (key, cell) list, key = sorted local ids of the three vertices
+      auto fkey = [](uint32_t a, uint32_t b, uint32_t c) {
+        if (a > b) std::swap(a, b);
+        if (b > c) std::swap(b, c);
+        if (a > b) std::swap(a, b);
+        return static_cast<uint64_t>(a) << 32 | static_cast<uint64_t>(b) << 16 | c;
+      };
+      std::vector<std::pair<uint64_t, int32_t>> faces;
+      faces.reserve(4 * static_cast<size_t>(ncb));
+      for (int32_t k = 0; k < ncb; ++k)
+        for (int o = 0; o < 4; ++o) {
+          uint32_t v[3];
+          int n = 0;
+          for (int a = 0; a < 4; ++a)
+            if (a != o) v[n++] = cl[k][a];
+          faces.emplace_back(fkey(v[0], v[1], v[2]), k);
+        }
+      std::sort(faces.begin(), faces.end());
+      std::vector<char> used(ncb, 0);
+      // the unassigned cell other than k across face (p, q, r), or -1
+      auto across = [&](int32_t k, uint32_t p, uint32_t q, uint32_t r) {
+        const uint64_t key = fkey(p, q, r);
+        auto it = std::lower_bound(faces.begin(), faces.end(), std::make_pair(key, int32_t{-1}));
+        for (; it != faces.end() && it->first == key; ++it)
+          if (it->second != k && !used[it->second]) return it->second;
+        return int32_t{-1};
+      };
+      auto fourth = [&](int32_t k, uint32_t p, uint32_t q, uint32_t r) {
+        for (int a = 0; a < 4; ++a) {
+          const uint32_t v = cl[k][a];
+          if (v != p && v != q && v != r) return v;
+        }
+        return uint32_t{0xffffffffu};
+      };
+      // rank(u -> v) from any cell appearance containing both
+      auto rank_in = [&](int32_t k, uint32_t u, uint32_t v) {
+        int au = -1, av = -1;
+        for (int a = 0; a < 4; ++a) {
+          if (cl[k][a] == u) au = a;
+          if (cl[k][a] == v) av = a;
+        }
+        if (au < 0 || av < 0) return uint32_t{0xffu};
+        return static_cast<uint32_t>((P.cell_rank[c0 + k] >> (4 * (au * 4 + av))) & 15u);
+      };
+      struct Chain {
+        uint32_t p, q;
+        std::vector<uint32_t> r;   // m + 1 nodes
+        std::vector<int32_t> cells;
+      };
+      std::vector<Chain> chains;
+      static const int EA[6] = {0, 0, 0, 1, 1, 2}, EB[6] = {1, 2, 3, 2, 3, 3};
+      for (int32_t k0 = 0; k0 < ncb; ++k0) {
+        if (used[k0]) continue;
+        Chain best;
+        for (int e = 0; e < 6; ++e) {
+          Chain ch;
+          ch.p = cl[k0][EA[e]];
+          ch.q = cl[k0][EB[e]];
+          uint32_t rs[2];
+          int n = 0;
+          for (int a = 0; a < 4; ++a)
+            if (a != EA[e] && a != EB[e]) rs[n++] = cl[k0][a];
+          std::vector<int32_t> fw{k0};
+          std::vector<uint32_t> fr{rs[0], rs[1]};
+          used[k0] = 1;
+          int32_t cur = k0;
+          while (static_cast<int>(fw.size()) < kChainLen) {  // forward across (p, q, r_last)
+            const int32_t c = across(cur, ch.p, ch.q, fr.back());
+            if (c < 0) break;
+            const uint32_t nx = fourth(c, ch.p, ch.q, fr.back());
+            fw.push_back(c);
+            fr.push_back(nx);
+            used[c] = 1;
+            cur = c;
+          }
+          cur = k0;
+          while (static_cast<int>(fw.size()) < kChainLen) {  // backward across (p, q, r_first)
+            const int32_t c = across(cur, ch.p, ch.q, fr.front());
+            if (c < 0) break;
+            const uint32_t nx = fourth(c, ch.p, ch.q, fr.front());
+            fw.insert(fw.begin(), c);
+            fr.insert(fr.begin(), nx);
+            used[c] = 1;
+            cur = c;
+          }
+          for (int32_t c : fw) used[c] = 0;
+          ch.cells = fw;
+          ch.r = fr;
+          if (ch.cells.size() > best.cells.size()) best = ch;
+          if (static_cast<int>(best.cells.size()) == kChainLen) break;
+        }
+        for (int32_t c : best.cells) used[c] = 1;
+        chains.push_back(std::move(best));
+      }
+      std::stable_sort(chains.begin(), chains.end(),
+                       [](const Chain& a, const Chain& b) { return a.cells.size() > b.cells.size(); });
+      auto& out = brec[b];
+      out.assign(chains.size() * kChainWords, 0u);
+      for (size_t i = 0; i < chains.size(); ++i) {
+        const Chain& ch = chains[i];
+        const int m = static_cast<int>(ch.cells.size());
+        uint32_t* w = out.data() + i * kChainWords;
+        uint32_t ids[9];
+        ids[0] = ch.p;
+        ids[1] = ch.q;
+        for (int k = 0; k < 7; ++k) ids[2 + k] = k <= m ? ch.r[k] : ch.p;
+        w[0] = ids[0] | ids[1] << 16;
+        w[1] = ids[2] | ids[3] << 16;
+        w[2] = ids[4] | ids[5] << 16;
+        w[3] = ids[6] | ids[7] << 16;
+        w[4] = ids[8] | static_cast<uint32_t>(m) << 16;
+        uint8_t bytes[24] = {0};
+        auto pair = [&](int j, int32_t cell, uint32_t u, uint32_t v) {
+          const uint32_t a = rank_in(cell, u, v), c = rank_in(cell, v, u);
+          if (a > 15 || c > 15) failed = 1;
+          bytes[j] = static_cast<uint8_t>((a & 15u) | (c & 15u) << 4);
+        };
+        for (int k = 0; k < m; ++k) {
+          pair(3 * k, ch.cells[k], ch.p, ch.r[k]);
+          pair(3 * k + 1, ch.cells[k], ch.q, ch.r[k]);
+          pair(3 * k + 2, ch.cells[k], ch.r[k], ch.r[k + 1]);
+        }
+        pair(18, ch.cells[m - 1], ch.p, ch.r[m]);
+        pair(19, ch.cells[m - 1], ch.q, ch.r[m]);
+        pair(20, ch.cells[0], ch.p, ch.q);
+        std::memcpy(w + 5, bytes, 24);
+      }
+    }
+  });
+  if (failed) throw std::runtime_error("edge chains: inconsistent cell records");
+  C.chain_ptr.assign(nb + 1, 0);
+  C.max_chains = 0;
+  for (int32_t b = 0; b < nb; ++b) {
+    const int32_t n = static_cast<int32_t>(brec[b].size() / kChainWords);
+    C.chain_ptr[b + 1] = C.chain_ptr[b] + n;
+    C.max_chains = std::max(C.max_chains, n);
+  }
+  C.chain_rec.resize(static_cast<size_t>(C.chain_ptr[nb]) * kChainWords);
+  for (int32_t b = 0; b < nb; ++b)
+    std::copy(brec[b].begin(), brec[b].end(), C.chain_rec.begin() + static_cast<size_t>(C.chain_ptr[b]) * kChainWords);
 }
 
 }  // namespace dfa

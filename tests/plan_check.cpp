@@ -107,6 +107,7 @@ extern "C" int plan_check_p1(int32_t gdim, int64_t nv, int64_t nc, const int32_t
     stats[2] = P.max_rows;
     stats[3] = P.max_nnz;
     stats[4] = static_cast<int32_t>(total_app);
+    stats[5] = P.max_cells;
     return 0;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "plan_check: %s\n", e.what());
@@ -216,6 +217,145 @@ extern "C" int plan_check_vp2(int64_t nc, const int32_t* cells, const int32_t* c
     return 0;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "plan_check_vp2: %s\n", e.what());
+    return -100;
+  }
+}
+
+
+// Edge-chain walk of the tetrahedral kappa-Poisson kernel: every block's chains are walked as
+// a lane does it (sliding window over r_k, contributions to (p, q), (p, r_k), (q, r_k) and b
+// carried until complete, one LDS add per owned direction at the recorded ranks, diagonal =
+// minus the row sum at write-out). Returns the assembled CSR values and load vector.
+extern "C" int plan_check_p1_chains(int64_t nv, int64_t nc, const int32_t* cells,
+                                    const int32_t* cd, const double* x, const double* kappa,
+                                    const double* f, int64_t ndofs, int32_t target_rows,
+                                    double* values, double* bvec, int64_t nnz_cap, int32_t* stats) {
+  try {
+    dfa::HostCSR csr;
+    dfa::build_csr(nc, 4, cd, ndofs, false, csr);
+    if (static_cast<int64_t>(csr.col.size()) > nnz_cap) return -10;
+    dfa::P1BlockPlan P;
+    dfa::build_p1_block_plan(3, nv, nc, cells, cd, cd, x, csr, target_rows, P);
+    dfa::P1ChainPlan C;
+    dfa::build_p1_chain_plan(P, C);
+    if (static_cast<int32_t>(C.chain_ptr.size()) != P.num_blocks + 1) return -1;
+    int64_t cells_walked = 0, nchains = 0;
+    std::vector<int> covered(ndofs, 0);
+    for (int b = 0; b < P.num_blocks; ++b) {
+      const int n0 = P.node_ptr[b], nn = P.node_ptr[b + 1] - n0;
+      const int r0 = P.row_ptr[b], nr = P.row_ptr[b + 1] - r0;
+      if (nr > P.max_rows || nn > P.max_nodes) return -2;
+      std::vector<double> sv(static_cast<size_t>(nr) * 17, 0.0), sb(nr, 0.0);
+      auto node = [&](uint32_t l, double* pc, double& k, double& fv) {
+        if (static_cast<int>(l) >= nn) throw std::runtime_error("chain node out of range");
+        const int32_t v = P.node_vertex[n0 + l];
+        for (int q = 0; q < 3; ++q) pc[q] = x[static_cast<int64_t>(v) * 3 + q];
+        k = kappa[P.node_kdof[n0 + l]];
+        fv = f[P.node_kdof[n0 + l]];
+      };
+      auto add = [&](uint32_t u, uint32_t v, uint32_t byte, double w) {
+        if (static_cast<int>(u) < nr) {
+          const int rlen = csr.row_ptr[P.row_dof[r0 + u] + 1] - csr.row_ptr[P.row_dof[r0 + u]];
+          if (static_cast<int>(byte & 15u) >= rlen) throw std::runtime_error("rank beyond row");
+          sv[u * 17 + (byte & 15u)] += w;
+        }
+        if (static_cast<int>(v) < nr) {
+          const int rlen = csr.row_ptr[P.row_dof[r0 + v] + 1] - csr.row_ptr[P.row_dof[r0 + v]];
+          if (static_cast<int>(byte >> 4) >= rlen) throw std::runtime_error("rank beyond row");
+          sv[v * 17 + (byte >> 4)] += w;
+        }
+      };
+      auto addb = [&](uint32_t u, double w) {
+        if (static_cast<int>(u) < nr) sb[u] += w;
+      };
+      int prev_m = 1 << 30;
+      for (int c = C.chain_ptr[b]; c < C.chain_ptr[b + 1]; ++c) {
+        const uint32_t* w = C.chain_rec.data() + static_cast<size_t>(c) * dfa::kChainWords;
+        const uint32_t lp = w[0] & 0xffffu, lq = w[0] >> 16;
+        uint32_t lr[7];
+        for (int k = 0; k < 7; ++k) lr[k] = (w[1 + k / 2] >> (16 * (k & 1))) & 0xffffu;
+        const int m = static_cast<int>(w[4] >> 16);
+        if (m < 1 || m > dfa::kChainLen) return -3;
+        if (m > prev_m) return -4;  // longest first
+        prev_m = m;
+        auto byte = [&](int j) { return (w[5 + j / 4] >> (8 * (j % 4))) & 0xffu; };
+        double P_[3], Q_[3], R_[3], N_[3], kp, kq, kr, kn, fp, fq, fr, fn;
+        node(lp, P_, kp, fp);
+        node(lq, Q_, kq, fq);
+        node(lr[0], R_, kr, fr);
+        double kpq = 0, bp = 0, bq = 0, cp = 0, cq = 0, cb = 0;
+        uint32_t cur = lr[0];
+        for (int k = 0; k < m; ++k) {
+          node(lr[k + 1], N_, kn, fn);
+          const double* pts[4] = {P_, Q_, R_, N_};
+          double e[3][3], cf[4][3];
+          for (int j = 0; j < 3; ++j)
+            for (int q = 0; q < 3; ++q) e[j][q] = pts[j + 1][q] - pts[0][q];
+          auto cross = [](const double* a, const double* b2, double* o) {
+            o[0] = a[1] * b2[2] - a[2] * b2[1];
+            o[1] = a[2] * b2[0] - a[0] * b2[2];
+            o[2] = a[0] * b2[1] - a[1] * b2[0];
+          };
+          cross(e[1], e[2], cf[1]);
+          cross(e[2], e[0], cf[2]);
+          cross(e[0], e[1], cf[3]);
+          for (int q = 0; q < 3; ++q) cf[0][q] = -(cf[1][q] + cf[2][q] + cf[3][q]);
+          const double det = e[0][0] * cf[1][0] + e[0][1] * cf[1][1] + e[0][2] * cf[1][2];
+          const double vol = std::fabs(det) / 6.0;
+          const double kb = 0.25 * (kp + kq + kr + kn);
+          double K[4][4];
+          for (int a = 0; a < 4; ++a)
+            for (int c2 = 0; c2 < 4; ++c2)
+              K[a][c2] = kb * vol * (cf[a][0] * cf[c2][0] + cf[a][1] * cf[c2][1] + cf[a][2] * cf[c2][2]) / (det * det);
+          const double F = fp + fq + fr + fn;
+          const double be[4] = {vol / 20 * (F + fp), vol / 20 * (F + fq), vol / 20 * (F + fr), vol / 20 * (F + fn)};
+          kpq += K[0][1];
+          bp += be[0];
+          bq += be[1];
+          cp += K[0][2];
+          cq += K[1][2];
+          cb += be[2];
+          add(lp, lr[k], byte(3 * k), cp);
+          add(lq, lr[k], byte(3 * k + 1), cq);
+          addb(lr[k], cb);
+          add(lr[k], lr[k + 1], byte(3 * k + 2), K[2][3]);
+          cp = K[0][3];
+          cq = K[1][3];
+          cb = be[3];
+          for (int q = 0; q < 3; ++q) R_[q] = N_[q];
+          kr = kn;
+          fr = fn;
+          cur = lr[k + 1];
+        }
+        add(lp, cur, byte(18), cp);
+        add(lq, cur, byte(19), cq);
+        addb(cur, cb);
+        add(lp, lq, byte(20), kpq);
+        addb(lp, bp);
+        addb(lq, bq);
+        cells_walked += m;
+        ++nchains;
+      }
+      if (cells_walked != P.cell_ptr[b + 1]) return -5;  // every appearance in one chain
+      for (int i = 0; i < nr; ++i) {
+        const int32_t d = P.row_dof[r0 + i];
+        covered[d]++;
+        const int rb = csr.row_ptr[d], len = csr.row_ptr[d + 1] - rb;
+        double rs = 0;
+        for (int s2 = 0; s2 < len; ++s2) rs += sv[i * 17 + s2];
+        for (int s2 = 0; s2 < len; ++s2) values[rb + s2] = (s2 == P.row_diag[r0 + i]) ? -rs : sv[i * 17 + s2];
+        bvec[d] = sb[i];
+      }
+    }
+    for (int64_t d = 0; d < ndofs; ++d)
+      if (covered[d] != 1) return -9;
+    stats[0] = P.num_blocks;
+    stats[1] = C.max_chains;
+    stats[2] = static_cast<int32_t>(nchains);
+    stats[3] = static_cast<int32_t>(cells_walked);
+    return 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "plan_check_p1_chains: %s\n", e.what());
     return -100;
   }
 }

@@ -111,6 +111,38 @@ def test_blocked_plan_walk(oracle, plan_lib, case, target):
     assert stats[2] <= target
 
 
+@pytest.mark.parametrize("n,target,perm", [(6, 32, False), (7, 160, True), (12, 160, False)])
+def test_chain_plan_walk(oracle, plan_lib, n, target, perm):
+    # edge chains of the tetrahedral kappa-Poisson kernel, walked on the CPU exactly as a lane
+    # does it, must reproduce the oracle's stiffness matrix and load vector
+    rng = np.random.default_rng(5)
+    mesh = dfa.UnitCubeMesh(n, n + 1, n - 1)
+    dfa.jitter_interior(mesh, 0.2 / n, rng)
+    V = dfa.FunctionSpace(mesh, "P", 1,
+                          node_permutation=rng.permutation(mesh.num_vertices()) if perm else None)
+    kappa = rng.uniform(0.5, 2.0, V.dim())
+    f = rng.uniform(-1.0, 1.0, V.dim())
+    cells = np.ascontiguousarray(mesh.cells(), np.int32)
+    cd = np.ascontiguousarray(V.cell_dofs(), np.int32)
+    x = np.ascontiguousarray(mesh.coordinates())
+    orp, ocol = oracle.sparsity(V.cell_dofs(), V.dim())
+    vals = np.zeros(ocol.shape[0])
+    b = np.zeros(V.dim())
+    stats = np.zeros(8, np.int32)
+    rc = plan_lib.plan_check_p1_chains(mesh.num_vertices(), mesh.num_cells(), p(cells), p(cd),
+                                       p(x), p(kappa), p(f), V.dim(), target, p(vals), p(b),
+                                       ocol.shape[0], p(stats))
+    assert rc == 0, f"chain plan check failed with code {rc}"
+    A_ref = oracle.assemble_matrix(oracle.P1_POISSON, mesh.coordinates(), mesh.cells(), cd, orp,
+                                   ocol, coeff_dofs=cd, c0=kappa)
+    b_ref = oracle.assemble_vector(oracle.P1_LOAD, mesh.coordinates(), mesh.cells(), cd, V.dim(),
+                                   coeff_dofs=cd, c0=f)
+    assert np.linalg.norm(vals - A_ref) <= 1e-12 * np.linalg.norm(A_ref)
+    assert np.linalg.norm(b - b_ref) <= 1e-12 * np.linalg.norm(b_ref)
+    # chains average well above 2 cells on a Kuhn mesh
+    assert stats[3] / stats[2] > 2.5
+
+
 def test_blocked_plan_3d_redundancy(plan_lib):
     # the work partition lists each cell in every block owning one of its dofs; on a
     # structured 3D mesh with ~256-row blocks that is well under 2x
