@@ -19,7 +19,7 @@ _CACHE = {}
 
 P1_ROWS = (128, 160, 192, 256)
 P1_THREADS = (256, 512)
-P2_VALS = (4096, 6144, 8192, 10240)
+P2_VALS = (4096, 4608, 6144, 8192)
 
 
 def set_plan_defaults(p1_rows=0, p1_threads=0, p2_vals=0):

@@ -63,7 +63,7 @@ int default_p2_block_vals() {
     const int v = std::atoi(e);
     if (v >= 1024 && v <= 16384) return v;
   }
-  return 6144;
+  return 4608;
 }
 
 int default_chain_threads() {
@@ -231,6 +231,7 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
             t.own_info = h->plan.upload(P.own_info, s);
             t.task_ptr = h->plan.upload(P.task_ptr, s);
             t.task = h->plan.upload(P.task, s);
+            t.task_nvert = h->plan.upload(P.task_nvert, s);
             t.task_cell = h->plan.upload(P.task_cell, s);
             t.blk_vals = h->plan.upload(P.blk_vals, s);
             t.row_ptr = h->d_row_ptr;

@@ -92,6 +92,7 @@ struct VP2BlockPlan {
   std::vector<int32_t> task_ptr;   // [nb+1]
   std::vector<uint64_t> task;      // 3 words per work item (see build_vp2_block_plan)
   std::vector<int32_t> task_cell;  // cell of each work item (Dirichlet path only)
+  std::vector<int32_t> task_nvert; // [nb] work items of vertex nodes (they come first)
   std::vector<int32_t> blk_vals;   // [nb] LDS values per block
 };
 

@@ -104,6 +104,7 @@ struct VP2BlockTopo {
   const int32_t* task_ptr = nullptr;
   const uint64_t* task = nullptr;
   const int32_t* task_cell = nullptr;
+  const int32_t* task_nvert = nullptr;  // [nb] vertex-node work items (first in each block)
   const int32_t* blk_vals = nullptr;
   const int32_t* row_ptr = nullptr;
   const int32_t* cell_dofs = nullptr;

@@ -301,109 +301,113 @@ __device__ __forceinline__ void lean_setup(const double (&p)[4][3], const double
   }
 }
 
-struct NodeV {
+// Node a of a work item: gradient directions w_t and, per coefficient field, the scalars
+// that fix v_t = u_t^T W (W in closed form above) at any index m:
+//   vertex i:    v_0[m] = s (m == i ? 2S + 10 c_i : 4 c_i - 2S - 2 c_m),   sum_m v_0[m] = s (24 c_i - 6S)
+//   edge (k, l): v_0[m] = 4s (1 + [m == l]) (S + c_l + c_m), v_1[m] = 4s (1 + [m == k]) (S + c_k + c_m)
+// with s = vol / 120. Only the base values c_i / c_l / c_k are kept in registers; v_t[m] is
+// formed where a compile-time m needs it.
+struct NodeW {
   double w[2][3];
-  double vm[2][4], vl[2][4];
-  double sm[2], sl[2];
+  double bm[2], bl[2];  // c_i (vertex) or c_l, c_k (edge), per field
+  int i0, i1;           // i (vertex) or l, k (edge)
 };
 
 template <bool VERT>
-__device__ __forceinline__ void node_v(const LeanGeom& g, int a, NodeV& f) {
-  const double s120 = g.vol * (1.0 / 120.0);
+__device__ __forceinline__ void node_w(const LeanGeom& g, int a, NodeW& f) {
   if constexpr (VERT) {
-    const int i = a;
+    f.i0 = a;
+    f.i1 = 0;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) f.w[0][q] = sel4(i, g.D[0][q], g.D[1][q], g.D[2][q], g.D[3][q]);
-    const double cim = sel4(i, g.cm[0], g.cm[1], g.cm[2], g.cm[3]);
-    const double cil = sel4(i, g.cl[0], g.cl[1], g.cl[2], g.cl[3]);
-    // v = 4 W[i][.] - R
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const double dd = (m == i) ? 2.0 : 1.0;
-      f.vm[0][m] = s120 * (4.0 * dd * (g.Sm + cim + g.cm[m]) - 6.0 * (g.Sm + g.cm[m]));
-      f.vl[0][m] = s120 * (4.0 * dd * (g.Sl + cil + g.cl[m]) - 6.0 * (g.Sl + g.cl[m]));
-    }
-    f.sm[0] = f.vm[0][0] + f.vm[0][1] + f.vm[0][2] + f.vm[0][3];
-    f.sl[0] = f.vl[0][0] + f.vl[0][1] + f.vl[0][2] + f.vl[0][3];
+    for (int q = 0; q < 3; ++q) f.w[0][q] = sel4(a, g.D[0][q], g.D[1][q], g.D[2][q], g.D[3][q]);
+    f.bm[0] = sel4(a, g.cm[0], g.cm[1], g.cm[2], g.cm[3]);
+    f.bl[0] = sel4(a, g.cl[0], g.cl[1], g.cl[2], g.cl[3]);
   } else {
     const int e = a - 4;
     const int k = (22 >> (2 * e)) & 3, l = (1775 >> (2 * e)) & 3;
+    f.i0 = l;
+    f.i1 = k;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       f.w[0][q] = sel4(k, g.D[0][q], g.D[1][q], g.D[2][q], g.D[3][q]);
       f.w[1][q] = sel4(l, g.D[0][q], g.D[1][q], g.D[2][q], g.D[3][q]);
     }
-    const double clm = sel4(l, g.cm[0], g.cm[1], g.cm[2], g.cm[3]);
-    const double cll = sel4(l, g.cl[0], g.cl[1], g.cl[2], g.cl[3]);
-    const double ckm = sel4(k, g.cm[0], g.cm[1], g.cm[2], g.cm[3]);
-    const double ckl = sel4(k, g.cl[0], g.cl[1], g.cl[2], g.cl[3]);
-    // u_0 = 4 e_l (gradient lambda_k), u_1 = 4 e_k (gradient lambda_l): v_t = 4 W[idx][.]
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const double d0 = (m == l) ? 8.0 : 4.0, d1 = (m == k) ? 8.0 : 4.0;
-      f.vm[0][m] = s120 * d0 * (g.Sm + clm + g.cm[m]);
-      f.vl[0][m] = s120 * d0 * (g.Sl + cll + g.cl[m]);
-      f.vm[1][m] = s120 * d1 * (g.Sm + ckm + g.cm[m]);
-      f.vl[1][m] = s120 * d1 * (g.Sl + ckl + g.cl[m]);
-    }
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      f.sm[t] = f.vm[t][0] + f.vm[t][1] + f.vm[t][2] + f.vm[t][3];
-      f.sl[t] = f.vl[t][0] + f.vl[t][1] + f.vl[t][2] + f.vl[t][3];
-    }
+    f.bm[0] = sel4(l, g.cm[0], g.cm[1], g.cm[2], g.cm[3]);
+    f.bl[0] = sel4(l, g.cl[0], g.cl[1], g.cl[2], g.cl[3]);
+    f.bm[1] = sel4(k, g.cm[0], g.cm[1], g.cm[2], g.cm[3]);
+    f.bl[1] = sel4(k, g.cl[0], g.cl[1], g.cl[2], g.cl[3]);
   }
 }
 
-// K[(i,a),(j,B)] for a compile-time node B and node kind of a.
-template <int B, bool VERT>
-__device__ __forceinline__ void node_block_lean(const LeanGeom& g, const NodeV& f, double (&K)[3][3]) {
+// v_t[M] (without the factor s) for field values S, c_M and the node's base value.
+template <bool VERT, int M>
+__device__ __forceinline__ double node_v_at(const NodeW& f, int t, double S, double cM, double base) {
+  if constexpr (VERT) {
+    return f.i0 == M ? 2.0 * S + 10.0 * base : 4.0 * base - 2.0 * S - 2.0 * cM;
+  } else {
+    const int idx = t == 0 ? f.i0 : f.i1;
+    return (idx == M ? 8.0 : 4.0) * (S + base + cM);
+  }
+}
+
+// Adds K[(i,a),(j,B)] (B compile-time) to LDS: row(i) + col(j) addresses the entry.
+template <int B, bool VERT, class Emit>
+__device__ __forceinline__ void node_block_emit(const LeanGeom& g, const NodeW& f, Emit&& emit) {
   constexpr bool VB = B < 4;
   constexpr int EK = VB ? B : (22 >> (2 * (B - 4))) & 3;
   constexpr int EL = VB ? 0 : (1775 >> (2 * (B - 4))) & 3;
-  constexpr int NS = VB ? 1 : 2;
   constexpr int NT = VERT ? 1 : 2;
-  constexpr int IDX0 = VB ? B : EL, GRD0 = VB ? B : EK;
-  constexpr int IDX1 = EK, GRD1 = EL;
-  double Sm[3][3], Sl[3][3];
-#pragma unroll
-  for (int p = 0; p < 3; ++p)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) Sm[p][q] = Sl[p][q] = 0.0;
+  const double s = g.vol * (1.0 / 120.0);
+  // z_t = sum_s alpha_ts grad lambda_{grd_s}, per field
+  double zm[2][3], zl[2][3];
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    // z = sum_s alpha_ts w'_s, then S += w_t (x) z
-    double zm[3] = {0.0, 0.0, 0.0}, zl[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int idx = s == 0 ? IDX0 : IDX1;
-      const int grd = s == 0 ? GRD0 : GRD1;
-      const double am = VB ? 4.0 * f.vm[t][idx] - f.sm[t] : 4.0 * f.vm[t][idx];
-      const double al = VB ? 4.0 * f.vl[t][idx] - f.sl[t] : 4.0 * f.vl[t][idx];
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        zm[q] = fma(am, g.D[grd][q], zm[q]);
-        zl[q] = fma(al, g.D[grd][q], zl[q]);
-      }
+    const double bm = f.bm[t], bl = f.bl[t];
+    double am0, al0, am1 = 0.0, al1 = 0.0;
+    if constexpr (VB) {
+      // u'_0 = 4 e_B - 1: alpha = 4 v_t[B] - sum_m v_t[m]
+      const double summ = VERT ? 24.0 * bm - 6.0 * g.Sm : 24.0 * (g.Sm + bm);
+      const double suml = VERT ? 24.0 * bl - 6.0 * g.Sl : 24.0 * (g.Sl + bl);
+      am0 = s * (4.0 * node_v_at<VERT, B>(f, t, g.Sm, g.cm[B], bm) - summ);
+      al0 = s * (4.0 * node_v_at<VERT, B>(f, t, g.Sl, g.cl[B], bl) - suml);
+    } else {
+      // u'_0 = 4 e_EL (direction EK), u'_1 = 4 e_EK (direction EL)
+      am0 = 4.0 * s * node_v_at<VERT, EL>(f, t, g.Sm, g.cm[EL], bm);
+      al0 = 4.0 * s * node_v_at<VERT, EL>(f, t, g.Sl, g.cl[EL], bl);
+      am1 = 4.0 * s * node_v_at<VERT, EK>(f, t, g.Sm, g.cm[EK], bm);
+      al1 = 4.0 * s * node_v_at<VERT, EK>(f, t, g.Sl, g.cl[EK], bl);
     }
 #pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        Sm[p][q] = fma(f.w[t][p], zm[q], Sm[p][q]);
-        Sl[p][q] = fma(f.w[t][p], zl[q], Sl[p][q]);
+    for (int q = 0; q < 3; ++q) {
+      zm[t][q] = am0 * g.D[EK][q];
+      zl[t][q] = al0 * g.D[EK][q];
+      if constexpr (!VB) {
+        zm[t][q] = fma(am1, g.D[EL][q], zm[t][q]);
+        zl[t][q] = fma(al1, g.D[EL][q], zl[t][q]);
       }
+    }
   }
-  const double tr = Sm[0][0] + Sm[1][1] + Sm[2][2];
+  // K[i][j] = delta_ij sum_t w_t . zm_t + sum_t (w_t[j] zm_t[i] + w_t[i] zl_t[j])
+  double tr = 0.0;
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) tr = fma(f.w[t][q], zm[t][q], tr);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int j = 0; j < 3; ++j) K[i][j] = (i == j ? tr : 0.0) + Sm[j][i] + Sl[i][j];
+    for (int j = 0; j < 3; ++j) {
+      double v = i == j ? tr : 0.0;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) v = fma(f.w[t][j], zm[t][i], fma(f.w[t][i], zl[t][j], v));
+      emit(i, j, v);
+    }
 }
 
 constexpr int kBT = 256;
 
 struct VP2Lds {
-  int vert, vals, info, rows, bacc, total;
+  int vert, vals, info, rows, base, bacc, total;
 };
 
 __host__ __device__ inline int al16(int n) { return (n + 15) & ~15; }
@@ -415,6 +419,7 @@ __host__ __device__ inline VP2Lds vp2_lds(int maxV, int maxO, int maxVals, bool 
   L.vals = o; o += al16(maxVals * 8);
   L.info = o; o += al16(maxO * 4);
   L.rows = o; o += al16(maxO * 3 * 4);
+  L.base = o; o += al16(maxO * 3 * 4);
   L.bacc = o; o += b ? al16(maxO * 3 * 8) : 0;
   L.total = o;
   return L;
@@ -429,6 +434,7 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
   double* svals = reinterpret_cast<double*>(lds + L.vals);
   uint32_t* sinfo = reinterpret_cast<uint32_t*>(lds + L.info);
   int32_t* srows = reinterpret_cast<int32_t*>(lds + L.rows);
+  int32_t* sbase = reinterpret_cast<int32_t*>(lds + L.base);
   double* sbacc = reinterpret_cast<double*>(lds + L.bacc);
   const int tid = threadIdx.x;
 
@@ -443,10 +449,12 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
     *reinterpret_cast<double2*>(rec + 4) = make_double2(need_c ? p.lmbda[kd] : 0.0, 0.0);
   }
   const int o0 = t.own_ptr[blk], no = t.own_ptr[blk + 1] - o0;
-  for (int i = tid; i < no; i += kBT) {
-    sinfo[i] = t.own_info[o0 + i];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) srows[3 * i + k] = t.own_rows[3 * (o0 + i) + k];
+  for (int i = tid; i < no; i += kBT) sinfo[i] = t.own_info[o0 + i];
+  // owned rows and their CSR starts, fetched now so that the write-out issues no loads
+  for (int i = tid; i < 3 * no; i += kBT) {
+    const int32_t row = t.own_rows[3 * o0 + i];
+    srows[i] = row;
+    if (WANT_A) sbase[i] = t.row_ptr[row];
   }
   if (WANT_A) {
     const int nvals = t.blk_vals[blk];
@@ -457,12 +465,25 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
     for (int i = tid; i < 3 * no; i += kBT) sbacc[i] = 0.0;
   __syncthreads();
 
-  // Each work item is split over two threads (node pairs b = 0..4 and b = 5..9): blocks hold
-  // ~130 items, so one item per thread would leave half of the workgroup idle.
+  // Work is laid out in wavefront-aligned segments so that every wavefront runs one code path:
+  // vertex-node items, one thread each for all ten node pairs b (one gradient factor); then
+  // edge-node items split over two threads, b = 0..4 (half 0) and b = 5..9 (half 1).
   const int t0 = t.task_ptr[blk], nt = t.task_ptr[blk + 1] - t0;
-  for (int kk = tid; kk < 2 * nt; kk += kBT) {
-    const int half = kk >= nt ? 1 : 0;  // wavefronts stay on one half
-    const int k = kk - half * nt;
+  const int nvx = t.task_nvert[blk], ned = nt - nvx;
+  const int pv = (nvx + 63) & ~63, pe = (ned + 63) & ~63;
+  for (int kk = tid; kk < pv + 2 * pe; kk += kBT) {
+    int k, half;  // half 2: a vertex item, all b
+    if (kk < pv) {
+      if (kk >= nvx) continue;
+      k = kk;
+      half = 2;
+    } else {
+      const int q = kk - pv;
+      half = q >= pe ? 1 : 0;
+      const int r = q - half * pe;
+      if (r >= ned) continue;
+      k = nvx + r;
+    }
     const uint64_t w0 = t.task[3 * (t0 + k)];
     const uint64_t w1 = t.task[3 * (t0 + k) + 1];
     const uint64_t w2 = t.task[3 * (t0 + k) + 2];
@@ -486,11 +507,7 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
     const uint32_t info = sinfo[o];
     const int off = static_cast<int>(info & 0xffffu);
     const int deg = static_cast<int>(info >> 16);
-    double be[3] = {0.0, 0.0, 0.0};
-    if (WANT_B && half == 0) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i) be[i] = p.f3[i] * (a < 4 ? -c.vol / 20.0 : c.vol / 5.0);
-    }
+    double be[3] = {0.0, 0.0, 0.0};  // Dirichlet lifting here; the body force is added below
     // Dirichlet data of the cell (rows of node a, columns of every node b)
     uint32_t bcrow = 0, bccol = 0;  // bit i: row (i,a); bit j*10+b: column (j,b)
     double growv[3] = {0, 0, 0};
@@ -509,41 +526,30 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
     if constexpr (WANT_A || BC) {
       auto run = [&](auto vconst) {
         constexpr bool VERT = decltype(vconst)::value;
-        NodeV fa;
-        node_v<VERT>(c, a, fa);
+        NodeW fa;
+        node_w<VERT>(c, a, fa);
         auto one = [&](auto bconst) {
           constexpr int b = decltype(bconst)::value;
-          double K[3][3];
-          node_block_lean<b, VERT>(c, fa, K);
-          if constexpr (BC) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-              for (int j = 0; j < 3; ++j) {
-                const bool cb = (bccol >> (j * 10 + b)) & 1u;
-                if (WANT_B && cb && !(bcrow >> i & 1u))
-                  be[i] -= K[i][j] * p.bc_value[cdofs[j * 10 + b]];
-                if (cb || (bcrow >> i & 1u)) K[i][j] = (i == j && a == b) ? 1.0 : 0.0;
-              }
-          }
-          if constexpr (WANT_A) {
-            const int rk = static_cast<int>(b < 5 ? (w1 >> (24 + 8 * b)) & 0xffu : (w2 >> (8 * (b - 5))) & 0xffu);
-            const int s2 = t.component_major ? deg : 1;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              double* row = svals + off + i * 3 * deg + (t.component_major ? rk : 3 * rk);
-#pragma unroll
-              for (int j = 0; j < 3; ++j) unsafeAtomicAdd(row + j * s2, K[i][j]);
+          const int rk = static_cast<int>(b < 5 ? (w1 >> (24 + 8 * b)) & 0xffu : (w2 >> (8 * (b - 5))) & 0xffu);
+          const int s2 = t.component_major ? deg : 1;
+          double* base = svals + off + (t.component_major ? rk : 3 * rk);
+          node_block_emit<b, VERT>(c, fa, [&](int i, int j, double v) {
+            if constexpr (BC) {
+              const bool cb = (bccol >> (j * 10 + b)) & 1u;
+              if (WANT_B && cb && !(bcrow >> i & 1u)) be[i] -= v * p.bc_value[cdofs[j * 10 + b]];
+              if (cb || (bcrow >> i & 1u)) v = (i == j && a == b) ? 1.0 : 0.0;
             }
-          }
+            if constexpr (WANT_A) unsafeAtomicAdd(base + i * 3 * deg + j * s2, v);
+          });
         };
-        if (half == 0) {
+        if (VERT || half == 0) {
           one(std::integral_constant<int, 0>{});
           one(std::integral_constant<int, 1>{});
           one(std::integral_constant<int, 2>{});
           one(std::integral_constant<int, 3>{});
           one(std::integral_constant<int, 4>{});
-        } else {
+        }
+        if (VERT || half == 1) {
           one(std::integral_constant<int, 5>{});
           one(std::integral_constant<int, 6>{});
           one(std::integral_constant<int, 7>{});
@@ -551,34 +557,45 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
           one(std::integral_constant<int, 9>{});
         }
       };
-      // work items are ordered vertex nodes first, so a wavefront mostly takes one branch
-      if (a < 4) run(std::integral_constant<bool, true>{});
+      // the segment decides the path: uniform over the wavefront
+      if (half == 2) run(std::integral_constant<bool, true>{});
       else run(std::integral_constant<bool, false>{});
     }
-    if constexpr (WANT_B) {
+    if (WANT_B && half != 1) {
+      // body force f3 . phi_a: -vol/20 at vertex nodes, vol/5 at edge nodes
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        // a Dirichlet row takes its value once (from the first half), never the lifting
-        const double v = (BC && (bcrow >> i & 1u)) ? (half == 0 ? growv[i] : 0.0) : be[i];
-        unsafeAtomicAdd(sbacc + 3 * o + i, v);
+      for (int i = 0; i < 3; ++i) be[i] += p.f3[i] * (a < 4 ? -c.vol / 20.0 : c.vol / 5.0);
+    }
+    if constexpr (WANT_B) {
+      // the load vector and a Dirichlet row's value come from the thread that is not the
+      // second half of an edge item (once per item); the second half adds only its lifting
+      if (half != 1 || BC) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const double v = (BC && (bcrow >> i & 1u)) ? (half != 1 ? growv[i] : 0.0) : be[i];
+          unsafeAtomicAdd(sbacc + 3 * o + i, v);
+        }
       }
     }
   }
   __syncthreads();
 
-  // write-out: each owned node's 3 rows are contiguous CSR ranges
+  // write-out, one owned node per wavefront: its three rows (3 deg values each) are contiguous
+  // in LDS and are three contiguous CSR ranges
   const int wave = tid >> 6, lane = tid & 63;
-  for (int i = wave; i < 3 * no; i += kBT / 64) {
-    const int o = i / 3, comp = i % 3;
+  for (int o = wave; o < no; o += kBT / 64) {
     const uint32_t info = sinfo[o];
-    const int off = static_cast<int>(info & 0xffffu) + comp * 3 * static_cast<int>(info >> 16);
-    const int len = 3 * static_cast<int>(info >> 16);
-    const int32_t row = srows[i];
+    const int off = static_cast<int>(info & 0xffffu);
+    const int len = 3 * static_cast<int>(info >> 16);  // one row
     if (WANT_A) {
-      const int64_t base = t.row_ptr[row];
-      for (int j = lane; j < len; j += 64) p.values[base + j] = svals[off + j];
+      const int64_t b0 = sbase[3 * o], b1 = sbase[3 * o + 1], b2 = sbase[3 * o + 2];
+      for (int j = lane; j < 3 * len; j += 64) {
+        const int comp = j >= len ? (j >= 2 * len ? 2 : 1) : 0;
+        const int64_t base = comp == 0 ? b0 : (comp == 1 ? b1 : b2);
+        p.values[base + (j - comp * len)] = svals[off + j];
+      }
     }
-    if (WANT_B && lane == 0) p.b[row] = sbacc[i];
+    if (WANT_B && lane < 3) p.b[srows[3 * o + lane]] = sbacc[3 * o + lane];
   }
 }
 

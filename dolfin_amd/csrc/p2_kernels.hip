@@ -143,8 +143,10 @@ hipError_t launch_t(const VP2Topo& t, const VP2Pass& p, hipStream_t s) {
 }
 
 
+// without Dirichlet rows: four waves per SIMD (at most 128 VGPRs)
 template <bool WANT_A, bool WANT_B, bool BC>
-__global__ __launch_bounds__(kBT) void vp2_blocked_kernel(VP2BlockTopo t, VP2Pass p, VP2Lds L) {
+__global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(BC ? 1 : 4)))
+void vp2_blocked_kernel(VP2BlockTopo t, VP2Pass p, VP2Lds L) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int nb = t.num_blocks;
   const int q = nb >> 3, r = nb & 7;

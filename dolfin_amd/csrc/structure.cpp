@@ -704,6 +704,7 @@ void build_vp2_block_plan(int64_t nc, const int32_t* cells, const int32_t* cd, c
   }
   P.task.resize(3 * ntask);
   P.task_cell.resize(ntask);
+  P.task_nvert.assign(nb, 0);
   P.blk_vals.assign(nb, 0);
   std::vector<std::vector<int32_t>> bverts(nb);
   std::vector<std::vector<uint32_t>> binfo(nb);
@@ -764,6 +765,7 @@ void build_vp2_block_plan(int64_t nc, const int32_t* cells, const int32_t* cd, c
           else w2 |= r << (8 * (bb - 5));
         }
         const int64_t k = t0 + static_cast<int64_t>(q);
+        if (a < 4) P.task_nvert[b]++;
         P.task[3 * k] = w0;
         P.task[3 * k + 1] = w1;
         P.task[3 * k + 2] = w2;

@@ -166,6 +166,8 @@ extern "C" int plan_check_vp2(int64_t nc, const int32_t* cells, const int32_t* c
         const uint64_t w0 = P.task[3 * k], w1 = P.task[3 * k + 1], w2 = P.task[3 * k + 2];
         const int o = static_cast<int>(w1 & 0xffffu), a = static_cast<int>((w1 >> 16) & 0xffu);
         if (o >= no || a >= 10) return -5;
+        // the kernel's segments: the block's first task_nvert items are vertex nodes, the rest edges
+        if ((k - P.task_ptr[b] < P.task_nvert[b]) != (a < 4)) return -15;
         const int64_t c = P.task_cell[k];
         seen[c * 10 + a]++;
         double xc[12], m[4], l[4];
