@@ -157,93 +157,13 @@ __device__ __forceinline__ void vp2_block(const C& c, int a, int b, double (&K)[
 
 namespace p2dev {
 
-// Register-only evaluation of one node's row blocks (no runtime-indexed register arrays).
 // grad phi_a = sum_t (u_t . lambda) w_t with t = 0, 1:
 //   vertex a:     u_0 = 4 e_a - 1, w_0 = grad lambda_a, w_1 = 0;
 //   edge (k, l):  u_0 = 4 e_l, w_0 = grad lambda_k;  u_1 = 4 e_k, w_1 = grad lambda_l.
-// v_t = u_t^T W (4-vectors, per coefficient field) are formed once per work item; for a
-// compile-time node b the scalars alpha_ts = v_t . u'_s need only constant indices.
-// ---------------------------------------------------------------------------------------
-struct NodeFactors {
-  double w[2][3];
-  double vm[2][4], vl[2][4];
-  double sm[2], sl[2];
-};
-
+// For a compile-time node b the scalars alpha_ts = v_t . u'_s (v_t = u_t^T W) need only
+// constant indices, so nothing below indexes a register array at run time.
 __device__ __forceinline__ double sel4(int i, double a0, double a1, double a2, double a3) {
   return i == 0 ? a0 : (i == 1 ? a1 : (i == 2 ? a2 : a3));
-}
-
-template <class C>
-__device__ __forceinline__ void node_factors(const C& c, int a, NodeFactors& f) {
-  // edge e = a - 4 joins (k, l): k = 2,1,1,0,0,0 and l = 3,3,2,3,2,1 packed two bits each
-  const int e = a - 4;
-  const bool vert = a < 4;
-  const int k = vert ? a : (22 >> (2 * e)) & 3;
-  const int l = vert ? 0 : (1775 >> (2 * e)) & 3;
-  const int g0 = k, g1 = l;            // gradient directions
-  const int i0 = vert ? a : l, i1 = k; // W rows
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    f.w[0][q] = sel4(g0, c.D[0][q], c.D[1][q], c.D[2][q], c.D[3][q]);
-    f.w[1][q] = vert ? 0.0 : sel4(g1, c.D[0][q], c.D[1][q], c.D[2][q], c.D[3][q]);
-  }
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const double wm0 = sel4(i0, c.Wm[0][m], c.Wm[1][m], c.Wm[2][m], c.Wm[3][m]);
-    const double wl0 = sel4(i0, c.Wl[0][m], c.Wl[1][m], c.Wl[2][m], c.Wl[3][m]);
-    const double wm1 = sel4(i1, c.Wm[0][m], c.Wm[1][m], c.Wm[2][m], c.Wm[3][m]);
-    const double wl1 = sel4(i1, c.Wl[0][m], c.Wl[1][m], c.Wl[2][m], c.Wl[3][m]);
-    f.vm[0][m] = 4.0 * wm0 - (vert ? c.Rm[m] : 0.0);
-    f.vl[0][m] = 4.0 * wl0 - (vert ? c.Rl[m] : 0.0);
-    f.vm[1][m] = 4.0 * wm1;
-    f.vl[1][m] = 4.0 * wl1;
-  }
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    f.sm[t] = f.vm[t][0] + f.vm[t][1] + f.vm[t][2] + f.vm[t][3];
-    f.sl[t] = f.vl[t][0] + f.vl[t][1] + f.vl[t][2] + f.vl[t][3];
-  }
-}
-
-// K[(i,a),(j,B)] for a compile-time node B.
-template <int B, class C>
-__device__ __forceinline__ void node_block(const C& c, const NodeFactors& f, double (&K)[3][3]) {
-  constexpr bool VB = B < 4;
-  constexpr int EK = VB ? B : (22 >> (2 * (B - 4))) & 3;
-  constexpr int EL = VB ? 0 : (1775 >> (2 * (B - 4))) & 3;
-  // factors of B: s = 0 (idx, grad) = VB ? (B, B) : (EL, EK); s = 1 (edge only): (EK, EL)
-  constexpr int NS = VB ? 1 : 2;
-  constexpr int IDX0 = VB ? B : EL, GRD0 = VB ? B : EK;
-  constexpr int IDX1 = EK, GRD1 = EL;
-  double Sm[3][3], Sl[3][3];
-#pragma unroll
-  for (int p = 0; p < 3; ++p)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) Sm[p][q] = Sl[p][q] = 0.0;
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int idx = s == 0 ? IDX0 : IDX1;
-      const int grd = s == 0 ? GRD0 : GRD1;
-      const double am = VB ? 4.0 * f.vm[t][idx] - f.sm[t] : 4.0 * f.vm[t][idx];
-      const double al = VB ? 4.0 * f.vl[t][idx] - f.sl[t] : 4.0 * f.vl[t][idx];
-#pragma unroll
-      for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const double o = f.w[t][p] * c.D[grd][q];
-          Sm[p][q] = fma(am, o, Sm[p][q]);
-          Sl[p][q] = fma(al, o, Sl[p][q]);
-        }
-    }
-  }
-  const double tr = Sm[0][0] + Sm[1][1] + Sm[2][2];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) K[i][j] = (i == j ? tr : 0.0) + Sm[j][i] + Sl[i][j];
 }
 
 // ---------------------------------------------------------------------------------------
