@@ -23,9 +23,10 @@ namespace dfa {
 namespace {
 
 constexpr int kBatchThreads = 256;
+constexpr int kBatchChainsPerThread = 2;  // edge-chain units: blocks of up to 512 chains
 
 struct BatchItem {
-  int32_t kind;  // 1: P1 kappa-Poisson on tetrahedra, 2: vector-P2 elasticity
+  int32_t kind;  // 1: P1 kappa-Poisson on tetrahedra, 2: vector-P2 elasticity, 3: as 1 by edge chains
   P1BlockTopo p1;
   P1Pass p1p;
   p1dev::LdsLayout p1L;
@@ -46,7 +47,10 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(const BatchItem* _
   for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
     const BatchUnit un = units[u];
     const BatchItem& it = items[un.item];
-    if (it.kind == 1)
+    if (it.kind == 3)
+      p1dev::p1_chain_body<false, kBatchThreads, kBatchChainsPerThread>(it.p1, it.p1p, it.p1L, lds,
+                                                                        un.block);
+    else if (it.kind == 1)
       p1dev::p1_block_body<3, DFA_FORM_P1_POISSON, false, false, true, kBatchThreads, true>(
           it.p1, it.p1p, it.p1L, lds, un.block);
     else
@@ -114,8 +118,9 @@ int dfa_batch_create(dfa_batch* out, int32_t n, const dfa_assembler* hs, const i
           return bfail("batch member " + std::to_string(i) +
                            ": P1 members must be tetrahedral meshes with a blocked plan",
                        DFA_ERR_ARG);
-        it.kind = 1;
         it.p1 = h->btopo;
+        it.kind = (it.p1.chain_ptr && it.p1.chain_rec &&
+                   it.p1.max_chains <= kBatchThreads * kBatchChainsPerThread) ? 3 : 1;
         it.p1p.gdim = 3;
         it.p1p.a_form = DFA_FORM_P1_POISSON;
         it.p1p.want_A = true;
