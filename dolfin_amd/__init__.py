@@ -9,8 +9,10 @@ from . import _lib
 from .mesh import Mesh, UnitCubeMesh, UnitIntervalMesh, UnitSquareMesh, jitter_interior
 from .function import Function, FunctionSpace, VectorFunctionSpace
 from .la import Matrix, Vector
-from .fem import (Assembler, DirichletBC, Form, SystemAssembler, assemble, assemble_system,
-                  body_force_form, elasticity_form, engine_for, load_form, mass_form, poisson_form)
+from .fem import (Assembler, DirichletBC, FacetForm, Form, FormSum, SystemAssembler, assemble,
+                  assemble_system, body_force_form, elasticity_form, engine_for, exterior_facets,
+                  facet_load_form, facet_mass_form, load_form, mark_exterior_facets, mass_form,
+                  poisson_form)
 
 SCATTER_AUTO = _lib.SCATTER_AUTO
 SCATTER_ATOMIC = _lib.SCATTER_ATOMIC
@@ -21,5 +23,7 @@ __all__ = [
     "Function", "FunctionSpace", "VectorFunctionSpace", "Matrix", "Vector",
     "Assembler", "DirichletBC", "Form", "SystemAssembler", "assemble", "assemble_system",
     "body_force_form", "elasticity_form", "engine_for", "load_form", "mass_form", "poisson_form",
+    "FacetForm", "FormSum", "exterior_facets", "facet_load_form", "facet_mass_form",
+    "mark_exterior_facets",
     "SCATTER_AUTO", "SCATTER_ATOMIC", "SCATTER_BLOCKED",
 ]

@@ -21,6 +21,8 @@ FORM_P1_LOAD = 1
 FORM_P1_MASS = 2
 FORM_VP2_ELASTICITY = 3
 FORM_VP2_BODY_FORCE = 4
+FORM_P1_FACET_MASS = 7
+FORM_P1_FACET_LOAD = 8
 
 SCATTER_AUTO = 0
 SCATTER_ATOMIC = 1
@@ -31,7 +33,8 @@ ADD_VALUES = 1
 EXPORTS = ["dfa_last_error", "dfa_version", "dfa_assembler_create", "dfa_assembler_destroy",
            "dfa_csr_dims", "dfa_csr_structure", "dfa_assemble_matrix", "dfa_assemble_vector",
            "dfa_assemble_system", "dfa_adjoint", "dfa_plan_info", "dfa_set_plan_defaults",
-           "dfa_batch_create", "dfa_batch_launch", "dfa_batch_info", "dfa_batch_destroy"]
+           "dfa_batch_create", "dfa_batch_launch", "dfa_batch_info", "dfa_batch_destroy",
+           "dfa_exterior_facets", "dfa_assemble_facets"]
 
 _lib = None
 
@@ -70,6 +73,9 @@ def load():
     L.dfa_batch_info.argtypes = [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i32), ctypes.POINTER(_i32),
                                  ctypes.POINTER(_i64)]
     L.dfa_batch_destroy.argtypes = [_vp]
+    L.dfa_exterior_facets.argtypes = [_vp, ctypes.POINTER(_i64), _vp, _vp]
+    L.dfa_assemble_facets.argtypes = [_vp, _i32, _i32, _vp, ctypes.c_double, _vp, _vp, _vp, _vp,
+                                      _vp, _vp, _vp]
     for name in EXPORTS:
         if name not in ("dfa_last_error", "dfa_version"):
             getattr(L, name).restype = ctypes.c_int

@@ -494,6 +494,79 @@ int dfa_adjoint(dfa_assembler h, int32_t a_form, const double* d_x, const double
   });
 }
 
+namespace {
+void ensure_facets(dfa_assembler h, hipStream_t s) {
+  if (h->has_facets) return;
+  if (h->element != DFA_ELEMENT_P1) throw std::runtime_error("exterior-facet forms need a P1 space");
+  std::vector<int32_t> cells(static_cast<size_t>(h->nc) * h->nvc);
+  if (!cells.empty()) {
+    hip_check(hipMemcpyAsync(cells.data(), h->d_cells, cells.size() * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  }
+  dfa::build_exterior_facets(h->gdim, h->nc, cells.data(), h->facet_cell, h->facet_local);
+  h->d_facet_cell = h->facets.upload(h->facet_cell, s);
+  h->d_facet_local = h->facets.upload(h->facet_local, s);
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  h->has_facets = true;
+}
+}  // namespace
+
+int dfa_exterior_facets(dfa_assembler h, int64_t* num_facets, int32_t* facet_cell,
+                        int32_t* facet_local) {
+  return guarded([&]() -> int {
+    if (!h || !num_facets) return fail("NULL argument", DFA_ERR_ARG);
+    ensure_facets(h, nullptr);
+    *num_facets = static_cast<int64_t>(h->facet_cell.size());
+    if (facet_cell) std::copy(h->facet_cell.begin(), h->facet_cell.end(), facet_cell);
+    if (facet_local)
+      for (size_t i = 0; i < h->facet_local.size(); ++i) facet_local[i] = h->facet_local[i];
+    return DFA_OK;
+  });
+}
+
+int dfa_assemble_facets(dfa_assembler h, int32_t a_form, int32_t L_form, const double* d_x,
+                        double alpha, const double* d_g, const uint8_t* d_facet_mask,
+                        const uint8_t* d_bc_mask, const double* d_bc_value, double* d_values,
+                        double* d_b, void* stream) {
+  return guarded([&]() -> int {
+    if (!h || !d_x) return fail("NULL handle or coordinates", DFA_ERR_ARG);
+    if (a_form != -1 && a_form != DFA_FORM_P1_FACET_MASS)
+      return fail("facet bilinear form must be DFA_FORM_P1_FACET_MASS or -1", DFA_ERR_ARG);
+    if (L_form != -1 && L_form != DFA_FORM_P1_FACET_LOAD)
+      return fail("facet linear form must be DFA_FORM_P1_FACET_LOAD or -1", DFA_ERR_ARG);
+    if (L_form >= 0 && !d_g) return fail("g is NULL", DFA_ERR_ARG);
+    if ((d_bc_mask == nullptr) != (d_bc_value == nullptr))
+      return fail("Dirichlet mask and values go together", DFA_ERR_ARG);
+    if (!d_values && !d_b) return fail("nothing to assemble (A and b are NULL)", DFA_ERR_ARG);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ensure_facets(h, s);
+    dfa::FacetTopo t;
+    t.num_facets = static_cast<int64_t>(h->facet_cell.size());
+    t.facet_cell = h->d_facet_cell;
+    t.facet_local = h->d_facet_local;
+    t.cells = h->d_cells;
+    t.cell_dofs = h->d_cdofs;
+    t.coeff_dofs = h->d_kdofs;
+    t.row_ptr = h->d_row_ptr;
+    t.col = h->d_col;
+    dfa::FacetPass p;
+    p.gdim = h->gdim;
+    p.has_a = a_form >= 0;
+    p.has_b = L_form >= 0;
+    p.alpha = alpha;
+    p.x = d_x;
+    p.g = d_g;
+    p.mask = d_facet_mask;
+    p.bc_mask = d_bc_mask;
+    p.bc_value = d_bc_value;
+    p.values = p.has_a ? d_values : nullptr;
+    p.b = d_b;
+    hip_check(dfa::launch_facets(t, p, s), "facet_kernel");
+    return DFA_OK;
+  });
+}
+
 int dfa_plan_info(dfa_assembler h, int32_t scatter, int64_t* plan_bytes, int64_t* num_blocks,
                   int64_t* cell_appearances, int32_t* lds_bytes) {
   return guarded([&]() -> int {

@@ -71,4 +71,11 @@ struct dfa_assembler_s {
   bool vp2_component_major = true;
   bool has_vp2_plan = false;
   dfa::VP2BlockTopo vtopo;
+  // exterior facets (P1), built on first use
+  bool has_facets = false;
+  std::vector<int32_t> facet_cell;
+  std::vector<int8_t> facet_local;
+  int32_t* d_facet_cell = nullptr;
+  int8_t* d_facet_local = nullptr;
+  dfa::DeviceArena facets;
 };

@@ -70,6 +70,12 @@ struct P1ChainPlan {
 };
 void build_p1_chain_plan(const P1BlockPlan& plan, P1ChainPlan& chains);
 
+// Exterior facets (facets of exactly one cell), sorted by cell then local facet index; local
+// facet k is the facet opposite local vertex k. Facet keys are grouped by their smallest
+// vertex (a counting sort), so the work is linear in the number of cell facets.
+void build_exterior_facets(int32_t gdim, int64_t num_cells, const int32_t* cells,
+                           std::vector<int32_t>& facet_cell, std::vector<int8_t>& facet_local);
+
 // Throws std::runtime_error on inconsistent input or exceeded limits.
 void build_p1_block_plan(int32_t gdim, int64_t num_vertices, int64_t num_cells,
                          const int32_t* cells, const int32_t* cell_dofs,

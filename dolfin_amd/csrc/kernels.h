@@ -66,6 +66,33 @@ hipError_t launch_p1_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t 
 size_t p1_blocked_lds_bytes(const P1BlockTopo& t, const P1Pass& p);
 hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);
 
+// ---- P1 exterior facets -----------------------------------------------------------------
+struct FacetTopo {
+  int64_t num_facets = 0;
+  const int32_t* facet_cell = nullptr;
+  const int8_t* facet_local = nullptr;  // facet opposite this local vertex
+  const int32_t* cells = nullptr;
+  const int32_t* cell_dofs = nullptr;
+  const int32_t* coeff_dofs = nullptr;
+  const int32_t* row_ptr = nullptr;
+  const int32_t* col = nullptr;
+};
+
+struct FacetPass {
+  int gdim = 3;
+  bool has_a = false;              // alpha*u*v*ds is part of the bilinear form
+  bool has_b = false;              // g*v*ds is part of the linear form
+  double alpha = 0.0;
+  const double* x = nullptr;
+  const double* g = nullptr;
+  const uint8_t* mask = nullptr;   // per exterior facet, NULL = all
+  const uint8_t* bc_mask = nullptr;
+  const double* bc_value = nullptr;
+  double* values = nullptr;        // NULL: matrix not written (its Dirichlet lifting still is)
+  double* b = nullptr;
+};
+hipError_t launch_facets(const FacetTopo& t, const FacetPass& p, hipStream_t s);
+
 // ---- vector P2 elasticity --------------------------------------------------------------
 struct VP2Topo {
   int64_t num_cells = 0;

@@ -392,6 +392,72 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   });
 }
 
+void build_exterior_facets(int32_t gdim, int64_t nc, const int32_t* cells,
+                           std::vector<int32_t>& facet_cell, std::vector<int8_t>& facet_local) {
+  const int nv = gdim + 1;
+  facet_cell.clear();
+  facet_local.clear();
+  if (nc == 0) return;
+  int32_t nvert = 0;
+  for (int64_t i = 0; i < nc * nv; ++i) nvert = std::max(nvert, cells[i] + 1);
+  // bucket every cell facet by its smallest vertex
+  auto facet_key = [&](int64_t c, int f, int32_t (&k)[3]) {
+    int n = 0;
+    for (int a = 0; a < nv; ++a)
+      if (a != f) k[n++] = cells[c * nv + a];
+    for (int i = n; i < 3; ++i) k[i] = -1;
+    std::sort(k, k + n);
+  };
+  std::vector<int64_t> start(static_cast<size_t>(nvert) + 1, 0);
+  for (int64_t c = 0; c < nc; ++c)
+    for (int f = 0; f < nv; ++f) {
+      int32_t k[3];
+      facet_key(c, f, k);
+      start[k[0] + 1]++;
+    }
+  for (int32_t v = 0; v < nvert; ++v) start[v + 1] += start[v];
+  std::vector<int64_t> item(static_cast<size_t>(nc) * nv);  // c * nv + f
+  {
+    std::vector<int64_t> pos(start.begin(), start.end() - 1);
+    for (int64_t c = 0; c < nc; ++c)
+      for (int f = 0; f < nv; ++f) {
+        int32_t k[3];
+        facet_key(c, f, k);
+        item[pos[k[0]]++] = c * nv + f;
+      }
+  }
+  // within a bucket, facets with equal keys are adjacent after sorting by the other vertices
+  std::vector<char> exterior(item.size(), 0);
+  parallel_for(nvert, [&](int64_t v0, int64_t v1, int) {
+    std::vector<std::pair<std::pair<int32_t, int32_t>, int64_t>> bucket;
+    for (int64_t v = v0; v < v1; ++v) {
+      bucket.clear();
+      for (int64_t i = start[v]; i < start[v + 1]; ++i) {
+        int32_t k[3];
+        facet_key(item[i] / nv, static_cast<int>(item[i] % nv), k);
+        bucket.push_back({{k[1], k[2]}, i});
+      }
+      std::sort(bucket.begin(), bucket.end());
+      for (size_t i = 0; i < bucket.size();) {
+        size_t j = i + 1;
+        while (j < bucket.size() && bucket[j].first == bucket[i].first) ++j;
+        if (j - i == 1) exterior[bucket[i].second] = 1;
+        i = j;
+      }
+    }
+  });
+  std::vector<int64_t> ext;
+  for (size_t i = 0; i < item.size(); ++i)
+    if (exterior[i]) ext.push_back(item[i]);
+  std::sort(ext.begin(), ext.end());
+  facet_cell.resize(ext.size());
+  facet_local.resize(ext.size());
+  for (size_t i = 0; i < ext.size(); ++i) {
+    facet_cell[i] = static_cast<int32_t>(ext[i] / nv);
+    facet_local[i] = static_cast<int8_t>(ext[i] % nv);
+  }
+}
+
 // Chains are grown greedily: an unassigned cell seeds one chain per edge, walking across the
 // faces that contain the edge in both directions through unassigned cells of the block, and
 // the longest (capped at kChainLen) is kept. Seeds follow the block's cell order.

@@ -149,6 +149,137 @@ def body_force_form(V, f3):
     return Form(_lib.FORM_VP2_BODY_FORCE, 1, V, (), f3=f3)
 
 
+# ---- exterior-facet forms -----------------------------------------------------------------
+def exterior_facets(V):
+    """(facet_cell, facet_local) of the exterior facets of V's mesh in the library's order
+    (sorted by cell, then by local facet = the facet opposite that local vertex)."""
+    eng = engine_for(V)
+    lib = _lib.load()
+    n = ctypes.c_int64()
+    _lib.check(lib.dfa_exterior_facets(eng.handle, ctypes.byref(n), None, None))
+    fc = np.empty(n.value, np.int32)
+    fl = np.empty(n.value, np.int32)
+    _lib.check(lib.dfa_exterior_facets(eng.handle, ctypes.byref(n), _lib.ptr(fc), _lib.ptr(fl)))
+    return fc, fl
+
+
+def mark_exterior_facets(V, inside, check_midpoint=True):
+    """SubDomain::mark over the exterior facets (dolfin/mesh/SubDomain.cpp:182-282): a facet is
+    marked when inside(x) holds at every vertex and, with check_midpoint, at its midpoint.
+    inside takes an array of points [n, gdim] and returns booleans [n]. Returns a uint8 mask
+    in exterior_facets(V) order."""
+    mesh = V.mesh()
+    fc, fl = exterior_facets(V)
+    cells = mesh.cells()
+    x = mesh.coordinates()
+    nvc = cells.shape[1]
+    cols = np.array([[a for a in range(nvc) if a != f] for f in range(nvc)])
+    fv = cells[fc[:, None], cols[fl]]               # [nf, gdim] facet vertices
+    pts = x[fv]                                       # [nf, gdim, gdim]
+    ok = np.ones(fc.shape[0], dtype=bool)
+    for j in range(fv.shape[1]):
+        ok &= np.asarray(inside(pts[:, j, :]), dtype=bool)
+    if check_midpoint:
+        ok &= np.asarray(inside(pts.mean(axis=1)), dtype=bool)
+    return ok.astype(np.uint8)
+
+
+class FacetForm:
+    """A P1 exterior-facet form: alpha*u*v*ds (rank 2) or g*v*ds (rank 1), optionally over
+    the marked facets only (an exterior-facet subdomain)."""
+
+    def __init__(self, kind, rank, V, alpha=0.0, g=None, marker=None):
+        _p1(V)
+        self.kind, self.rank, self.V = kind, rank, V
+        self.alpha = float(alpha)
+        self.g = g
+        if g is not None and (not isinstance(g, Function) or g.function_space().element != _lib.ELEMENT_P1
+                              or g.function_space().mesh() is not V.mesh()):
+            raise ValueError("g must be a P1 Function on the same mesh")
+        self._marker = marker
+        self._mask = None
+
+    def facet_mask(self):
+        """Device uint8 mask over the exterior facets, or None for all of them."""
+        if self._marker is None:
+            return None
+        if self._mask is None:
+            m = self._marker
+            if callable(m):
+                m = mark_exterior_facets(self.V, m)
+            m = np.ascontiguousarray(np.asarray(m, dtype=np.uint8))
+            self._mask = torch.from_numpy(m).cuda()
+        return self._mask
+
+    def __add__(self, other):
+        return FormSum([self]) + other
+
+
+class FormSum:
+    """Sum of cell and exterior-facet forms of one rank on one space (a ufl Form with dx and
+    ds integrals)."""
+
+    def __init__(self, parts):
+        self.parts = list(parts)
+        self.rank = self.parts[0].rank
+        self.V = self.parts[0].V
+        for f in self.parts:
+            if f.rank != self.rank or f.V is not self.V:
+                raise ValueError("summed forms need the same rank and function space")
+
+    def __add__(self, other):
+        return FormSum(self.parts + (other.parts if isinstance(other, FormSum) else [other]))
+
+    def cell_parts(self):
+        return [f for f in self.parts if isinstance(f, Form)]
+
+    def facet_parts(self):
+        return [f for f in self.parts if isinstance(f, FacetForm)]
+
+
+Form.__add__ = lambda self, other: FormSum([self]) + other
+
+
+def facet_mass_form(V, alpha=1.0, marker=None):
+    """alpha*u*v*ds (alpha constant), over the exterior facets selected by marker."""
+    return FacetForm(_lib.FORM_P1_FACET_MASS, 2, V, alpha=alpha, marker=marker)
+
+
+def facet_load_form(V, g, marker=None):
+    """g*v*ds (g a P1 Function), over the exterior facets selected by marker."""
+    return FacetForm(_lib.FORM_P1_FACET_LOAD, 1, V, g=g, marker=marker)
+
+
+def _split(form):
+    if isinstance(form, FormSum):
+        return form.cell_parts(), form.facet_parts()
+    if isinstance(form, FacetForm):
+        return [], [form]
+    return [form], []
+
+
+def _assemble_facets(eng, x, a_facets, L_facets, A_values, b_values, mask=None, vals=None):
+    """Adds exterior-facet terms; a facet matrix and vector with different markers run as
+    separate passes."""
+    lib = _lib.load()
+    groups = []
+    for fa in a_facets:
+        groups.append((fa, None))
+    for fl in L_facets:
+        groups.append((None, fl))
+    # one pass when the matrix and vector terms share their facet selection
+    if len(a_facets) == 1 and len(L_facets) == 1 and a_facets[0]._marker is L_facets[0]._marker:
+        groups = [(a_facets[0], L_facets[0])]
+    for fa, fl in groups:
+        fmask = (fa or fl).facet_mask()
+        _lib.check(lib.dfa_assemble_facets(
+            eng.handle, fa.kind if fa is not None else -1, fl.kind if fl is not None else -1,
+            _lib.ptr(x), fa.alpha if fa is not None else 0.0,
+            _lib.ptr(fl.g.vector() if fl is not None else None), _lib.ptr(fmask), _lib.ptr(mask),
+            _lib.ptr(vals), _lib.ptr(A_values if fa is not None else None),
+            _lib.ptr(b_values), _lib.stream()))
+
+
 # ---- boundary conditions ------------------------------------------------------------------
 def boundary_facet_nodes(V):
     """Dofs on boundary facets (DirichletBC 'on_boundary', topological method): facets of
@@ -237,6 +368,35 @@ class Assembler:
         self.scatter = scatter
 
     def assemble(self, A, a):
+        cells, facets = _split(a)
+        if facets:
+            # cell integrals first (zeroing unless add_values), then the exterior facets
+            # (Assembler.cpp:52-103 order)
+            if len(cells) > 1:
+                raise ValueError("at most one cell-integral form per sum")
+            eng = engine_for(a.V, self.keep_diagonal)
+            fin, self.finalize_tensor = self.finalize_tensor, False
+            try:
+                if cells:
+                    self.assemble(A, cells[0])
+                else:
+                    if a.rank == 2 and (A.empty() or A.engine() is not eng):
+                        A.init(eng)
+                    if a.rank == 1 and (A.empty() or A.size() != a.V.dim()):
+                        A.init(a.V.dim())
+                    if not self.add_values:
+                        A.zero()
+            finally:
+                self.finalize_tensor = fin
+            x = a.V.mesh().coordinates_device()
+            if a.rank == 2:
+                _assemble_facets(eng, x, facets, [], A.values, None)
+            else:
+                _assemble_facets(eng, x, [], facets, None, A.values)
+            if self.finalize_tensor:
+                A.apply("add")
+            return A
+        a = cells[0]
         L = _lib.load()
         eng = engine_for(a.V, self.keep_diagonal)
         x = a.V.mesh().coordinates_device()
@@ -273,7 +433,20 @@ class SystemAssembler:
             raise ValueError("SystemAssembler needs a bilinear and a linear form")
         if a.V is not L.V:
             raise ValueError("a and L must share the function space")
-        self.a, self.L = a, L
+        a_cells, self.a_facets = _split(a)
+        L_cells, self.L_facets = _split(L)
+        if len(a_cells) != 1 or len(L_cells) > 1:
+            raise ValueError("SystemAssembler needs one cell-integral bilinear form and at most "
+                             "one cell-integral linear form (plus exterior-facet forms)")
+        if L_cells:
+            L = L_cells[0]
+        else:
+            # a linear form of facet terms only: its cell part is f*v*dx with f = 0
+            if a.V.element != _lib.ELEMENT_P1:
+                raise ValueError("exterior-facet forms need a P1 space")
+            zero = Function(a.V)
+            L = load_form(a.V, zero)
+        self.a, self.L = a_cells[0], L
         self.bcs = [] if bcs is None else (list(bcs) if isinstance(bcs, (list, tuple)) else [bcs])
         self.scatter = scatter
         self.keep_diagonal = False
@@ -298,6 +471,15 @@ class SystemAssembler:
             _lib.ptr(lc[0] if lc else None), _lib.ptr(self.L.f3), _lib.ptr(mask), _lib.ptr(vals),
             _lib.ptr(None if A is None else A.values), _lib.ptr(None if b is None else b.values),
             self.scatter, 0, _lib.stream()))
+        if self.a_facets or self.L_facets:
+            # facet tensors belong to their cell's tensor before apply_bc
+            # (SystemAssembler.cpp:421-481): Dirichlet rows keep the cell pass's values and
+            # Dirichlet columns are lifted into b
+            if len(self.a_facets) > 1 or len(self.L_facets) > 1:
+                raise ValueError("at most one exterior-facet form each in a and L")
+            _assemble_facets(eng, x, self.a_facets, self.L_facets,
+                             None if A is None else A.values, None if b is None else b.values,
+                             mask=mask, vals=vals)
         return A, b
 
 

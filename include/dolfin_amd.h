@@ -47,6 +47,8 @@ extern "C" {
 #define DFA_FORM_P1_MASS 2          /* u*v*dx                                             */
 #define DFA_FORM_VP2_ELASTICITY 3   /* inner(2 mu eps(u) + lmbda tr(eps(u)) I, eps(v))*dx */
 #define DFA_FORM_VP2_BODY_FORCE 4   /* inner(f, v)*dx, f a constant 3-vector              */
+#define DFA_FORM_P1_FACET_MASS 7    /* alpha*u*v*ds over exterior facets, alpha constant   */
+#define DFA_FORM_P1_FACET_LOAD 8    /* g*v*ds over exterior facets, g P1                   */
 
 /* Scatter strategies (chosen per call, or by dfa_autotune). */
 #define DFA_SCATTER_AUTO 0
@@ -168,6 +170,32 @@ int dfa_batch_destroy(dfa_batch b);
    read once) and the bytes the plan of the chosen strategy actually stores. */
 int dfa_plan_info(dfa_assembler h, int32_t scatter, int64_t* plan_bytes, int64_t* num_blocks,
                   int64_t* cell_appearances, int32_t* lds_bytes);
+
+/*
+ * Exterior facets of a P1 handle (Facet::exterior: one incident cell), sorted by cell and
+ * local facet index (local facet k lies opposite local vertex k). Writes their number and,
+ * when the arrays are non-NULL, each facet's cell and local index -- the order d_facet_mask
+ * below uses (the counterpart of a MeshFunction over the exterior facets). Built on first
+ * use and kept in the handle.
+ */
+int dfa_exterior_facets(dfa_assembler h, int64_t* num_facets, int32_t* facet_cell,
+                        int32_t* facet_local);
+
+/*
+ * Exterior-facet integrals of P1 forms, ADDED to d_values and d_b: replaces
+ * Assembler::assemble_exterior_facets (dolfin/fem/Assembler.cpp:196-298) and the facet terms
+ * SystemAssembler merges into each cell tensor (dolfin/fem/SystemAssembler.cpp:421-481).
+ * a_form: DFA_FORM_P1_FACET_MASS (alpha*u*v*ds) or -1; L_form: DFA_FORM_P1_FACET_LOAD
+ * (g*v*ds, d_g on the coefficient dofs) or -1. d_facet_mask (NULL = every exterior facet)
+ * selects facets as an exterior-facet subdomain does. With d_bc_mask / d_bc_value the pass
+ * completes dfa_assemble_system with the same Dirichlet data: Dirichlet rows are left as the
+ * cell pass wrote them and Dirichlet columns are lifted into d_b. d_values may be NULL (the
+ * lifting of the facet matrix into d_b is still applied); d_b may be NULL.
+ */
+int dfa_assemble_facets(dfa_assembler h, int32_t a_form, int32_t L_form, const double* d_x,
+                        double alpha, const double* d_g, const uint8_t* d_facet_mask,
+                        const uint8_t* d_bc_mask, const double* d_bc_value, double* d_values,
+                        double* d_b, void* stream);
 
 #ifdef __cplusplus
 }

@@ -535,6 +535,176 @@ int32_t orc_assemble_system(int32_t a_kind, int32_t L_kind, int32_t gdim, int64_
 // -- the discrete form of the shape derivative checked in
 // python/test/unit/fem/test_coordinate_derivative.py:47-49.
 // ---------------------------------------------------------------------------------------
+// ---- exterior facets ------------------------------------------------------------------
+// Facet::exterior (a facet with one incident cell), computed by sorting the vertex keys of
+// all cell facets (the role of TopologyComputation's key matching, TopologyComputation.cpp
+// :152-330, here only to count incidences).
+int64_t orc_exterior_facets(int32_t gdim, int64_t num_cells, const int32_t* cells,
+                            int32_t* facet_cell, int32_t* facet_local) {
+  const int nv = gdim + 1;
+  std::vector<std::tuple<std::array<int32_t, 3>, int64_t, int>> keys;
+  keys.reserve(static_cast<size_t>(num_cells) * nv);
+  for (int64_t c = 0; c < num_cells; ++c)
+    for (int f = 0; f < nv; ++f) {
+      std::array<int32_t, 3> k = {-1, -1, -1};
+      int n = 0;
+      for (int a = 0; a < nv; ++a)
+        if (a != f) k[n++] = cells[c * nv + a];
+      std::sort(k.begin(), k.begin() + n);
+      keys.emplace_back(k, c, f);
+    }
+  std::sort(keys.begin(), keys.end());
+  std::vector<std::pair<int64_t, int>> ext;
+  for (size_t i = 0; i < keys.size();) {
+    size_t j = i + 1;
+    while (j < keys.size() && std::get<0>(keys[j]) == std::get<0>(keys[i])) ++j;
+    if (j - i == 1) ext.emplace_back(std::get<1>(keys[i]), std::get<2>(keys[i]));
+    i = j;
+  }
+  std::sort(ext.begin(), ext.end());
+  if (facet_cell)
+    for (size_t i = 0; i < ext.size(); ++i) {
+      facet_cell[i] = static_cast<int32_t>(ext[i].first);
+      facet_local[i] = ext[i].second;
+    }
+  return static_cast<int64_t>(ext.size());
+}
+
+namespace {
+// measure of the facet opposite local vertex lf
+double facet_measure(int gdim, const double* xc, int lf) {
+  int v[3], n = 0;
+  for (int a = 0; a <= gdim; ++a)
+    if (a != lf) v[n++] = a;
+  if (gdim == 1) return 1.0;
+  if (gdim == 2) {
+    const double dx = xc[v[1] * 2] - xc[v[0] * 2], dy = xc[v[1] * 2 + 1] - xc[v[0] * 2 + 1];
+    return std::sqrt(dx * dx + dy * dy);
+  }
+  double e1[3], e2[3];
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = xc[v[1] * 3 + i] - xc[v[0] * 3 + i];
+    e2[i] = xc[v[2] * 3 + i] - xc[v[0] * 3 + i];
+  }
+  const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+               cz = e1[0] * e2[1] - e1[1] * e2[0];
+  return 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+}
+}  // namespace
+
+void orc_p1_facet_mass(int32_t gdim, const double* xc, int32_t lf, double alpha, double* A) {
+  const int nv = gdim + 1;
+  const double w = alpha * facet_measure(gdim, xc, lf) / (gdim * (gdim + 1.0));
+  for (int a = 0; a < nv; ++a)
+    for (int b = 0; b < nv; ++b)
+      A[a * nv + b] = (a == lf || b == lf) ? 0.0 : (a == b ? 2.0 * w : w);
+}
+
+void orc_p1_facet_load(int32_t gdim, const double* xc, int32_t lf, const double* g, double* b) {
+  const int nv = gdim + 1;
+  const double w = facet_measure(gdim, xc, lf) / (gdim * (gdim + 1.0));
+  double gs = 0.0;
+  for (int a = 0; a < nv; ++a)
+    if (a != lf) gs += g[a];
+  for (int a = 0; a < nv; ++a) b[a] = (a == lf) ? 0.0 : w * (g[a] + gs);
+}
+
+int32_t orc_assemble_facets(int32_t a_kind, int32_t L_kind, int32_t gdim, const int32_t* cells,
+                            const double* x, const int32_t* cell_dofs, const int32_t* coeff_dofs,
+                            int64_t num_facets, const int32_t* facet_cell,
+                            const int32_t* facet_local, const uint8_t* facet_mask, double alpha,
+                            const double* g, const int32_t* row_ptr, const int32_t* col,
+                            double* values, double* b) {
+  const int nv = gdim + 1;
+  double xc[12], gl[4], A[16], be[4];
+  for (int64_t k = 0; k < num_facets; ++k) {
+    if (facet_mask && !facet_mask[k]) continue;
+    const int64_t c = facet_cell[k];
+    const int lf = facet_local[k];
+    gather_cell_coords(gdim, cells + c * nv, x, xc);
+    const int32_t* d = cell_dofs + c * nv;
+    if (a_kind == ORC_P1_FACET_MASS && values) {
+      orc_p1_facet_mass(gdim, xc, lf, alpha, A);
+      for (int i = 0; i < nv; ++i)
+        for (int j = 0; j < nv; ++j) {
+          const int64_t slot = csr_find(row_ptr, col, d[i], d[j]);
+          if (slot < 0) return -2;
+          values[slot] += A[i * nv + j];
+        }
+    } else if (a_kind >= 0) {
+      return -1;
+    }
+    if (L_kind == ORC_P1_FACET_LOAD && b) {
+      restrict_p1(nv, coeff_dofs + c * nv, g, gl);
+      orc_p1_facet_load(gdim, xc, lf, gl, be);
+      for (int i = 0; i < nv; ++i) b[d[i]] += be[i];
+    } else if (L_kind >= 0) {
+      return -1;
+    }
+  }
+  return 0;
+}
+
+int32_t orc_assemble_system_facets(int32_t a_kind, int32_t L_kind, int32_t fa_kind,
+                                   int32_t fL_kind, int32_t gdim, int64_t num_cells,
+                                   const int32_t* cells, const double* x,
+                                   const int32_t* cell_dofs, const int32_t* coeff_dofs,
+                                   const double* a_c0, const double* L_c0, double alpha,
+                                   const double* g, int64_t num_facets, const int32_t* facet_cell,
+                                   const int32_t* facet_local, const uint8_t* facet_mask,
+                                   const uint8_t* bc_mask, const double* bc_value,
+                                   const int32_t* row_ptr, const int32_t* col, double* values,
+                                   double* b) {
+  const int nv = gdim + 1;
+  // exterior facets of each cell (first index into the facet list; facets are cell-sorted)
+  std::vector<int64_t> first(num_cells + 1, 0);
+  for (int64_t k = 0; k < num_facets; ++k) first[facet_cell[k] + 1]++;
+  for (int64_t c = 0; c < num_cells; ++c) first[c + 1] += first[c];
+  double xc[12], w0[4], l0[4], gl[4], A[16], be[4], F[16], fb[4];
+  for (int64_t c = 0; c < num_cells; ++c) {
+    gather_cell_coords(gdim, cells + c * nv, x, xc);
+    const int32_t* cd = coeff_dofs + c * nv;
+    const int32_t* d = cell_dofs + c * nv;
+    if (num_coeffs(a_kind) > 0) restrict_p1(nv, cd, a_c0, w0);
+    if (num_coeffs(L_kind) > 0) restrict_p1(nv, cd, L_c0, l0);
+    if (element_matrix(a_kind, gdim, xc, w0, nullptr, A) != nv) return -1;
+    if (element_vector(L_kind, gdim, xc, l0, nullptr, be) != nv) return -1;
+    for (int64_t k = first[c]; k < first[c + 1]; ++k) {
+      if (facet_mask && !facet_mask[k]) continue;
+      const int lf = facet_local[k];
+      if (fa_kind == ORC_P1_FACET_MASS) {
+        orc_p1_facet_mass(gdim, xc, lf, alpha, F);
+        for (int i = 0; i < nv * nv; ++i) A[i] += F[i];
+      }
+      if (fL_kind == ORC_P1_FACET_LOAD) {
+        restrict_p1(nv, cd, g, gl);
+        orc_p1_facet_load(gdim, xc, lf, gl, fb);
+        for (int i = 0; i < nv; ++i) be[i] += fb[i];
+      }
+    }
+    for (int i = 0; i < nv; ++i) {
+      const int32_t ii = d[i];
+      if (!bc_mask[ii]) continue;
+      const double gv = bc_value[ii];
+      for (int j = 0; j < nv; ++j) A[i * nv + j] = 0.0;
+      for (int r = 0; r < nv; ++r) be[r] -= A[r * nv + i] * gv;
+      for (int r = 0; r < nv; ++r) A[r * nv + i] = 0.0;
+      be[i] = gv;
+      A[i * nv + i] = 1.0;
+    }
+    if (values)
+      for (int i = 0; i < nv; ++i)
+        for (int j = 0; j < nv; ++j) {
+          const int64_t slot = csr_find(row_ptr, col, d[i], d[j]);
+          if (slot < 0) return -2;
+          values[slot] += A[i * nv + j];
+        }
+    if (b)
+      for (int i = 0; i < nv; ++i) b[d[i]] += be[i];
+  }
+  return 0;
+}
+
 double orc_adjoint_p1(int32_t gdim, int64_t num_cells, const int32_t* cells, const double* x,
                       int64_t nv, const int32_t* cell_dofs, int64_t ndofs, const double* kappa,
                       const double* f, const double* u, double* dJdkappa, double* dJdx) {

@@ -25,6 +25,13 @@
  *   orc_assemble_*      Assembler.cpp:105-194 / SystemAssembler.cpp:303-492,1036-1112;
  *                       pinned by 96.847818767384 / 96.564760289080 and
  *                       811.75365721381274 / 1.43583841167606474.
+ *   orc_exterior_facets / orc_p1_facet_* / orc_assemble_facets / orc_assemble_system_facets
+ *                       Facet::exterior + the P1 facet integrals; pinned by the boundary
+ *                       measures of python/test/unit/fem/test_assembler.py:140-153 (4.0 on
+ *                       UnitSquareMesh(4,4), 6.0 on UnitCubeMesh(2,2,2)) and :296-322 (1.0
+ *                       on the marked lower-left boundary of UnitSquareMesh(8,8)), the
+ *                       exterior facet counts 4n / 12n^2, and a Monte-Carlo-free closed
+ *                       form check of one facet (tests).
  *   orc_adjoint_*       closed-form derivative of J = u^T K u - F^T u; pinned by central
  *                       finite differences of orc_assemble_* (tests) -- the FD check is
  *                       independent of the derivative formula.
@@ -45,7 +52,9 @@ enum {
   ORC_VP2_ELASTICITY = 3, /* inner(2 mu eps(u) + lmbda tr eps(u) I, eps(v))*dx   */
   ORC_VP2_BODY_FORCE = 4, /* inner(f, v)*dx, f constant vector                    */
   ORC_VP1_ELASTICITY = 5, /* vector P1 (or DG1) version of ORC_VP2_ELASTICITY     */
-  ORC_VP1_BODY_FORCE = 6
+  ORC_VP1_BODY_FORCE = 6,
+  ORC_P1_FACET_MASS = 7,  /* alpha*u*v*ds, alpha constant (Robin term)              */
+  ORC_P1_FACET_LOAD = 8   /* g*v*ds, g P1 (Neumann term)                            */
 };
 
 /* ---- meshes ---------------------------------------------------------------- */
@@ -95,6 +104,42 @@ int32_t orc_assemble_system(int32_t a_kind, int32_t L_kind, int32_t gdim, int64_
                             const double* f3, const uint8_t* bc_mask, const double* bc_value,
                             const int32_t* row_ptr, const int32_t* col, double* values,
                             double* b);
+
+/* ---- exterior facets ----------------------------------------------------------------- */
+/* Facets that belong to exactly one cell (Facet::exterior), listed by cell and then by local
+   facet index (local facet k = the facet opposite local vertex k, UFC). Returns their number;
+   facet_cell / facet_local are written when non-NULL. */
+int64_t orc_exterior_facets(int32_t gdim, int64_t num_cells, const int32_t* cells,
+                            int32_t* facet_cell, int32_t* facet_local);
+/* P1 exterior-facet tensors in cell-local numbering ((gdim+1)^2 / gdim+1 entries, zero off
+   the facet): mass alpha |F| (1 + delta_ab) / (d (d + 1)) and load
+   |F| / (d (d + 1)) (g_a + sum_{b in F} g_b), d = gdim facet vertices, |F| = 1 for a point. */
+void orc_p1_facet_mass(int32_t gdim, const double* xc, int32_t local_facet, double alpha,
+                       double* A);
+void orc_p1_facet_load(int32_t gdim, const double* xc, int32_t local_facet, const double* g,
+                       double* b);
+/* Assembler::assemble_exterior_facets (Assembler.cpp:196-298) for the P1 facet forms over
+   the facets with facet_mask[k] != 0 (NULL: all), adding into values (a_kind) and b (L_kind,
+   g through coeff_dofs). a_kind / L_kind = -1 skips that tensor. */
+int32_t orc_assemble_facets(int32_t a_kind, int32_t L_kind, int32_t gdim, const int32_t* cells,
+                            const double* x, const int32_t* cell_dofs, const int32_t* coeff_dofs,
+                            int64_t num_facets, const int32_t* facet_cell,
+                            const int32_t* facet_local, const uint8_t* facet_mask, double alpha,
+                            const double* g, const int32_t* row_ptr, const int32_t* col,
+                            double* values, double* b);
+/* SystemAssembler::cell_wise_assembly with exterior-facet terms: each cell's facet tensors
+   are added to its cell tensors before apply_bc (SystemAssembler.cpp:421-481). P1 only;
+   fa_kind / fL_kind = -1 leaves out the facet matrix / vector. */
+int32_t orc_assemble_system_facets(int32_t a_kind, int32_t L_kind, int32_t fa_kind,
+                                   int32_t fL_kind, int32_t gdim, int64_t num_cells,
+                                   const int32_t* cells, const double* x,
+                                   const int32_t* cell_dofs, const int32_t* coeff_dofs,
+                                   const double* a_c0, const double* L_c0, double alpha,
+                                   const double* g, int64_t num_facets, const int32_t* facet_cell,
+                                   const int32_t* facet_local, const uint8_t* facet_mask,
+                                   const uint8_t* bc_mask, const double* bc_value,
+                                   const int32_t* row_ptr, const int32_t* col, double* values,
+                                   double* b);
 
 /* ---- adjoint ------------------------------------------------------------------------ */
 /* J = u^T K(kappa,x) u - F(f,x)^T u with K the P1 kappa-Poisson matrix and F the P1 load

@@ -12,7 +12,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _SO = os.path.join(_HERE, "_build", "liboracle.so")
 
-P1_POISSON, P1_LOAD, P1_MASS, VP2_ELASTICITY, VP2_BODY_FORCE, VP1_ELASTICITY, VP1_BODY_FORCE = range(7)
+(P1_POISSON, P1_LOAD, P1_MASS, VP2_ELASTICITY, VP2_BODY_FORCE, VP1_ELASTICITY, VP1_BODY_FORCE,
+ P1_FACET_MASS, P1_FACET_LOAD) = range(9)
 
 _lib = None
 
@@ -69,6 +70,18 @@ def _declare(L):
     L.orc_assemble_system.argtypes = [_i32, _i32, _i32, _i64, _i32p, _f64p, _i32, _i32p, _i32p,
                                       _vp, _vp, _vp, _vp, _u8p, _f64p, _i32p, _i32p, _vp, _vp]
     L.orc_assemble_system.restype = _i32
+    L.orc_exterior_facets.argtypes = [_i32, _i64, _i32p, _vp, _vp]
+    L.orc_exterior_facets.restype = _i64
+    L.orc_p1_facet_mass.argtypes = [_i32, _f64p, _i32, ctypes.c_double, _f64p]
+    L.orc_p1_facet_load.argtypes = [_i32, _f64p, _i32, _f64p, _f64p]
+    L.orc_assemble_facets.argtypes = [_i32, _i32, _i32, _i32p, _f64p, _i32p, _i32p, _i64, _i32p,
+                                      _i32p, _vp, ctypes.c_double, _vp, _i32p, _i32p, _vp, _vp]
+    L.orc_assemble_facets.restype = _i32
+    L.orc_assemble_system_facets.argtypes = [_i32, _i32, _i32, _i32, _i32, _i64, _i32p, _f64p,
+                                             _i32p, _i32p, _vp, _vp, ctypes.c_double, _vp, _i64,
+                                             _i32p, _i32p, _vp, _u8p, _f64p, _i32p, _i32p, _vp,
+                                             _vp]
+    L.orc_assemble_system_facets.restype = _i32
     L.orc_adjoint_p1.argtypes = [_i32, _i64, _i32p, _f64p, _i64, _i32p, _i64, _f64p, _f64p,
                                  _f64p, _f64p, _f64p]
     L.orc_adjoint_p1.restype = ctypes.c_double
@@ -230,6 +243,81 @@ def assemble_system(a_kind, L_kind, x, cells, cell_dofs, row_ptr, col, bc_mask, 
                                    _c(row_ptr, np.int32), _c(col, np.int32), _ptr(vals), _ptr(b))
     if rc != 0:
         raise RuntimeError(f"oracle assemble_system failed ({rc})")
+    return vals, b
+
+
+# ---- exterior facets -------------------------------------------------------------------
+def exterior_facets(cells):
+    """(facet_cell, facet_local) of the facets with one incident cell."""
+    cells = _c(cells, np.int32)
+    gdim = cells.shape[1] - 1
+    n = lib().orc_exterior_facets(gdim, cells.shape[0], cells, None, None)
+    fc = np.empty(n, np.int32)
+    fl = np.empty(n, np.int32)
+    lib().orc_exterior_facets(gdim, cells.shape[0], cells, _ptr(fc), _ptr(fl))
+    return fc, fl
+
+
+def p1_facet_mass(xc, local_facet, alpha=1.0):
+    xc = _c(xc, np.float64)
+    gdim = xc.shape[1]
+    A = np.empty((gdim + 1) ** 2)
+    lib().orc_p1_facet_mass(gdim, xc.ravel(), int(local_facet), float(alpha), A)
+    return A.reshape(gdim + 1, gdim + 1)
+
+
+def p1_facet_load(xc, local_facet, g):
+    xc = _c(xc, np.float64)
+    gdim = xc.shape[1]
+    b = np.empty(gdim + 1)
+    lib().orc_p1_facet_load(gdim, xc.ravel(), int(local_facet), _c(g, np.float64), b)
+    return b
+
+
+def assemble_facets(a_kind, L_kind, x, cells, cell_dofs, facets, row_ptr=None, col=None,
+                    ndofs=None, coeff_dofs=None, alpha=1.0, g=None, facet_mask=None):
+    """Exterior-facet terms (Assembler semantics): returns (values or None, b or None)."""
+    x = _c(x, np.float64)
+    cells = _c(cells, np.int32)
+    cd = _c(cell_dofs, np.int32)
+    kd = _c(cells if coeff_dofs is None else coeff_dofs, np.int32)
+    fc, fl = (_c(a, np.int32) for a in facets)
+    mask = None if facet_mask is None else _c(facet_mask, np.uint8)
+    g = None if g is None else _c(g, np.float64)
+    vals = np.zeros(col.shape[0]) if a_kind >= 0 else None
+    b = np.zeros(ndofs if ndofs is not None else row_ptr.shape[0] - 1) if L_kind >= 0 else None
+    rp = _c(row_ptr, np.int32) if row_ptr is not None else np.zeros(1, np.int32)
+    cl = _c(col, np.int32) if col is not None else np.zeros(1, np.int32)
+    rc = lib().orc_assemble_facets(a_kind, L_kind, x.shape[1], cells, x.ravel(), cd, kd,
+                                   fc.shape[0], fc, fl, _ptr(mask), float(alpha), _ptr(g), rp, cl,
+                                   _ptr(vals), _ptr(b))
+    if rc != 0:
+        raise RuntimeError(f"oracle assemble_facets failed ({rc})")
+    return vals, b
+
+
+def assemble_system_facets(a_kind, L_kind, fa_kind, fL_kind, x, cells, cell_dofs, facets,
+                           row_ptr, col, bc_mask, bc_value, coeff_dofs=None, a_c0=None,
+                           L_c0=None, alpha=1.0, g=None, facet_mask=None):
+    """SystemAssembler with the facet tensors merged into each cell tensor before apply_bc."""
+    x = _c(x, np.float64)
+    cells = _c(cells, np.int32)
+    cd = _c(cell_dofs, np.int32)
+    kd = _c(cells if coeff_dofs is None else coeff_dofs, np.int32)
+    fc, fl = (_c(a, np.int32) for a in facets)
+    mask = None if facet_mask is None else _c(facet_mask, np.uint8)
+    arrs = [None if a is None else _c(a, np.float64) for a in (a_c0, L_c0, g)]
+    vals = np.zeros(col.shape[0])
+    b = np.zeros(row_ptr.shape[0] - 1)
+    rc = lib().orc_assemble_system_facets(a_kind, L_kind, fa_kind, fL_kind, x.shape[1],
+                                          cells.shape[0], cells, x.ravel(), cd, kd,
+                                          _ptr(arrs[0]), _ptr(arrs[1]), float(alpha),
+                                          _ptr(arrs[2]), fc.shape[0], fc, fl, _ptr(mask),
+                                          _c(bc_mask, np.uint8), _c(bc_value, np.float64),
+                                          _c(row_ptr, np.int32), _c(col, np.int32), _ptr(vals),
+                                          _ptr(b))
+    if rc != 0:
+        raise RuntimeError(f"oracle assemble_system_facets failed ({rc})")
     return vals, b
 
 

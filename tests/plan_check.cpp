@@ -361,3 +361,17 @@ extern "C" int plan_check_p1_chains(int64_t nv, int64_t nc, const int32_t* cells
     return -100;
   }
 }
+
+// The library's exterior-facet list (host code) for comparison with the oracle's.
+extern "C" int64_t facets_host(int32_t gdim, int64_t nc, const int32_t* cells, int32_t* fcell,
+                               int32_t* flocal) {
+  std::vector<int32_t> fc;
+  std::vector<int8_t> fl;
+  dfa::build_exterior_facets(gdim, nc, cells, fc, fl);
+  if (fcell)
+    for (size_t i = 0; i < fc.size(); ++i) {
+      fcell[i] = fc[i];
+      flocal[i] = fl[i];
+    }
+  return static_cast<int64_t>(fc.size());
+}
