@@ -179,6 +179,28 @@ class DeviceVector {
   DeviceArray<double> values_;
 };
 
+// Exterior-facet terms of P1 forms (Assembler::assemble_exterior_facets,
+// dolfin/fem/Assembler.cpp:196-298): alpha*u*v*ds in the bilinear form and g*v*ds in the
+// linear form, over the facets selected by a device mask in dfa_exterior_facets order (an
+// exterior-facet subdomain; NULL = all exterior facets).
+struct FacetTerms {
+  bool robin = false;                 // alpha*u*v*ds present
+  double alpha = 0.0;
+  const double* g = nullptr;          // g on the coefficient dofs (device); NULL: no g*v*ds
+  const uint8_t* facet_mask = nullptr;
+  bool empty() const { return !robin && !g; }
+};
+
+// Exterior facets of a space: (cell, local facet) pairs, the order facet masks use.
+inline void exterior_facets(const Space& V, std::vector<int32_t>& facet_cell,
+                            std::vector<int32_t>& facet_local) {
+  int64_t n = 0;
+  check(dfa_exterior_facets(V.handle(), &n, nullptr, nullptr));
+  facet_cell.resize(static_cast<size_t>(n));
+  facet_local.resize(static_cast<size_t>(n));
+  check(dfa_exterior_facets(V.handle(), &n, facet_cell.data(), facet_local.data()));
+}
+
 // dolfin::AssemblerBase flags (dolfin/fem/AssemblerBase.h:46-67).
 struct AssemblerFlags {
   bool add_values = false;
@@ -208,6 +230,31 @@ class HipAssembler : public AssemblerFlags {
     if (finalize_tensor) b.apply("add");
   }
 
+  // Cell integrals, then exterior-facet terms (the order of Assembler::assemble,
+  // dolfin/fem/Assembler.cpp:52-103).
+  void assemble(DeviceCSRMatrix& A, const Form& a, const FacetTerms& ds) {
+    const bool fin = finalize_tensor;
+    finalize_tensor = false;
+    assemble(A, a);
+    finalize_tensor = fin;
+    if (ds.robin)
+      check(dfa_assemble_facets(a.space->handle(), DFA_FORM_P1_FACET_MASS, -1, a.x, ds.alpha,
+                                nullptr, ds.facet_mask, nullptr, nullptr, A.values(), nullptr,
+                                stream_));
+    if (finalize_tensor) A.apply("add");
+  }
+
+  void assemble(DeviceVector& b, const Form& L, const FacetTerms& ds) {
+    const bool fin = finalize_tensor;
+    finalize_tensor = false;
+    assemble(b, L);
+    finalize_tensor = fin;
+    if (ds.g)
+      check(dfa_assemble_facets(L.space->handle(), -1, DFA_FORM_P1_FACET_LOAD, L.x, 0.0, ds.g,
+                                ds.facet_mask, nullptr, nullptr, nullptr, b.values(), stream_));
+    if (finalize_tensor) b.apply("add");
+  }
+
  private:
   int32_t scatter_;
   hipStream_t stream_;
@@ -219,10 +266,12 @@ class HipSystemAssembler {
  public:
   HipSystemAssembler(Form a, Form L, const uint8_t* d_bc_mask = nullptr,
                      const double* d_bc_value = nullptr, int32_t scatter = DFA_SCATTER_AUTO,
-                     hipStream_t s = nullptr)
+                     hipStream_t s = nullptr, FacetTerms ds = FacetTerms())
       : a_(std::move(a)), L_(std::move(L)), mask_(d_bc_mask), value_(d_bc_value),
-        scatter_(scatter), stream_(s) {
+        scatter_(scatter), stream_(s), ds_(ds) {
     if (a_.space != L_.space) throw std::runtime_error("a and L must share the function space");
+    if ((mask_ == nullptr) != (value_ == nullptr))
+      throw std::runtime_error("Dirichlet mask and values go together");
   }
 
   void assemble(DeviceCSRMatrix& A, DeviceVector& b) { run(&A, &b); }
@@ -236,12 +285,20 @@ class HipSystemAssembler {
     check(dfa_assemble_system(a_.space->handle(), a_.kind, L_.kind, a_.x, a_.c0, a_.c1, L_.c0,
                               L_.f3, mask_, value_, A ? A->values() : nullptr,
                               b ? b->values() : nullptr, scatter_, 0, stream_));
+    // exterior-facet tensors join their cell's tensor before apply_bc
+    // (dolfin/fem/SystemAssembler.cpp:421-481)
+    if (!ds_.empty())
+      check(dfa_assemble_facets(a_.space->handle(), ds_.robin ? DFA_FORM_P1_FACET_MASS : -1,
+                                ds_.g ? DFA_FORM_P1_FACET_LOAD : -1, a_.x, ds_.alpha, ds_.g,
+                                ds_.facet_mask, mask_, value_, A ? A->values() : nullptr,
+                                b ? b->values() : nullptr, stream_));
   }
   Form a_, L_;
   const uint8_t* mask_;
   const double* value_;
   int32_t scatter_;
   hipStream_t stream_;
+  FacetTerms ds_;
 };
 
 }  // namespace dolfin_amd

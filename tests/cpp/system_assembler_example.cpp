@@ -4,7 +4,8 @@
 //   P1 Poisson + DirichletBC(1) on UnitCubeMesh(4): |A|_F = 96.847818767384,
 //   |b|_2 = 96.564760289080 (python/test/unit/fem/test_system_assembler.py:73-91);
 //   P1 Poisson on UnitIntervalMesh(48): |A|_F = 811.75365721381274, |b| = 1.43583841167606474
-//   (python/test/unit/fem/test_assembler.py:39-55).
+//   (python/test/unit/fem/test_assembler.py:39-55);
+//   the boundary measure 6.0 of the unit cube through g*v*ds (test_assembler.py:140-153).
 // Exit code 0 when every check holds.
 #include <algorithm>
 #include <cmath>
@@ -76,6 +77,20 @@ int main() {
     ok &= close(nA, 96.847818767384) && close(nb, 96.564760289080);
     sys.assemble(A);  // re-assembly of A only
     ok &= close(A.norm("frobenius"), 96.847818767384);
+    // 0*v*dx + 1*v*ds: the entries sum to the boundary measure 6.0
+    // (python/test/unit/fem/test_assembler.py:140-153)
+    DeviceArray<double> zero;
+    std::vector<double> h0(nv, 0.0);
+    zero.upload(h0.data(), h0.size());
+    FacetTerms ds;
+    ds.g = one.get();
+    HipAssembler fa;
+    DeviceVector bs;
+    fa.assemble(bs, Form{V, DFA_FORM_P1_LOAD, 1, dx.get(), zero.get()}, ds);
+    double area = 0.0;
+    for (double v : bs.get_local()) area += v;
+    std::printf("UnitCubeMesh(4) 1*v*ds: sum = %.15f\n", area);
+    ok &= close(area, 6.0);
   }
   {
     const int n = 48;
