@@ -567,6 +567,36 @@ int dfa_assemble_facets(dfa_assembler h, int32_t a_form, int32_t L_form, const d
   });
 }
 
+int dfa_csr_spmv(dfa_assembler h, const double* d_values, double alpha, const double* d_x,
+                 double beta, double* d_y, void* stream) {
+  return guarded([&]() -> int {
+    if (!h || !d_values || !d_x || !d_y) return fail("NULL argument", DFA_ERR_ARG);
+    dfa::CsrView A;
+    A.nrows = h->ndofs;
+    A.nnz = h->nnz;
+    A.row_ptr = h->d_row_ptr;
+    A.col = h->d_col;
+    A.values = d_values;
+    hip_check(dfa::launch_spmv(A, alpha, d_x, beta, d_y, static_cast<hipStream_t>(stream)),
+              "spmv_kernel");
+    return DFA_OK;
+  });
+}
+
+int dfa_csr_diagonal(dfa_assembler h, const double* d_values, double* d_diag, void* stream) {
+  return guarded([&]() -> int {
+    if (!h || !d_values || !d_diag) return fail("NULL argument", DFA_ERR_ARG);
+    dfa::CsrView A;
+    A.nrows = h->ndofs;
+    A.nnz = h->nnz;
+    A.row_ptr = h->d_row_ptr;
+    A.col = h->d_col;
+    A.values = d_values;
+    hip_check(dfa::launch_diagonal(A, d_diag, static_cast<hipStream_t>(stream)), "diagonal_kernel");
+    return DFA_OK;
+  });
+}
+
 int dfa_plan_info(dfa_assembler h, int32_t scatter, int64_t* plan_bytes, int64_t* num_blocks,
                   int64_t* cell_appearances, int32_t* lds_bytes) {
   return guarded([&]() -> int {

@@ -93,6 +93,18 @@ struct FacetPass {
 };
 hipError_t launch_facets(const FacetTopo& t, const FacetPass& p, hipStream_t s);
 
+// ---- device CSR consumer side ------------------------------------------------------------
+struct CsrView {
+  int64_t nrows = 0, nnz = 0;
+  const int32_t* row_ptr = nullptr;
+  const int32_t* col = nullptr;
+  const double* values = nullptr;
+};
+int spmv_group(int64_t nrows, int64_t nnz);
+hipError_t launch_spmv(const CsrView& A, double alpha, const double* x, double beta, double* y,
+                       hipStream_t s);
+hipError_t launch_diagonal(const CsrView& A, double* d, hipStream_t s);
+
 // ---- vector P2 elasticity --------------------------------------------------------------
 struct VP2Topo {
   int64_t num_cells = 0;

@@ -2,8 +2,12 @@
 
 Matrix mirrors EigenMatrix (row-major CSR, sorted columns, int32 indices; dolfin/la/
 EigenMatrix.h:46-230): init from the engine's structure, zero, apply (no-op: values live in
-HBM), nnz, size, norm, data() export (EigenMatrix.cpp:341-355).
+HBM), nnz, size, norm, data() export (EigenMatrix.cpp:341-355), mult (y = A x,
+EigenMatrix.cpp:230-255) and the diagonal. solve() is dolfin.solve(A, x, b, "cg", "jacobi")
+(python/dolfin/la/solver.py:27, dolfin/la/solve.cpp:40) on the device: Jacobi-preconditioned
+conjugate gradients as EigenKrylovSolver runs them, with every vector in HBM.
 """
+
 import numpy as np
 import torch
 
@@ -60,6 +64,24 @@ class Matrix:
         torch.cuda.synchronize()
         return rp.cpu().numpy(), col.cpu().numpy(), self.values.cpu().numpy()
 
+    def mult(self, x, y=None, alpha=1.0, beta=0.0):
+        """y = alpha * A x + beta * y (device tensors or Vectors); returns y."""
+        xv = x.values if isinstance(x, Vector) else x
+        if y is None:
+            y = torch.empty(self.size(), dtype=torch.float64, device=self.values.device)
+            beta = 0.0
+        yv = y.values if isinstance(y, Vector) else y
+        _lib.check(_lib.load().dfa_csr_spmv(self._engine.handle, _lib.ptr(self.values),
+                                            float(alpha), _lib.ptr(xv), float(beta), _lib.ptr(yv),
+                                            _lib.stream()))
+        return y
+
+    def diagonal(self):
+        d = torch.empty(self.size(), dtype=torch.float64, device=self.values.device)
+        _lib.check(_lib.load().dfa_csr_diagonal(self._engine.handle, _lib.ptr(self.values),
+                                                _lib.ptr(d), _lib.stream()))
+        return d
+
     def array(self):
         rp, col, v = self.data()
         n = rp.shape[0] - 1
@@ -100,3 +122,50 @@ class Vector:
 
     def get_local(self):
         return self.values.cpu().numpy()
+
+
+def solve(A, x, b, method="cg", preconditioner="jacobi", rtol=1e-10, atol=0.0, maxiter=None,
+          check_every=10):
+    """Solve A x = b on the device (x holds the initial guess and receives the solution).
+    method "cg" with preconditioner "jacobi" or "none" (EigenKrylovSolver's methods, dolfin/la/
+    EigenKrylovSolver.cpp:43-56); stops when |r| <= max(rtol |b|, atol). Returns the number
+    of iterations (dolfin::solve returns the same count); raises when it does not converge."""
+    if method != "cg":
+        raise ValueError("only the conjugate gradient method is provided")
+    if preconditioner not in ("jacobi", "none", "default"):
+        raise ValueError("preconditioner must be 'jacobi' or 'none'")
+    xv = x.values if isinstance(x, Vector) else x
+    bv = b.values if isinstance(b, Vector) else b
+    n = A.size()
+    if maxiter is None:
+        maxiter = 10 * n
+    if preconditioner == "none":
+        dinv = None
+    else:
+        d = A.diagonal()
+        dinv = torch.where(d != 0, 1.0 / d, torch.ones_like(d))
+    r = torch.empty_like(bv)
+    r.copy_(bv)
+    A.mult(xv, r, alpha=-1.0, beta=1.0)          # r = b - A x
+    z = r * dinv if dinv is not None else r.clone()
+    p = z.clone()
+    rz = torch.dot(r, z)
+    tol = max(rtol * float(torch.linalg.vector_norm(bv)), atol)
+    Ap = torch.empty_like(bv)
+    it = 0
+    while it < maxiter:
+        if it % check_every == 0 and float(torch.linalg.vector_norm(r)) <= tol:
+            return it
+        A.mult(p, Ap)
+        alpha = rz / torch.dot(p, Ap)
+        xv.add_(alpha * p)
+        r.sub_(alpha * Ap)
+        z = r * dinv if dinv is not None else r
+        rz_new = torch.dot(r, z)
+        p.mul_(rz_new / rz).add_(z)
+        rz = rz_new
+        it += 1
+    if float(torch.linalg.vector_norm(r)) <= tol:
+        return it
+    raise RuntimeError(f"conjugate gradients did not converge in {maxiter} iterations")
+

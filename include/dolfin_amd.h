@@ -197,6 +197,18 @@ int dfa_assemble_facets(dfa_assembler h, int32_t a_form, int32_t L_form, const d
                         const uint8_t* d_bc_mask, const double* d_bc_value, double* d_values,
                         double* d_b, void* stream);
 
+/*
+ * d_y = alpha * A d_x + beta * d_y for the matrix with the handle's CSR structure and values
+ * d_values (device): EigenMatrix::mult (dolfin/la/EigenMatrix.cpp:230-255) without leaving
+ * HBM. beta = 0 does not read d_y.
+ */
+int dfa_csr_spmv(dfa_assembler h, const double* d_values, double alpha, const double* d_x,
+                 double beta, double* d_y, void* stream);
+
+/* The diagonal of that matrix (0 where the pattern has no diagonal entry): the Jacobi
+   preconditioner of EigenKrylovSolver (dolfin/la/EigenKrylovSolver.cpp:52-56). */
+int dfa_csr_diagonal(dfa_assembler h, const double* d_values, double* d_diag, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

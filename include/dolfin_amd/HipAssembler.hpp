@@ -8,8 +8,8 @@
 //                        (dolfin/fem/AssemblerBase.h:46-67), assemble(tensor, form)
 //   HipSystemAssembler   (a, L, Dirichlet values) -> assemble(A, b) / (A) / (b)
 //   DeviceCSRMatrix      the GenericMatrix subset the assembler uses: empty, size, nnz, zero,
-//                        apply, norm("frobenius"), data() (dolfin/la/EigenMatrix.cpp:77-105,
-//                        180-186, 341-355, 417-420)
+//                        apply, norm("frobenius"), data(), mult, diagonal
+//                        (dolfin/la/EigenMatrix.cpp:77-105, 180-186, 230-255, 341-355, 417-420)
 //   DeviceVector         GenericVector subset: size, zero, apply, norm("l2"), get_local
 // Errors from the library are rethrown as std::runtime_error with the library's message,
 // like dolfin_error (dolfin/log/Logger.cpp:142-185).
@@ -139,6 +139,15 @@ class DeviceCSRMatrix {
     double s = 0.0;
     for (double v : values_.download()) s += v * v;
     return std::sqrt(s);
+  }
+  // y = alpha A x + beta y on the device (EigenMatrix::mult, EigenMatrix.cpp:230-255);
+  // x and y hold size(0) doubles in device memory.
+  void mult(const double* x, double* y, double alpha = 1.0, double beta = 0.0, hipStream_t s = nullptr) const {
+    check(dfa_csr_spmv(space_->handle(), values_.get(), alpha, x, beta, y, s));
+  }
+  // The diagonal into d (size(0) doubles, device memory): the Jacobi preconditioner's input.
+  void diagonal(double* d, hipStream_t s = nullptr) const {
+    check(dfa_csr_diagonal(space_->handle(), values_.get(), d, s));
   }
   // CSR export (EigenMatrix::data): row_ptr, col, values on the host.
   void data(std::vector<int32_t>& row_ptr, std::vector<int32_t>& col, std::vector<double>& vals) const {

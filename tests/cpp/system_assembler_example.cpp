@@ -114,6 +114,19 @@ int main() {
     assembler.assemble(b, Form{V, DFA_FORM_P1_LOAD, 1, dx.get(), ten.get()});
     std::printf("UnitIntervalMesh(48) P1: |A| = %.14f |b| = %.17f\n", A.norm("frobenius"), b.norm("l2"));
     ok &= close(A.norm("frobenius"), 811.75365721381274) && close(b.norm("l2"), 1.43583841167606474);
+    // stiffness rows sum to zero, so A 1 = 0; the diagonal is 2/h inside and 1/h at the ends
+    DeviceVector y, d;
+    y.init(n + 1);
+    d.init(n + 1);
+    A.mult(one.get(), y.values());
+    A.diagonal(d.values());
+    double ymax = 0.0;
+    for (double v : y.get_local()) ymax = std::fmax(ymax, std::fabs(v));
+    const std::vector<double> hd = d.get_local();
+    bool dok = close(hd[0], n) && close(hd[n], n);
+    for (int i = 1; i < n; ++i) dok &= close(hd[i], 2.0 * n);
+    std::printf("UnitIntervalMesh(48) P1: max|A 1| = %.3e, diagonal %s\n", ymax, dok ? "ok" : "wrong");
+    ok &= ymax < 1e-10 && dok;
   }
   std::printf(ok ? "adapter checks passed\n" : "adapter checks FAILED\n");
   return ok ? 0 : 1;
