@@ -150,6 +150,17 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
       h->d_kdofs = h->base.upload(h_coeff_dofs, static_cast<size_t>(nc * nvc), s);
       h->d_row_ptr = h->base.upload(csr.row_ptr, s);
       h->d_col = h->base.upload(csr.col, s);
+      {
+        const std::vector<int32_t> sb = dfa::build_spmv_blocks(csr.row_ptr, dfa::kSpmvBudget,
+                                                               dfa::kSpmvMaxRows);
+        std::vector<int32_t> pairs(2 * sb.size());
+        for (size_t i = 0; i < sb.size(); ++i) {
+          pairs[2 * i] = sb[i];
+          pairs[2 * i + 1] = csr.row_ptr[sb[i]];
+        }
+        h->spmv_nblocks = static_cast<int64_t>(sb.size()) - 1;
+        h->d_spmv_blocks = h->base.upload(pairs, s);
+      }
       if (element == DFA_ELEMENT_P1) {
         if (h_x) {
           try {
@@ -577,6 +588,8 @@ int dfa_csr_spmv(dfa_assembler h, const double* d_values, double alpha, const do
     A.row_ptr = h->d_row_ptr;
     A.col = h->d_col;
     A.values = d_values;
+    A.blocks = h->d_spmv_blocks;
+    A.nblocks = h->spmv_nblocks;
     hip_check(dfa::launch_spmv(A, alpha, d_x, beta, d_y, static_cast<hipStream_t>(stream)),
               "spmv_kernel");
     return DFA_OK;

@@ -56,6 +56,8 @@ struct dfa_assembler_s {
   int32_t* d_kdofs = nullptr;
   int32_t* d_row_ptr = nullptr;
   int32_t* d_col = nullptr;
+  int32_t* d_spmv_blocks = nullptr;  // SpMV row blocks: (first row, its row_ptr) pairs
+  int64_t spmv_nblocks = 0;
   dfa::DeviceArena base;
   // blocked P1 plan
   bool has_plan = false;

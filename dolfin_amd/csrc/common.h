@@ -76,6 +76,12 @@ void build_p1_chain_plan(const P1BlockPlan& plan, P1ChainPlan& chains);
 void build_exterior_facets(int32_t gdim, int64_t num_cells, const int32_t* cells,
                            std::vector<int32_t>& facet_cell, std::vector<int8_t>& facet_local);
 
+// Row blocks of the streamed CSR SpMV: maximal runs of consecutive rows holding at most
+// `budget` nonzeros and at most `max_rows` rows; a row longer than `budget` is a block of its
+// own. Returns the first row of each block followed by the row count.
+std::vector<int32_t> build_spmv_blocks(const std::vector<int32_t>& row_ptr, int32_t budget,
+                                       int32_t max_rows);
+
 // Throws std::runtime_error on inconsistent input or exceeded limits.
 void build_p1_block_plan(int32_t gdim, int64_t num_vertices, int64_t num_cells,
                          const int32_t* cells, const int32_t* cell_dofs,

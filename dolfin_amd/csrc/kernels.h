@@ -94,13 +94,18 @@ struct FacetPass {
 hipError_t launch_facets(const FacetTopo& t, const FacetPass& p, hipStream_t s);
 
 // ---- device CSR consumer side ------------------------------------------------------------
+// Streamed SpMV: a workgroup takes one row block (<= kSpmvBudget nonzeros, <= kSpmvMaxRows
+// rows, from build_spmv_blocks) and stages its products in LDS.
+constexpr int kSpmvBudget = 4096;
+constexpr int kSpmvMaxRows = 256;
 struct CsrView {
   int64_t nrows = 0, nnz = 0;
   const int32_t* row_ptr = nullptr;
   const int32_t* col = nullptr;
   const double* values = nullptr;
+  const int32_t* blocks = nullptr;  // nblocks + 1 pairs (first row, row_ptr of that row)
+  int64_t nblocks = 0;
 };
-int spmv_group(int64_t nrows, int64_t nnz);
 hipError_t launch_spmv(const CsrView& A, double alpha, const double* x, double beta, double* y,
                        hipStream_t s);
 hipError_t launch_diagonal(const CsrView& A, double* d, hipStream_t s);

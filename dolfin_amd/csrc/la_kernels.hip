@@ -3,10 +3,14 @@
 // Jacobi preconditioner of EigenKrylovSolver, dolfin/la/EigenKrylovSolver.cpp:52-56), so
 // that an assembled matrix can be used without leaving HBM.
 //
-// One power-of-two group of G lanes per row (G from the mean row length: 16 for P1
-// tetrahedra, 64 for vector P2): the group reads the row's values and columns coalesced,
-// gathers x, and reduces with cross-lane shuffles. HBM-bound: 12 bytes per nonzero plus the
-// row pointers and y; x is gathered through L2.
+// Streamed SpMV (the CSR-stream scheme of CSR-Adaptive, Greathouse & Daga, SC'14): a
+// workgroup takes one row block of at most 4096 nonzeros and 256 rows, reads its values and
+// columns as one contiguous coalesced stream (eight independent loads per thread in flight),
+// gathers x, stages the products in LDS and reduces each row there with a power-of-two group
+// of lanes. A row longer than 4096 nonzeros is a block of its own, summed by the whole
+// workgroup. Workgroups are mapped so that each XCD sweeps a contiguous range of rows, which
+// keeps the x gathers of neighbouring rows in one L2. HBM-bound: 12 bytes per nonzero plus
+// the row pointers and y.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -14,25 +18,66 @@
 namespace dfa {
 namespace {
 
-constexpr int kLaThreads = 256;
+constexpr int kLaThreads = 512;
+constexpr int kNpt = kSpmvBudget / kLaThreads;  // products per thread
 
-template <int G>
-__global__ __launch_bounds__(kLaThreads) void spmv_kernel(int64_t nrows, const int32_t* __restrict__ rp,
-                                                          const int32_t* __restrict__ col,
-                                                          const double* __restrict__ vals,
-                                                          double alpha, const double* __restrict__ x,
-                                                          double beta, double* __restrict__ y) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kLaThreads + threadIdx.x;
-  const int64_t row = gid / G;
-  const int ln = threadIdx.x % G;
-  double s = 0.0;
-  if (row < nrows) {
-    const int b = rp[row], e = rp[row + 1];
-    for (int k = b + ln; k < e; k += G) s = fma(vals[k], x[col[k]], s);
+__global__ __launch_bounds__(kLaThreads) void spmv_stream_kernel(
+    int nblocks, const int32_t* __restrict__ blocks, const int32_t* __restrict__ rp,
+    const int32_t* __restrict__ col, const double* __restrict__ vals, double alpha,
+    const double* __restrict__ x, double beta, double* __restrict__ y) {
+  __shared__ double prod[kSpmvBudget];
+  __shared__ int32_t srp[kSpmvMaxRows + 1];
+  __shared__ double wsum[kLaThreads / 64];
+  const int q = nblocks >> 3, rem = nblocks & 7;
+  const int xcd = blockIdx.x & 7;
+  const int blk = xcd * q + min(xcd, rem) + static_cast<int>(blockIdx.x >> 3);
+  const int tid = threadIdx.x;
+  // (first row, first nonzero) pairs: the value stream starts without waiting for row_ptr
+  const int2 b0 = reinterpret_cast<const int2*>(blocks)[blk];
+  const int2 b1 = reinterpret_cast<const int2*>(blocks)[blk + 1];
+  const int r0 = b0.x, nr = b1.x - b0.x;
+  const int base = b0.y, end = b1.y;
+  if (end - base > kSpmvBudget) {  // a single long row
+    double s = 0.0;
+    for (int k = base + tid; k < end; k += kLaThreads) s = fma(vals[k], x[col[k]], s);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((tid & 63) == 0) wsum[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kLaThreads / 64; ++w) t += wsum[w];
+      y[r0] = beta == 0.0 ? alpha * t : fma(beta, y[r0], alpha * t);
+    }
+    return;
+  }
+  for (int i = tid; i <= nr; i += kLaThreads) srp[i] = rp[r0 + i] - base;
+  int c[kNpt];
+  double v[kNpt];
+#pragma unroll
+  for (int j = 0; j < kNpt; ++j) {
+    const int k = base + j * kLaThreads + tid;
+    c[j] = k < end ? col[k] : 0;
+    v[j] = k < end ? vals[k] : 0.0;
   }
 #pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
-  if (row < nrows && ln == 0) y[row] = beta == 0.0 ? alpha * s : fma(beta, y[row], alpha * s);
+  for (int j = 0; j < kNpt; ++j) {
+    const int k = j * kLaThreads + tid;
+    if (base + k < end) prod[k] = v[j] * x[c[j]];
+  }
+  __syncthreads();
+  int lg = 0;  // lanes per row: 2^lg, the largest power of two with nr * 2^lg <= threads, <= 64
+  while (lg < 6 && (nr << (lg + 1)) <= kLaThreads) ++lg;
+  const int L = 1 << lg, groups = kLaThreads >> lg;
+  const int ln = tid & (L - 1);
+  for (int lr = tid >> lg; lr - (tid >> lg) < nr; lr += groups) {  // uniform trip count
+    double s = 0.0;
+    if (lr < nr)
+      for (int k = srp[lr] + ln; k < srp[lr + 1]; k += L) s += prod[k];
+    for (int o = L >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lr < nr && ln == 0) y[r0 + lr] = beta == 0.0 ? alpha * s : fma(beta, y[r0 + lr], alpha * s);
+  }
 }
 
 __global__ __launch_bounds__(kLaThreads) void diagonal_kernel(int64_t nrows, const int32_t* __restrict__ rp,
@@ -56,35 +101,16 @@ __global__ __launch_bounds__(kLaThreads) void diagonal_kernel(int64_t nrows, con
   d[r] = v;
 }
 
-template <int G>
-hipError_t launch_spmv_g(const CsrView& A, double alpha, const double* x, double beta, double* y,
-                         hipStream_t s) {
-  const int64_t threads = A.nrows * G;
-  const int64_t grid = (threads + kLaThreads - 1) / kLaThreads;
-  hipLaunchKernelGGL((spmv_kernel<G>), dim3(static_cast<unsigned>(grid)), dim3(kLaThreads), 0, s,
-                     A.nrows, A.row_ptr, A.col, A.values, alpha, x, beta, y);
-  return hipGetLastError();
-}
-
 }  // namespace
-
-int spmv_group(int64_t nrows, int64_t nnz) {
-  const double mean = nrows > 0 ? static_cast<double>(nnz) / static_cast<double>(nrows) : 1.0;
-  int g = 4;
-  while (g < 64 && g < mean) g *= 2;
-  return g;
-}
 
 hipError_t launch_spmv(const CsrView& A, double alpha, const double* x, double beta, double* y,
                        hipStream_t s) {
   if (A.nrows == 0) return hipSuccess;
-  switch (spmv_group(A.nrows, A.nnz)) {
-    case 4: return launch_spmv_g<4>(A, alpha, x, beta, y, s);
-    case 8: return launch_spmv_g<8>(A, alpha, x, beta, y, s);
-    case 16: return launch_spmv_g<16>(A, alpha, x, beta, y, s);
-    case 32: return launch_spmv_g<32>(A, alpha, x, beta, y, s);
-    default: return launch_spmv_g<64>(A, alpha, x, beta, y, s);
-  }
+  if (!A.blocks || A.nblocks <= 0 || A.nblocks > INT32_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(spmv_stream_kernel, dim3(static_cast<unsigned>(A.nblocks)), dim3(kLaThreads), 0,
+                     s, static_cast<int>(A.nblocks), A.blocks, A.row_ptr, A.col, A.values, alpha, x,
+                     beta, y);
+  return hipGetLastError();
 }
 
 hipError_t launch_diagonal(const CsrView& A, double* d, hipStream_t s) {

@@ -392,6 +392,22 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   });
 }
 
+std::vector<int32_t> build_spmv_blocks(const std::vector<int32_t>& row_ptr, int32_t budget,
+                                       int32_t max_rows) {
+  const int64_t n = static_cast<int64_t>(row_ptr.size()) - 1;
+  std::vector<int32_t> blocks;
+  blocks.reserve(static_cast<size_t>(n / 16 + 2));
+  int64_t r = 0;
+  while (r < n) {
+    blocks.push_back(static_cast<int32_t>(r));
+    int64_t e = r + 1;  // the first row always joins, however long
+    while (e < n && e - r < max_rows && row_ptr[e + 1] - row_ptr[r] <= budget) ++e;
+    r = e;
+  }
+  blocks.push_back(static_cast<int32_t>(n));
+  return blocks;
+}
+
 void build_exterior_facets(int32_t gdim, int64_t nc, const int32_t* cells,
                            std::vector<int32_t>& facet_cell, std::vector<int8_t>& facet_local) {
   const int nv = gdim + 1;

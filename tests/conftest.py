@@ -68,6 +68,8 @@ def plan_lib():
         L.plan_check_p1_chains.restype = ctypes.c_int
         L.facets_host.argtypes = [ctypes.c_int32, ctypes.c_int64, vp, vp, vp]
         L.facets_host.restype = ctypes.c_int64
+        L.spmv_blocks_host.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, vp]
+        L.spmv_blocks_host.restype = ctypes.c_int64
         L.plan_check_vp2.argtypes = [ctypes.c_int64, vp, vp, vp, vp, ctypes.c_int64, vp, vp,
                                      ctypes.c_int32, vp, ctypes.c_int64, vp]
         L.plan_check_vp2.restype = ctypes.c_int

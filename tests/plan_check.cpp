@@ -375,3 +375,14 @@ extern "C" int64_t facets_host(int32_t gdim, int64_t nc, const int32_t* cells, i
     }
   return static_cast<int64_t>(fc.size());
 }
+
+// The library's SpMV row blocks (host code); returns the number of blocks, fills blocks
+// (nblocks + 1 entries) when given.
+extern "C" int64_t spmv_blocks_host(const int32_t* row_ptr, int64_t nrows, int32_t budget,
+                                    int32_t max_rows, int32_t* blocks) {
+  std::vector<int32_t> rp(row_ptr, row_ptr + nrows + 1);
+  const std::vector<int32_t> b = dfa::build_spmv_blocks(rp, budget, max_rows);
+  if (blocks) std::copy(b.begin(), b.end(), blocks);
+  return static_cast<int64_t>(b.size()) - 1;
+}
+

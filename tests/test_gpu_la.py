@@ -57,6 +57,26 @@ def test_spmv_and_diagonal_vs_scipy(kind, n):
     assert np.array_equal(d.cpu().numpy(), S.diagonal())
 
 
+def test_spmv_long_row_fan_mesh():
+    # a fan of 5000 triangles around one vertex: its row holds 5001 nonzeros, more than one
+    # streamed row block takes, and is summed by a whole workgroup
+    m = 5000
+    t = 2 * np.pi * np.arange(m) / m
+    x = np.vstack([[0.0, 0.0], np.c_[np.cos(t), np.sin(t)]])
+    cells = np.array([[0, 1 + i, 1 + (i + 1) % m] for i in range(m)], dtype=np.int32)
+    cells.sort(axis=1)
+    mesh = dfa.Mesh(x, cells)
+    V = dfa.FunctionSpace(mesh, "P", 1)
+    A = dfa.assemble(dfa.poisson_form(V, dfa.Function(V, np.ones(V.dim()))))
+    S = _scipy(A)
+    assert np.diff(S.indptr).max() == m + 1
+    xv = np.random.default_rng(3).uniform(-1, 1, S.shape[0])
+    y = A.mult(torch.from_numpy(xv).cuda())
+    torch.cuda.synchronize()
+    assert _rel(y.cpu().numpy(), S @ xv) < 1e-14
+    assert np.array_equal(A.diagonal().cpu().numpy(), S.diagonal())
+
+
 def test_spmv_accepts_vectors():
     # Vector objects are accepted wherever device tensors are
     A = _p1_matrix(3)

@@ -162,6 +162,32 @@ def test_xcd_block_mapping_is_bijective():
         assert np.array_equal(np.sort(blk), np.arange(nb))
 
 
+@pytest.mark.parametrize("case", ["p1_cube", "fan", "empty_rows"])
+def test_spmv_row_blocks(plan_lib, case):
+    # row blocks of the streamed SpMV: cover every row once, in order; <= 256 rows and
+    # <= 4096 nonzeros per block unless the block is one longer row; maximal (the next row
+    # would break a limit)
+    if case == "p1_cube":
+        rp = host_csr(plan_lib, dfa.UnitCubeMesh(9, 7, 8).cells(), 720)[0]
+    elif case == "fan":
+        lens = np.r_[np.full(300, 4), 5001, np.full(700, 4), 9000, 4096, 4097, np.full(600, 15)]
+        rp = np.r_[0, np.cumsum(lens)].astype(np.int32)
+    else:
+        lens = np.random.default_rng(2).integers(0, 3, 5000)
+        rp = np.r_[0, np.cumsum(lens)].astype(np.int32)
+    n = rp.shape[0] - 1
+    nb = plan_lib.spmv_blocks_host(p(rp), n, 4096, 256, None)
+    b = np.empty(nb + 1, np.int32)
+    plan_lib.spmv_blocks_host(p(rp), n, 4096, 256, p(b))
+    assert b[0] == 0 and b[-1] == n and np.all(np.diff(b) > 0)
+    for r0, r1 in zip(b[:-1], b[1:]):
+        nnz = rp[r1] - rp[r0]
+        assert r1 - r0 <= 256
+        assert nnz <= 4096 or r1 - r0 == 1
+        if r1 < n and nnz <= 4096:
+            assert r1 - r0 == 256 or rp[r1 + 1] - rp[r0] > 4096
+
+
 def test_library_exports_header_symbols():
     from dolfin_amd import _lib
     hdr = open(os.path.join(ROOT, "include", "dolfin_amd.h")).read()
