@@ -155,10 +155,13 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   ids.reserve(ndofs);
   for (int64_t d = 0; d < ndofs; ++d)
     if (dofv[d] >= 0) ids.push_back(static_cast<int32_t>(d));
-  std::vector<std::pair<int64_t, int64_t>> leaves;
-  std::function<void(int64_t, int64_t)> rcb = [&](int64_t lo, int64_t hi) {
+  using Leaves = std::vector<std::pair<int64_t, int64_t>>;
+  // The two halves of the upper levels are split on threads of their own; each subtree's
+  // leaves are appended in depth-first order, so the result equals the serial recursion.
+  std::function<void(int64_t, int64_t, int, Leaves&)> rcb = [&](int64_t lo, int64_t hi, int depth,
+                                                               Leaves& out) {
     if (hi - lo <= target_rows) {
-      if (hi > lo) leaves.emplace_back(lo, hi);
+      if (hi > lo) out.emplace_back(lo, hi);
       return;
     }
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
@@ -181,10 +184,20 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
                        const double pb = x[static_cast<int64_t>(dofv[b]) * gdim + ax];
                        return pa < pb || (pa == pb && a < b);
                      });
-    rcb(lo, mid);
-    rcb(mid, hi);
+    if (depth < 4 && hi - lo > 65536) {
+      Leaves left, right;
+      std::thread th([&]() { rcb(lo, mid, depth + 1, left); });
+      rcb(mid, hi, depth + 1, right);
+      th.join();
+      out.insert(out.end(), left.begin(), left.end());
+      out.insert(out.end(), right.begin(), right.end());
+    } else {
+      rcb(lo, mid, depth + 1, out);
+      rcb(mid, hi, depth + 1, out);
+    }
   };
-  rcb(0, static_cast<int64_t>(ids.size()));
+  Leaves leaves;
+  rcb(0, static_cast<int64_t>(ids.size()), 0, leaves);
   const int32_t nb = static_cast<int32_t>(leaves.size());
   P.num_blocks = nb;
   std::vector<int32_t> owner(ndofs, -1);
@@ -203,20 +216,56 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
     }
     return n;
   };
-  for (int64_t c = 0; c < nc; ++c) {
-    int32_t bl[4];
-    const int n = distinct_blocks(c, bl);
-    for (int k = 0; k < n; ++k) P.cell_ptr[bl[k] + 1]++;
+  // counted and filled over contiguous cell ranges, one per thread; thread t's cells of a
+  // block follow thread t-1's, so each block lists its cells in ascending order
+  const int nthr = static_cast<int>(std::min<int64_t>(num_threads(), std::max<int64_t>(nc / 65536, 1)));
+  std::vector<std::vector<int32_t>> tcount(nthr, std::vector<int32_t>(nb, 0));
+  auto cell_range = [&](int t, int64_t& c0, int64_t& c1) {
+    c0 = nc * t / nthr;
+    c1 = nc * (t + 1) / nthr;
+  };
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthr; ++t)
+      th.emplace_back([&, t]() {
+        int64_t c0, c1;
+        cell_range(t, c0, c1);
+        for (int64_t c = c0; c < c1; ++c) {
+          int32_t bl[4];
+          const int n = distinct_blocks(c, bl);
+          for (int k = 0; k < n; ++k) tcount[t][bl[k]]++;
+        }
+      });
+    for (auto& x : th) x.join();
   }
-  for (int32_t b = 0; b < nb; ++b) P.cell_ptr[b + 1] += P.cell_ptr[b];
+  for (int32_t b = 0; b < nb; ++b) {
+    int32_t s0 = 0;
+    for (int t = 0; t < nthr; ++t) s0 += tcount[t][b];
+    P.cell_ptr[b + 1] = P.cell_ptr[b] + s0;
+  }
   std::vector<int32_t> app(P.cell_ptr[nb]);
   {
-    std::vector<int32_t> pos(P.cell_ptr.begin(), P.cell_ptr.end() - 1);
-    for (int64_t c = 0; c < nc; ++c) {
-      int32_t bl[4];
-      const int n = distinct_blocks(c, bl);
-      for (int k = 0; k < n; ++k) app[pos[bl[k]]++] = static_cast<int32_t>(c);
+    for (int32_t b = 0; b < nb; ++b) {  // tcount becomes each thread's first position per block
+      int32_t pos = P.cell_ptr[b];
+      for (int t = 0; t < nthr; ++t) {
+        const int32_t n = tcount[t][b];
+        tcount[t][b] = pos;
+        pos += n;
+      }
     }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthr; ++t)
+      th.emplace_back([&, t]() {
+        int64_t c0, c1;
+        cell_range(t, c0, c1);
+        auto& pos = tcount[t];
+        for (int64_t c = c0; c < c1; ++c) {
+          int32_t bl[4];
+          const int n = distinct_blocks(c, bl);
+          for (int k = 0; k < n; ++k) app[pos[bl[k]]++] = static_cast<int32_t>(c);
+        }
+      });
+    for (auto& x : th) x.join();
   }
 
   // 4. per-block tables, built in parallel into per-block vectors then concatenated
@@ -230,30 +279,39 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   P.blk_nnz.assign(nb, 0);
   std::atomic<int> failed{0};
   std::string fail_msg;
-  std::vector<std::vector<int32_t>> blk_dofs(nb);
-  for (int32_t b = 0; b < nb; ++b) {
-    blk_dofs[b].assign(ids.begin() + leaves[b].first, ids.begin() + leaves[b].second);
-    std::sort(blk_dofs[b].begin(), blk_dofs[b].end());
-  }
   parallel_for(nb, [&](int64_t b0, int64_t b1, int) {
     for (int64_t b = b0; b < b1; ++b) {
       // Local node numbering: the owned rows first, in ascending dof order (local node i is
       // owned row i), then the ghost vertices in ascending vertex order.
+      std::vector<int32_t> blk_dofs(ids.begin() + leaves[b].first, ids.begin() + leaves[b].second);
+      std::sort(blk_dofs.begin(), blk_dofs.end());
       auto& nodes = bnodes[b];
-      std::vector<int32_t> all;
+      // vertex -> local node: a per-thread dense map, cleared again when the block is done
+      thread_local std::vector<int32_t> vloc;
+      if (static_cast<int64_t>(vloc.size()) < nv) vloc.assign(static_cast<size_t>(nv), -1);
+      struct Clear {
+        std::vector<int32_t>& m;
+        const std::vector<int32_t>& n;
+        ~Clear() {
+          for (int32_t v : n) m[v] = -1;
+        }
+      } clear_on_exit{vloc, nodes};
+      for (int32_t d : blk_dofs) {
+        vloc[dofv[d]] = static_cast<int32_t>(nodes.size());
+        nodes.push_back(dofv[d]);
+      }
+      const size_t nown = nodes.size();
       for (int32_t k = P.cell_ptr[b]; k < P.cell_ptr[b + 1]; ++k)
-        for (int a = 0; a < nvc; ++a) all.push_back(cells[static_cast<int64_t>(app[k]) * nvc + a]);
-      std::sort(all.begin(), all.end());
-      all.erase(std::unique(all.begin(), all.end()), all.end());
-      for (int32_t d : blk_dofs[b]) nodes.push_back(dofv[d]);
-      for (int32_t v : all)
-        if (owner[vdof[v]] != static_cast<int32_t>(b)) nodes.push_back(v);
-      std::vector<std::pair<int32_t, uint32_t>> vmap(nodes.size());
-      for (size_t i = 0; i < nodes.size(); ++i) vmap[i] = {nodes[i], static_cast<uint32_t>(i)};
-      std::sort(vmap.begin(), vmap.end());
-      auto local = [&](int32_t v) {
-        return std::lower_bound(vmap.begin(), vmap.end(), std::make_pair(v, 0u))->second;
-      };
+        for (int a = 0; a < nvc; ++a) {
+          const int32_t v = cells[static_cast<int64_t>(app[k]) * nvc + a];
+          if (vloc[v] == -1) {  // a ghost, first seen
+            vloc[v] = -2;
+            nodes.push_back(v);
+          }
+        }
+      std::sort(nodes.begin() + static_cast<std::ptrdiff_t>(nown), nodes.end());
+      for (size_t i = nown; i < nodes.size(); ++i) vloc[nodes[i]] = static_cast<int32_t>(i);
+      auto local = [&](int32_t v) { return static_cast<uint32_t>(vloc[v]); };
       if (nodes.size() > static_cast<size_t>(kMaxBlockNodes)) {
         failed = 1;
         continue;
@@ -261,7 +319,7 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
       // owned rows
       std::vector<int32_t> row_off_of_node(nodes.size(), -1);
       uint32_t off = 0;
-      for (int32_t d : blk_dofs[b]) {
+      for (int32_t d : blk_dofs) {
         const uint32_t ln = local(dofv[d]);
         {
           const int32_t* rb = csr.col.data() + csr.row_ptr[d];
@@ -278,7 +336,7 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
         continue;
       }
       {
-        const auto& bd = blk_dofs[b];
+        const auto& bd = blk_dofs;
         uint32_t loff = 0;
         size_t i = 0;
         while (i < bd.size()) {
@@ -319,26 +377,44 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
           for (int a = 1; a < nvc; ++a) lmin = std::min(lmin, ln[a]);
           col[k - kb0] = {colour * 65536 + static_cast<int>(lmin), static_cast<int32_t>(c)};
         }
-        std::stable_sort(col.begin(), col.end(),
-                         [](const std::pair<int, int32_t>& x, const std::pair<int, int32_t>& y) {
-                           return x.first < y.first;
-                         });
+        // stable order by (colour, smallest local node): two stable counting sorts
+        std::vector<std::pair<int, int32_t>> tmp(col.size());
+        auto counting = [](const std::vector<std::pair<int, int32_t>>& in,
+                           std::vector<std::pair<int, int32_t>>& out, int range, auto digit) {
+          std::vector<int32_t> cnt(static_cast<size_t>(range) + 1, 0);
+          for (const auto& e : in) cnt[digit(e) + 1]++;
+          for (int i = 0; i < range; ++i) cnt[i + 1] += cnt[i];
+          for (const auto& e : in) out[cnt[digit(e)]++] = e;
+        };
+        counting(col, tmp, static_cast<int>(nodes.size()), [](const std::pair<int, int32_t>& e) { return e.first & 0xffff; });
+        counting(tmp, col, 64, [](const std::pair<int, int32_t>& e) { return e.first >> 16; });
         for (int32_t k = kb0; k < kb1; ++k) app[k] = col[k - kb0].second;
+      }
+      // column ranks: rankof[i * nn + j] = position of local node j's dof in owned row i (every
+      // column of an owned row is a vertex of one of the block's cells, so it has a local node)
+      const size_t nn = nodes.size();
+      thread_local std::vector<uint16_t> rankof;
+      if (rankof.size() < nown * nn) rankof.resize(nown * nn);
+      for (size_t i = 0; i < nown; ++i) {
+        const int32_t d = blk_dofs[i];
+        const int32_t r0 = csr.row_ptr[d];
+        for (int32_t r = r0; r < csr.row_ptr[d + 1]; ++r)
+          rankof[i * nn + static_cast<size_t>(vloc[dofv[csr.col[r]]])] =
+              static_cast<uint16_t>(std::min<int32_t>(r - r0, 65535));
       }
       for (int32_t k = P.cell_ptr[b]; k < P.cell_ptr[b + 1]; ++k) {
         const int64_t c = app[k];
         uint64_t idw = 0, rk = 0;
-        for (int a = 0; a < nvc; ++a)
-          idw |= static_cast<uint64_t>(local(cells[c * nvc + a])) << (16 * a);
+        uint32_t ln[4];
         for (int a = 0; a < nvc; ++a) {
-          const int32_t da = cd[c * nvc + a];
-          if (owner[da] != b) continue;
-          const int32_t* rb = csr.col.data() + csr.row_ptr[da];
-          const int32_t* re = csr.col.data() + csr.row_ptr[da + 1];
+          ln[a] = local(cells[c * nvc + a]);
+          idw |= static_cast<uint64_t>(ln[a]) << (16 * a);
+        }
+        for (int a = 0; a < nvc; ++a) {
+          if (ln[a] >= nown) continue;  // not an owned row
           for (int bb = 0; bb < nvc; ++bb) {
-            const int32_t db = cd[c * nvc + bb];
-            const int64_t r = std::lower_bound(rb, re, db) - rb;
-            if (r >= kMaxRank) {
+            const uint32_t r = rankof[ln[a] * nn + ln[bb]];
+            if (r >= static_cast<uint32_t>(kMaxRank)) {
               failed = 3;
               continue;
             }
@@ -488,31 +564,52 @@ void build_p1_chain_plan(const P1BlockPlan& P, P1ChainPlan& C) {
       std::vector<std::array<uint32_t, 4>> cl(ncb);
       for (int32_t k = 0; k < ncb; ++k)
         for (int a = 0; a < 4; ++a) cl[k][a] = static_cast<uint32_t>((P.cell_ids[c0 + k] >> (16 * a)) & 0xffffu);
-      // faces: sorted (key, cell) list, key = sorted local ids of the three vertices
+      // face key = the sorted local ids of its three vertices
       auto fkey = [](uint32_t a, uint32_t b, uint32_t c) {
         if (a > b) std::swap(a, b);
         if (b > c) std::swap(b, c);
         if (a > b) std::swap(a, b);
         return static_cast<uint64_t>(a) << 32 | static_cast<uint64_t>(b) << 16 | c;
       };
-      std::vector<std::pair<uint64_t, int32_t>> faces;
-      faces.reserve(4 * static_cast<size_t>(ncb));
-      for (int32_t k = 0; k < ncb; ++k)
-        for (int o = 0; o < 4; ++o) {
-          uint32_t v[3];
-          int n = 0;
-          for (int a = 0; a < 4; ++a)
-            if (a != o) v[n++] = cl[k][a];
-          faces.emplace_back(fkey(v[0], v[1], v[2]), k);
-        }
-      std::sort(faces.begin(), faces.end());
+      // face neighbours: nbr[4k + o] = the other cell of the block across the face of cell k
+      // opposite its vertex o (a face has at most two cells), -1 on the block's boundary,
+      // matched through an open-addressing table of face keys (linear in the block's faces)
+      std::vector<int32_t> nbr(4 * static_cast<size_t>(ncb), -1);
+      {
+        size_t cap = 16;
+        while (cap < 8 * static_cast<size_t>(ncb)) cap <<= 1;
+        std::vector<uint64_t> hkey(cap, ~0ull);
+        std::vector<int32_t> hval(cap, -1);
+        for (int32_t k = 0; k < ncb; ++k)
+          for (int o = 0; o < 4; ++o) {
+            uint32_t v[3];
+            int n = 0;
+            for (int a = 0; a < 4; ++a)
+              if (a != o) v[n++] = cl[k][a];
+            const uint64_t key = fkey(v[0], v[1], v[2]);
+            size_t h = static_cast<size_t>((key * 0x9E3779B97F4A7C15ull) >> 32) & (cap - 1);
+            while (hkey[h] != ~0ull && hkey[h] != key) h = (h + 1) & (cap - 1);
+            const int32_t me = 4 * k + o;
+            if (hkey[h] == key) {
+              const int32_t other = hval[h];
+              nbr[me] = other >> 2;
+              nbr[other] = k;
+            } else {
+              hkey[h] = key;
+              hval[h] = me;
+            }
+          }
+      }
       std::vector<char> used(ncb, 0);
-      // the unassigned cell other than k across face (p, q, r), or -1
+      // the unassigned cell other than k across face (p, q, r) of cell k, or -1
       auto across = [&](int32_t k, uint32_t p, uint32_t q, uint32_t r) {
-        const uint64_t key = fkey(p, q, r);
-        auto it = std::lower_bound(faces.begin(), faces.end(), std::make_pair(key, int32_t{-1}));
-        for (; it != faces.end() && it->first == key; ++it)
-          if (it->second != k && !used[it->second]) return it->second;
+        for (int o = 0; o < 4; ++o) {
+          const uint32_t v = cl[k][o];
+          if (v != p && v != q && v != r) {
+            const int32_t c = nbr[4 * static_cast<size_t>(k) + o];
+            return c >= 0 && !used[c] ? c : int32_t{-1};
+          }
+        }
         return int32_t{-1};
       };
       auto fourth = [&](int32_t k, uint32_t p, uint32_t q, uint32_t r) {
@@ -532,63 +629,73 @@ void build_p1_chain_plan(const P1BlockPlan& P, P1ChainPlan& C) {
         if (au < 0 || av < 0) return uint32_t{0xffu};
         return static_cast<uint32_t>((P.cell_rank[c0 + k] >> (4 * (au * 4 + av))) & 15u);
       };
+      // cell i of a chain is (p, q, r[i], r[i + 1]); fixed-size storage, no allocation
       struct Chain {
-        uint32_t p, q;
-        std::vector<uint32_t> r;   // m + 1 nodes
-        std::vector<int32_t> cells;
+        uint32_t p = 0, q = 0;
+        int m = 0;
+        uint32_t r[kChainLen + 1];
+        int32_t cells[kChainLen];
       };
       std::vector<Chain> chains;
+      chains.reserve(static_cast<size_t>(ncb));
       static const int EA[6] = {0, 0, 0, 1, 1, 2}, EB[6] = {1, 2, 3, 2, 3, 3};
+      constexpr int K = kChainLen;
       for (int32_t k0 = 0; k0 < ncb; ++k0) {
         if (used[k0]) continue;
         Chain best;
         for (int e = 0; e < 6; ++e) {
-          Chain ch;
-          ch.p = cl[k0][EA[e]];
-          ch.q = cl[k0][EB[e]];
+          const uint32_t p = cl[k0][EA[e]], q = cl[k0][EB[e]];
           uint32_t rs[2];
           int n = 0;
           for (int a = 0; a < 4; ++a)
             if (a != EA[e] && a != EB[e]) rs[n++] = cl[k0][a];
-          std::vector<int32_t> fw{k0};
-          std::vector<uint32_t> fr{rs[0], rs[1]};
+          // cells cb[lo, hi), nodes rb[lo, hi]: room for K - 1 cells on either side of k0
+          int32_t cb[2 * K];
+          uint32_t rb[2 * K + 1];
+          int lo = K - 1, hi = K;
+          cb[lo] = k0;
+          rb[lo] = rs[0];
+          rb[hi] = rs[1];
           used[k0] = 1;
           int32_t cur = k0;
-          while (static_cast<int>(fw.size()) < kChainLen) {  // forward across (p, q, r_last)
-            const int32_t c = across(cur, ch.p, ch.q, fr.back());
+          while (hi - lo < K) {  // forward across (p, q, r_last)
+            const int32_t c = across(cur, p, q, rb[hi]);
             if (c < 0) break;
-            const uint32_t nx = fourth(c, ch.p, ch.q, fr.back());
-            fw.push_back(c);
-            fr.push_back(nx);
+            rb[hi + 1] = fourth(c, p, q, rb[hi]);
+            cb[hi++] = c;
             used[c] = 1;
             cur = c;
           }
           cur = k0;
-          while (static_cast<int>(fw.size()) < kChainLen) {  // backward across (p, q, r_first)
-            const int32_t c = across(cur, ch.p, ch.q, fr.front());
+          while (hi - lo < K) {  // backward across (p, q, r_first)
+            const int32_t c = across(cur, p, q, rb[lo]);
             if (c < 0) break;
-            const uint32_t nx = fourth(c, ch.p, ch.q, fr.front());
-            fw.insert(fw.begin(), c);
-            fr.insert(fr.begin(), nx);
+            const uint32_t nx = fourth(c, p, q, rb[lo]);
+            cb[--lo] = c;
+            rb[lo] = nx;
             used[c] = 1;
             cur = c;
           }
-          for (int32_t c : fw) used[c] = 0;
-          ch.cells = fw;
-          ch.r = fr;
-          if (ch.cells.size() > best.cells.size()) best = ch;
-          if (static_cast<int>(best.cells.size()) == kChainLen) break;
+          for (int i = lo; i < hi; ++i) used[cb[i]] = 0;
+          if (hi - lo > best.m) {
+            best.p = p;
+            best.q = q;
+            best.m = hi - lo;
+            for (int i = lo; i < hi; ++i) best.cells[i - lo] = cb[i];
+            for (int i = lo; i <= hi; ++i) best.r[i - lo] = rb[i];
+          }
+          if (best.m == K) break;
         }
-        for (int32_t c : best.cells) used[c] = 1;
-        chains.push_back(std::move(best));
+        for (int i = 0; i < best.m; ++i) used[best.cells[i]] = 1;
+        chains.push_back(best);
       }
       std::stable_sort(chains.begin(), chains.end(),
-                       [](const Chain& a, const Chain& b) { return a.cells.size() > b.cells.size(); });
+                       [](const Chain& a, const Chain& b) { return a.m > b.m; });
       auto& out = brec[b];
       out.assign(chains.size() * kChainWords, 0u);
       for (size_t i = 0; i < chains.size(); ++i) {
         const Chain& ch = chains[i];
-        const int m = static_cast<int>(ch.cells.size());
+        const int m = ch.m;
         uint32_t* w = out.data() + i * kChainWords;
         uint32_t ids[9];
         ids[0] = ch.p;

@@ -104,6 +104,15 @@ int32_t orc_assemble_system(int32_t a_kind, int32_t L_kind, int32_t gdim, int64_
                             const double* f3, const uint8_t* bc_mask, const double* bc_value,
                             const int32_t* row_ptr, const int32_t* col, double* values,
                             double* b);
+/* The same loop on nthreads host threads with atomic scatter (CPU timing only). */
+int32_t orc_assemble_system_threads(int32_t a_kind, int32_t L_kind, int32_t gdim,
+                                    int64_t num_cells, const int32_t* cells, const double* x,
+                                    int32_t ndofs_cell, const int32_t* cell_dofs,
+                                    const int32_t* coeff_dofs, const double* a_c0,
+                                    const double* a_c1, const double* L_c0, const double* f3,
+                                    const uint8_t* bc_mask, const double* bc_value,
+                                    const int32_t* row_ptr, const int32_t* col, double* values,
+                                    double* b, int32_t nthreads);
 
 /* ---- exterior facets ----------------------------------------------------------------- */
 /* Facets that belong to exactly one cell (Facet::exterior), listed by cell and then by local

@@ -70,6 +70,8 @@ def _declare(L):
     L.orc_assemble_system.argtypes = [_i32, _i32, _i32, _i64, _i32p, _f64p, _i32, _i32p, _i32p,
                                       _vp, _vp, _vp, _vp, _u8p, _f64p, _i32p, _i32p, _vp, _vp]
     L.orc_assemble_system.restype = _i32
+    L.orc_assemble_system_threads.argtypes = L.orc_assemble_system.argtypes + [_i32]
+    L.orc_assemble_system_threads.restype = _i32
     L.orc_exterior_facets.argtypes = [_i32, _i64, _i32p, _vp, _vp]
     L.orc_exterior_facets.restype = _i64
     L.orc_p1_facet_mass.argtypes = [_i32, _f64p, _i32, ctypes.c_double, _f64p]
@@ -228,7 +230,9 @@ def assemble_vector(kind, x, cells, cell_dofs, ndofs, coeff_dofs=None, c0=None, 
 
 def assemble_system(a_kind, L_kind, x, cells, cell_dofs, row_ptr, col, bc_mask, bc_value,
                     coeff_dofs=None, a_c0=None, a_c1=None, L_c0=None, f3=None,
-                    want_A=True, want_b=True):
+                    want_A=True, want_b=True, threads=0):
+    """threads=0: the serial loop (ascending cells); threads>0: the same loop on that many host
+    threads with atomic scatter (for CPU timing; equal up to summation order)."""
     x = _c(x, np.float64)
     cells = _c(cells, np.int32)
     cd = _c(cell_dofs, np.int32)
@@ -237,10 +241,13 @@ def assemble_system(a_kind, L_kind, x, cells, cell_dofs, row_ptr, col, bc_mask, 
     ndofs = row_ptr.shape[0] - 1
     vals = np.zeros(col.shape[0]) if want_A else None
     b = np.zeros(ndofs) if want_b else None
-    rc = lib().orc_assemble_system(a_kind, L_kind, x.shape[1], cells.shape[0], cells, x.ravel(),
-                                   cd.shape[1], cd, kd, *[_ptr(a) for a in arrs],
-                                   _c(bc_mask, np.uint8), _c(bc_value, np.float64),
-                                   _c(row_ptr, np.int32), _c(col, np.int32), _ptr(vals), _ptr(b))
+    args = (a_kind, L_kind, x.shape[1], cells.shape[0], cells, x.ravel(), cd.shape[1], cd, kd,
+            *[_ptr(a) for a in arrs], _c(bc_mask, np.uint8), _c(bc_value, np.float64),
+            _c(row_ptr, np.int32), _c(col, np.int32), _ptr(vals), _ptr(b))
+    if threads > 0:
+        rc = lib().orc_assemble_system_threads(*args, int(threads))
+    else:
+        rc = lib().orc_assemble_system(*args)
     if rc != 0:
         raise RuntimeError(f"oracle assemble_system failed ({rc})")
     return vals, b

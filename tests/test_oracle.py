@@ -25,6 +25,23 @@ def test_system_assembler_p1_bc_unitcube4(oracle):
     np.testing.assert_array_equal(b, b2)
 
 
+def test_threaded_system_loop_matches_serial(oracle):
+    # the multi-core timing variant: same values up to summation order, same known answers
+    O = oracle
+    x, cells = O.box_mesh(6, 5, 4)
+    rp, col = O.sparsity(cells, x.shape[0])
+    bc = O.boundary_vertices(x).astype(np.uint8)
+    rng = np.random.default_rng(4)
+    k, f, g = rng.uniform(0.5, 2, x.shape[0]), rng.uniform(-1, 1, x.shape[0]), rng.uniform(-1, 1, x.shape[0])
+    A, b = O.assemble_system(O.P1_POISSON, O.P1_LOAD, x, cells, cells, rp, col, bc, g,
+                             a_c0=k, L_c0=f)
+    for t in (1, 3, 8):
+        At, bt = O.assemble_system(O.P1_POISSON, O.P1_LOAD, x, cells, cells, rp, col, bc, g,
+                                   a_c0=k, L_c0=f, threads=t)
+        assert np.linalg.norm(At - A) <= 1e-14 * np.linalg.norm(A)
+        assert np.linalg.norm(bt - b) <= 1e-14 * np.linalg.norm(b)
+
+
 def test_interval_p1(oracle):
     # python/test/unit/fem/test_assembler.py:39-55
     O = oracle
