@@ -24,8 +24,8 @@ inline void hip_check(hipError_t e, const char* what) {
 struct DeviceArena {
   std::vector<void*> ptrs;
   int64_t bytes = 0;
-  template <class T>
-  T* upload(const std::vector<T>& v, hipStream_t s) {
+  template <class T, class A>
+  T* upload(const std::vector<T, A>& v, hipStream_t s) {
     return upload(v.data(), v.size(), s);
   }
   template <class T>

@@ -3,6 +3,8 @@
 
 #include <cstdint>
 #include <string>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include "../../include/dolfin_amd.h"
@@ -13,9 +15,30 @@ namespace dfa {
 void set_error(const std::string& msg);
 const char* last_error();
 
+// Allocator whose resize() leaves new elements uninitialised: large host arrays that are
+// filled in parallel right after allocation are first touched by the filling threads.
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = UninitAlloc<U>;
+  };
+  UninitAlloc() = default;
+  template <class U>
+  UninitAlloc(const UninitAlloc<U>&) noexcept {}
+  template <class U>
+  void construct(U* p) noexcept {
+    ::new (static_cast<void*>(p)) U;
+  }
+  template <class U, class... A>
+  void construct(U* p, A&&... a) {
+    ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+  }
+};
+
 struct HostCSR {
   std::vector<int32_t> row_ptr;
-  std::vector<int32_t> col;
+  std::vector<int32_t, UninitAlloc<int32_t>> col;
 };
 
 // Sorted unique column set per row from cell_dofs x cell_dofs (+ optional diagonal):
@@ -102,8 +125,8 @@ struct VP2BlockPlan {
   std::vector<int32_t> own_rows;   // 3 global rows (component dofs) per owned node
   std::vector<uint32_t> own_info;  // LDS value offset | (neighbour count << 16)
   std::vector<int32_t> task_ptr;   // [nb+1]
-  std::vector<uint64_t> task;      // 3 words per work item (see build_vp2_block_plan)
-  std::vector<int32_t> task_cell;  // cell of each work item (Dirichlet path only)
+  std::vector<uint64_t, UninitAlloc<uint64_t>> task;  // 3 words per work item (see build_vp2_block_plan)
+  std::vector<int32_t, UninitAlloc<int32_t>> task_cell;  // cell of each work item (Dirichlet path only)
   std::vector<int32_t> task_nvert; // [nb] work items of vertex nodes (they come first)
   std::vector<int32_t> blk_vals;   // [nb] LDS values per block
 };

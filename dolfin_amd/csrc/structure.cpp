@@ -5,6 +5,7 @@
 #include <array>
 #include <atomic>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <numeric>
 #include <stdexcept>
@@ -60,6 +61,24 @@ void build_csr(int64_t num_cells, int32_t nd, const int32_t* cd, int64_t num_row
     for (int64_t c = 0; c < num_cells; ++c)
       for (int j = 0; j < nd; ++j) adj[pos[cd[c * nd + j]]++] = static_cast<int32_t>(c);
   }
+  // Rows with the same (non-empty) cell list have the same columns, the union of those
+  // cells' dofs (the dofs of one vector-element node, say): rep[r] is the smallest row of r's
+  // first cell with r's cell list, and only representatives are built; the rest are copies.
+  const int32_t* adjp = adj.data();
+  std::vector<int32_t> rep(num_rows);
+  parallel_for(num_rows, [&](int64_t b, int64_t e, int) {
+    for (int64_t r = b; r < e; ++r) {
+      rep[r] = static_cast<int32_t>(r);
+      const int64_t n = aptr[r + 1] - aptr[r];
+      if (n == 0) continue;
+      const int32_t c0 = adjp[aptr[r]];
+      for (int j = 0; j < nd; ++j) {
+        const int32_t q = cd[static_cast<int64_t>(c0) * nd + j];
+        if (q >= rep[r] || aptr[q + 1] - aptr[q] != n) continue;
+        if (std::equal(adjp + aptr[q], adjp + aptr[q + 1], adjp + aptr[r])) rep[r] = q;
+      }
+    }
+  });
   std::vector<int32_t> len(num_rows);
   const int T = num_threads();
   std::vector<std::vector<int32_t>> marks(T);
@@ -67,6 +86,7 @@ void build_csr(int64_t num_cells, int32_t nd, const int32_t* cd, int64_t num_row
     auto& mark = marks[t];
     mark.assign(num_rows, -1);
     for (int64_t r = b; r < e; ++r) {
+      if (rep[r] != r) continue;
       int32_t n = 0;
       for (int64_t k = aptr[r]; k < aptr[r + 1]; ++k) {
         const int32_t* d = cd + static_cast<int64_t>(adj[k]) * nd;
@@ -80,6 +100,9 @@ void build_csr(int64_t num_cells, int32_t nd, const int32_t* cd, int64_t num_row
       len[r] = n;
     }
   });
+  parallel_for(num_rows, [&](int64_t b, int64_t e, int) {
+    for (int64_t r = b; r < e; ++r) len[r] = len[rep[r]];
+  });
   out.row_ptr.assign(num_rows + 1, 0);
   int64_t acc = 0;
   for (int64_t r = 0; r < num_rows; ++r) {
@@ -92,6 +115,7 @@ void build_csr(int64_t num_cells, int32_t nd, const int32_t* cd, int64_t num_row
     auto& mark = marks[t];
     std::fill(mark.begin(), mark.end(), -1);
     for (int64_t r = b; r < e; ++r) {
+      if (rep[r] != r) continue;
       int32_t* dst = out.col.data() + out.row_ptr[r];
       int32_t n = 0;
       for (int64_t k = aptr[r]; k < aptr[r + 1]; ++k) {
@@ -106,6 +130,12 @@ void build_csr(int64_t num_cells, int32_t nd, const int32_t* cd, int64_t num_row
       std::sort(dst, dst + n);
     }
     std::vector<int32_t>().swap(mark);
+  });
+  parallel_for(num_rows, [&](int64_t b, int64_t e, int) {
+    for (int64_t r = b; r < e; ++r)
+      if (rep[r] != r)
+        std::copy(out.col.data() + out.row_ptr[rep[r]], out.col.data() + out.row_ptr[rep[r] + 1],
+                  out.col.data() + out.row_ptr[r]);
   });
 }
 
@@ -769,34 +799,40 @@ bool build_vp2_ranks(int64_t nc, const int32_t* cd, int64_t ndofs, const HostCSR
   if (!layout) layout = 1;
 
   rank.assign(static_cast<size_t>(nc) * 100, 0);
-  bool bad = false;
-  for (int64_t c = 0; c < nc && !bad; ++c)
-    for (int a = 0; a < 10; ++a) {
-      const int32_t ra = cd[c * 30 + a];  // component-0 row of node a
-      const int32_t rb0 = csr.row_ptr[ra], re0 = csr.row_ptr[ra + 1];
-      const int32_t len = re0 - rb0;
-      if (len % 3) { bad = true; break; }
-      const int32_t deg = len / 3;
-      for (int b = 0; b < 10; ++b) {
-        // locate column (component 0, node b) in row ra
-        const int32_t colb = cd[c * 30 + b];
-        const int32_t* p = std::lower_bound(csr.col.data() + rb0, csr.col.data() + re0, colb);
-        if (p == csr.col.data() + re0 || *p != colb) { bad = true; break; }
-        const int64_t pos = p - (csr.col.data() + rb0);
-        const int64_t rk = (layout == 1) ? pos : pos / 3;
-        if (rk > 255 || rk >= deg) { bad = true; break; }
-        rank[c * 100 + a * 10 + b] = static_cast<uint8_t>(rk);
-        // verify all 9 component combinations land on the expected columns
-        for (int i = 0; i < 3 && !bad; ++i) {
-          const int32_t row = cd[c * 30 + i * 10 + a];
-          if (csr.row_ptr[row + 1] - csr.row_ptr[row] != len) { bad = true; break; }
-          for (int j = 0; j < 3; ++j) {
-            const int64_t slot = csr.row_ptr[row] + ((layout == 1) ? (rk + j * deg) : (rk * 3 + j));
-            if (csr.col[slot] != cd[c * 30 + j * 10 + b]) { bad = true; break; }
+  std::atomic<bool> failed{false};
+  parallel_for(nc, [&](int64_t c0, int64_t c1, int) {
+    bool bad = false;
+    for (int64_t c = c0; c < c1 && !bad; ++c)
+      for (int a = 0; a < 10 && !bad; ++a) {
+        const int32_t ra = cd[c * 30 + a];  // component-0 row of node a
+        const int32_t rb0 = csr.row_ptr[ra], re0 = csr.row_ptr[ra + 1];
+        const int32_t len = re0 - rb0;
+        if (len % 3) { bad = true; break; }
+        const int32_t deg = len / 3;
+        for (int b = 0; b < 10; ++b) {
+          // locate column (component 0, node b) in row ra
+          const int32_t colb = cd[c * 30 + b];
+          const int32_t* p = std::lower_bound(csr.col.data() + rb0, csr.col.data() + re0, colb);
+          if (p == csr.col.data() + re0 || *p != colb) { bad = true; break; }
+          const int64_t pos = p - (csr.col.data() + rb0);
+          const int64_t rk = (layout == 1) ? pos : pos / 3;
+          if (rk > 255 || rk >= deg) { bad = true; break; }
+          rank[c * 100 + a * 10 + b] = static_cast<uint8_t>(rk);
+          // verify all 9 component combinations land on the expected columns
+          for (int i = 0; i < 3 && !bad; ++i) {
+            const int32_t row = cd[c * 30 + i * 10 + a];
+            if (csr.row_ptr[row + 1] - csr.row_ptr[row] != len) { bad = true; break; }
+            for (int j = 0; j < 3; ++j) {
+              const int64_t slot = csr.row_ptr[row] + ((layout == 1) ? (rk + j * deg) : (rk * 3 + j));
+              if (csr.col[slot] != cd[c * 30 + j * 10 + b]) { bad = true; break; }
+            }
           }
+          if (bad) break;
         }
       }
-    }
+    if (bad) failed = true;
+  });
+  const bool bad = failed;
   if (bad) throw std::runtime_error("vector P2 sparsity is not node-blocked (unsupported dofmap)");
   return layout == 1;
 }
@@ -839,13 +875,15 @@ void build_vp2_block_plan(int64_t nc, const int32_t* cells, const int32_t* cd, c
   for (int64_t n = 0; n < ndofs; ++n)
     if (isnode[n]) ids.push_back(static_cast<int32_t>(n));
   auto vals_of = [&](int32_t n) { return 3 * (csr.row_ptr[n + 1] - csr.row_ptr[n]); };
-  std::vector<std::pair<int64_t, int64_t>> leaves;
-  std::function<void(int64_t, int64_t)> rcb = [&](int64_t lo, int64_t hi) {
+  using Leaves = std::vector<std::pair<int64_t, int64_t>>;
+  // upper levels split on threads; leaves appended depth-first as in the serial recursion
+  std::function<void(int64_t, int64_t, int, Leaves&)> rcb = [&](int64_t lo, int64_t hi, int depth,
+                                                               Leaves& out) {
     int64_t tot = 0;
     for (int64_t i = lo; i < hi; ++i) tot += vals_of(ids[i]);
     if (tot <= max_block_vals || hi - lo == 1) {
       if (tot > max_block_vals) throw std::runtime_error("vector P2 node rows exceed the LDS block");
-      leaves.emplace_back(lo, hi);
+      out.emplace_back(lo, hi);
       return;
     }
     float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
@@ -865,10 +903,34 @@ void build_vp2_block_plan(int64_t nc, const int32_t* cells, const int32_t* cd, c
                        const float pa = nx[3 * a + ax], pb = nx[3 * b + ax];
                        return pa < pb || (pa == pb && a < b);
                      });
-    rcb(lo, mid);
-    rcb(mid, hi);
+    if (depth < 4 && hi - lo > 65536) {
+      Leaves left, right;
+      std::exception_ptr err;
+      std::thread th([&]() {
+        try {
+          rcb(lo, mid, depth + 1, left);
+        } catch (...) {
+          err = std::current_exception();
+        }
+      });
+      std::exception_ptr err2;
+      try {
+        rcb(mid, hi, depth + 1, right);
+      } catch (...) {
+        err2 = std::current_exception();
+      }
+      th.join();
+      if (err) std::rethrow_exception(err);
+      if (err2) std::rethrow_exception(err2);
+      out.insert(out.end(), left.begin(), left.end());
+      out.insert(out.end(), right.begin(), right.end());
+    } else {
+      rcb(lo, mid, depth + 1, out);
+      rcb(mid, hi, depth + 1, out);
+    }
   };
-  rcb(0, static_cast<int64_t>(ids.size()));
+  Leaves leaves;
+  rcb(0, static_cast<int64_t>(ids.size()), 0, leaves);
   const int32_t nb = static_cast<int32_t>(leaves.size());
   P.num_blocks = nb;
   std::vector<int32_t> owner(ndofs, -1), own_index(ndofs, -1);
