@@ -243,6 +243,7 @@ def main():
         sec["p1_spmv"] = {"ms": mv_ms, "hbm_achieved_GBs": mv_bytes / (mv_ms * 1e-3) / 1e9,
                           "hbm_frac": mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                           "algorithmic_bytes": mv_bytes,
+                          "traffic": load_traffic(f"p1_spmv_{args.n}"),
                           "workload": f"y = A x on the assembled P1 {args.n}^3 matrix"}
         log("spmv timed")
         if args.p2_n > 0:
@@ -275,7 +276,8 @@ def main():
                 "hbm_achieved_GBs": alg2 / (ms2 * 1e-3) / 1e9,
                 "hbm_frac": alg2 / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "fp64_TFLOPs_at_39900_per_cell": nc2 * 39900 / (ms2 * 1e-3) / 1e12,
-                "algorithmic_bytes": alg2, "first_assembly_s": t2_first}
+                "algorithmic_bytes": alg2, "traffic": load_traffic(f"vp2_{sc2}_{args.p2_n}"),
+                "first_assembly_s": t2_first}
             xs2 = torch.rand(V2.dim(), dtype=torch.float64, device="cuda")
             ys2 = torch.empty_like(xs2)
             w_mv2, ev_mv2 = timed_steps(lambda: A2.mult(xs2, ys2), max(10, args.steps // 5), 2)
@@ -285,6 +287,7 @@ def main():
                                "hbm_achieved_GBs": mv2_bytes / (mv2_ms * 1e-3) / 1e9,
                                "hbm_frac": mv2_bytes / (mv2_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                "algorithmic_bytes": mv2_bytes,
+                               "traffic": load_traffic(f"vp2_spmv_{args.p2_n}"),
                                "workload": f"y = A x on the assembled vector-P2 {args.p2_n}^3 matrix"}
         if args.batch > 0:
             sec["batch"] = bench_batch(args.batch, log)
