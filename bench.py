@@ -246,6 +246,30 @@ def main():
                           "traffic": load_traffic(f"p1_spmv_{args.n}"),
                           "workload": f"y = A x on the assembled P1 {args.n}^3 matrix"}
         log("spmv timed")
+        # the same workload with DOLFIN's default numbering (reverse Cuthill-McKee nodes)
+        t = time.perf_counter()
+        Vr = dfa.FunctionSpace(mesh, "P", 1, reorder=True)
+        t_rcm = time.perf_counter() - t
+        new_of = np.empty(V.dim(), np.int64)  # new dof of each vertex
+        new_of[mesh.cells().ravel()] = Vr.cell_dofs().ravel()
+        kr = np.empty_like(kappa)
+        fr = np.empty_like(f)
+        kr[new_of] = kappa
+        fr[new_of] = f
+        asm_r = dfa.SystemAssembler(dfa.poisson_form(Vr, workloads.to_function(Vr, kr)),
+                                    dfa.load_form(Vr, workloads.to_function(Vr, fr)), scatter=scatter)
+        Ar, br = dfa.Matrix(), dfa.Vector()
+        t = time.perf_counter()
+        asm_r.assemble(Ar, br)
+        torch.cuda.synchronize()
+        t_rfirst = time.perf_counter() - t
+        w_r, ev_r = timed_steps(lambda: asm_r.assemble(Ar, br), max(20, args.steps), 5)
+        sec["p1_rcm"] = {"ms": float(np.mean(ev_r)), "first_assembly_s": t_rfirst,
+                         "reordering_s": t_rcm,
+                         "workload": f"P1 {args.n}^3 A+b with reverse Cuthill-McKee dof numbering "
+                                     f"(reorder_dofs_serial)"}
+        del Ar, br, asm_r
+        log("rcm-numbered assembly timed")
         if args.p2_n > 0:
             log(f"vector P2 {args.p2_n}^3 setup")
             del A, b

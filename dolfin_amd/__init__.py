@@ -7,7 +7,8 @@ torch provides device memory and streams.
 """
 from . import _lib
 from .mesh import Mesh, UnitCubeMesh, UnitIntervalMesh, UnitSquareMesh, jitter_interior
-from .function import Function, FunctionSpace, VectorFunctionSpace
+from .function import (Function, FunctionSpace, VectorFunctionSpace, node_reordering,
+                       parameters)
 from .la import Matrix, Vector, solve
 from .fem import (Assembler, DirichletBC, FacetForm, Form, FormSum, SystemAssembler, assemble,
                   assemble_system, body_force_form, elasticity_form, engine_for, exterior_facets,
@@ -20,7 +21,7 @@ SCATTER_BLOCKED = _lib.SCATTER_BLOCKED
 
 __all__ = [
     "Mesh", "UnitCubeMesh", "UnitIntervalMesh", "UnitSquareMesh", "jitter_interior",
-    "Function", "FunctionSpace", "VectorFunctionSpace", "Matrix", "Vector", "solve",
+    "Function", "FunctionSpace", "VectorFunctionSpace", "node_reordering", "parameters", "Matrix", "Vector", "solve",
     "Assembler", "DirichletBC", "Form", "SystemAssembler", "assemble", "assemble_system",
     "body_force_form", "elasticity_form", "engine_for", "load_form", "mass_form", "poisson_form",
     "FacetForm", "FormSum", "exterior_facets", "facet_load_form", "facet_mass_form",

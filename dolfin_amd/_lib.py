@@ -34,7 +34,8 @@ EXPORTS = ["dfa_last_error", "dfa_version", "dfa_assembler_create", "dfa_assembl
            "dfa_csr_dims", "dfa_csr_structure", "dfa_assemble_matrix", "dfa_assemble_vector",
            "dfa_assemble_system", "dfa_adjoint", "dfa_plan_info", "dfa_set_plan_defaults",
            "dfa_batch_create", "dfa_batch_launch", "dfa_batch_info", "dfa_batch_destroy",
-           "dfa_exterior_facets", "dfa_assemble_facets", "dfa_csr_spmv", "dfa_csr_diagonal"]
+           "dfa_exterior_facets", "dfa_assemble_facets", "dfa_csr_spmv", "dfa_csr_diagonal",
+           "dfa_node_reordering", "dfa_graph_reordering"]
 
 _lib = None
 
@@ -76,6 +77,8 @@ def load():
     L.dfa_exterior_facets.argtypes = [_vp, ctypes.POINTER(_i64), _vp, _vp]
     L.dfa_csr_spmv.argtypes = [_vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp]
     L.dfa_csr_diagonal.argtypes = [_vp, _vp, _vp, _vp]
+    L.dfa_node_reordering.argtypes = [_i64, _i32, _vp, _i64, _vp]
+    L.dfa_graph_reordering.argtypes = [_i64, _vp, _vp, _vp]
     L.dfa_assemble_facets.argtypes = [_vp, _i32, _i32, _vp, ctypes.c_double, _vp, _vp, _vp, _vp,
                                       _vp, _vp, _vp]
     for name in EXPORTS:

@@ -523,6 +523,49 @@ void ensure_facets(dfa_assembler h, hipStream_t s) {
 }
 }  // namespace
 
+int dfa_node_reordering(int64_t num_cells, int32_t nodes_per_cell, const int32_t* cell_nodes,
+                        int64_t num_nodes, int32_t* old_to_new) {
+  return guarded([&]() -> int {
+    if (num_cells < 0 || nodes_per_cell <= 0 || num_nodes < 0 || (num_cells > 0 && !cell_nodes) ||
+        (num_nodes > 0 && !old_to_new))
+      return fail("invalid arguments", DFA_ERR_ARG);
+    if (num_nodes > INT32_MAX) return fail("sizes exceed int32", DFA_ERR_ARG);
+    // node graph: the sparsity of cell_nodes x cell_nodes without the diagonal
+    dfa::HostCSR g;
+    dfa::build_csr(num_cells, nodes_per_cell, cell_nodes, num_nodes, false, g);
+    std::vector<int32_t> ptr(static_cast<size_t>(num_nodes) + 1, 0), adj;
+    adj.reserve(g.col.size());
+    for (int64_t v = 0; v < num_nodes; ++v) {
+      for (int32_t k = g.row_ptr[v]; k < g.row_ptr[v + 1]; ++k)
+        if (g.col[k] != v) adj.push_back(g.col[k]);
+      ptr[v + 1] = static_cast<int32_t>(adj.size());
+    }
+    dfa::rcm_ordering(num_nodes, ptr.data(), adj.data(), old_to_new);
+    return DFA_OK;
+  });
+}
+
+int dfa_graph_reordering(int64_t num_nodes, const int32_t* graph_ptr, const int32_t* graph_adj,
+                         int32_t* old_to_new) {
+  return guarded([&]() -> int {
+    if (num_nodes < 0 || (num_nodes > 0 && (!graph_ptr || !old_to_new)))
+      return fail("invalid arguments", DFA_ERR_ARG);
+    if (num_nodes > INT32_MAX) return fail("sizes exceed int32", DFA_ERR_ARG);
+    if (num_nodes == 0) return DFA_OK;
+    if (graph_ptr[0] != 0) return fail("graph_ptr[0] must be 0", DFA_ERR_ARG);
+    for (int64_t v = 0; v < num_nodes; ++v) {
+      if (graph_ptr[v + 1] < graph_ptr[v]) return fail("graph_ptr must be non-decreasing", DFA_ERR_ARG);
+      for (int32_t k = graph_ptr[v]; k < graph_ptr[v + 1]; ++k) {
+        const int32_t u = graph_adj[k];
+        if (u < 0 || u >= num_nodes || u == v || (k > graph_ptr[v] && graph_adj[k - 1] >= u))
+          return fail("adjacency lists must be sorted, in range and without self loops", DFA_ERR_ARG);
+      }
+    }
+    dfa::rcm_ordering(num_nodes, graph_ptr, graph_adj, old_to_new);
+    return DFA_OK;
+  });
+}
+
 int dfa_exterior_facets(dfa_assembler h, int64_t* num_facets, int32_t* facet_cell,
                         int32_t* facet_local) {
   return guarded([&]() -> int {

@@ -99,6 +99,16 @@ void build_p1_chain_plan(const P1BlockPlan& plan, P1ChainPlan& chains);
 void build_exterior_facets(int32_t gdim, int64_t num_cells, const int32_t* cells,
                            std::vector<int32_t>& facet_cell, std::vector<int8_t>& facet_local);
 
+// Reverse Cuthill-McKee numbering of a graph given as sorted adjacency lists (no self
+// loops): old_to_new[v] = new index of v. Components in order of their lowest vertex, each
+// started at a pseudo-peripheral vertex (George & Liu), breadth-first with each vertex's
+// unvisited neighbours appended in ascending degree, the whole sequence reversed -- the
+// ordering BoostGraphOrdering::compute_cuthill_mckee(graph, true) returns.
+void rcm_ordering(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int32_t* old_to_new);
+// Same from a start vertex given by the caller (one component).
+void cuthill_mckee_from(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int32_t start,
+                        std::vector<int32_t>& order);
+
 // Row blocks of the streamed CSR SpMV: maximal runs of consecutive rows holding at most
 // `budget` nonzeros and at most `max_rows` rows; a row longer than `budget` is a block of its
 // own. Returns the first row of each block followed by the row count.

@@ -498,6 +498,103 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   });
 }
 
+namespace {
+
+// Breadth-first sweep from s over the white vertices of its component; returns the number
+// of levels - 1 (the eccentricity of s) and in `last_min` the first vertex of smallest
+// degree in the last level (the rcm_queue bookkeeping of the Boost implementation).
+int bfs_levels(const int32_t* ap, const int32_t* adj, int32_t s, std::vector<int32_t>& mark,
+               int32_t stamp, std::vector<int32_t>& queue, int32_t& last_min) {
+  auto deg = [&](int32_t v) { return ap[v + 1] - ap[v]; };
+  queue.clear();
+  queue.push_back(s);
+  mark[s] = stamp;
+  size_t head = 0;
+  int ecc = -1;
+  while (head < queue.size()) {
+    const size_t level_end = queue.size();
+    last_min = queue[head];
+    for (; head < level_end; ++head) {
+      const int32_t u = queue[head];
+      if (deg(u) < deg(last_min)) last_min = u;
+      for (int32_t k = ap[u]; k < ap[u + 1]; ++k)
+        if (mark[adj[k]] != stamp) {
+          mark[adj[k]] = stamp;
+          queue.push_back(adj[k]);
+        }
+    }
+    ++ecc;
+  }
+  return ecc;
+}
+
+}  // namespace
+
+void cuthill_mckee_from(int64_t n, const int32_t* ap, const int32_t* adj, int32_t start,
+                        std::vector<int32_t>& order) {
+  std::vector<char> seen(static_cast<size_t>(n), 0);
+  auto deg = [&](int32_t v) { return ap[v + 1] - ap[v]; };
+  const size_t first = order.size();
+  order.push_back(start);
+  seen[start] = 1;
+  for (size_t head = first; head < order.size(); ++head) {
+    const int32_t u = order[head];
+    const size_t begin = order.size();
+    for (int32_t k = ap[u]; k < ap[u + 1]; ++k)
+      if (!seen[adj[k]]) {
+        seen[adj[k]] = 1;
+        order.push_back(adj[k]);
+      }
+    // the same comparison sort the Boost visitor applies to the newly queued vertices
+    std::sort(order.begin() + static_cast<std::ptrdiff_t>(begin), order.end(),
+              [&](int32_t a, int32_t b) { return deg(a) < deg(b); });
+  }
+}
+
+void rcm_ordering(int64_t n, const int32_t* ap, const int32_t* adj, int32_t* old_to_new) {
+  if (n <= 0) return;
+  if (ap[n] == 0) {  // no edges: the identity (BoostGraphOrdering.cpp:50-55)
+    for (int64_t v = 0; v < n; ++v) old_to_new[v] = static_cast<int32_t>(v);
+    return;
+  }
+  std::vector<int32_t> mark(static_cast<size_t>(n), -1), queue;
+  std::vector<int32_t> comp_of(static_cast<size_t>(n), -1);
+  std::vector<int32_t> order;
+  order.reserve(static_cast<size_t>(n));
+  std::vector<char> done(static_cast<size_t>(n), 0);
+  auto deg = [&](int32_t v) { return ap[v + 1] - ap[v]; };
+  int32_t stamp = 0;
+  for (int64_t r0 = 0; r0 < n; ++r0) {
+    if (done[r0]) continue;
+    // pseudo-peripheral start of r0's component
+    int32_t r = static_cast<int32_t>(r0), x = 0, y = 0;
+    int ecc_r = bfs_levels(ap, adj, r, mark, stamp++, queue, x);
+    int ecc_x = bfs_levels(ap, adj, x, mark, stamp++, queue, y);
+    while (ecc_x > ecc_r) {
+      r = x;
+      ecc_r = ecc_x;
+      x = y;
+      ecc_x = bfs_levels(ap, adj, x, mark, stamp++, queue, y);
+    }
+    // Cuthill-McKee sweep of the component from x
+    const size_t first = order.size();
+    order.push_back(x);
+    done[x] = 1;
+    for (size_t head = first; head < order.size(); ++head) {
+      const int32_t u = order[head];
+      const size_t begin = order.size();
+      for (int32_t k = ap[u]; k < ap[u + 1]; ++k)
+        if (!done[adj[k]]) {
+          done[adj[k]] = 1;
+          order.push_back(adj[k]);
+        }
+      std::sort(order.begin() + static_cast<std::ptrdiff_t>(begin), order.end(),
+                [&](int32_t a, int32_t b) { return deg(a) < deg(b); });
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) old_to_new[order[i]] = static_cast<int32_t>(n - 1 - i);
+}
+
 std::vector<int32_t> build_spmv_blocks(const std::vector<int32_t>& row_ptr, int32_t budget,
                                        int32_t max_rows) {
   const int64_t n = static_cast<int64_t>(row_ptr.size()) - 1;

@@ -17,10 +17,35 @@ import torch
 from . import _lib
 
 
+# Global parameters read at FunctionSpace construction (the dof-ordering subset of
+# dolfin::parameters, dolfin/parameter/GlobalParameters.h:76-88). reorder_dofs_serial
+# defaults to False here (DOLFIN: True) so that P1 dofs are vertex indices.
+parameters = {"reorder_dofs_serial": False}
+
+
+def node_reordering(cell_nodes, num_nodes):
+    """old_to_new of DOLFIN's reverse Cuthill-McKee node renumbering (DofMapBuilder::
+    compute_node_reordering, DofMapBuilder.cpp:1473-1569, Boost ordering library)."""
+    cn = np.ascontiguousarray(cell_nodes, dtype=np.int32)
+    out = np.empty(int(num_nodes), dtype=np.int32)
+    _lib.check(_lib.load().dfa_node_reordering(cn.shape[0], cn.shape[1], _lib.ptr(cn),
+                                               int(num_nodes), _lib.ptr(out)))
+    return out
+
+
 class FunctionSpace:
-    def __init__(self, mesh, family="P", degree=1, vector=False, node_permutation=None):
+    def __init__(self, mesh, family="P", degree=1, vector=False, node_permutation=None,
+                 reorder=None):
+        """node_permutation: new index of each node (vertex for P1; vertex, then nv + edge
+        for vector P2), or reorder=True for DOLFIN's reverse Cuthill-McKee numbering
+        (default: parameters["reorder_dofs_serial"]). A permuted vector space numbers its
+        dofs 3 * node + component, as DOLFIN does after reordering (DofMapBuilder.cpp:1689)."""
         if family not in ("P", "Lagrange", "CG"):
             raise ValueError("only continuous Lagrange spaces are supported")
+        if reorder is None:
+            reorder = parameters["reorder_dofs_serial"]
+        if reorder and node_permutation is not None:
+            raise ValueError("give either node_permutation or reorder, not both")
         self._mesh = mesh
         self.degree = degree
         self.vector = vector
@@ -28,6 +53,8 @@ class FunctionSpace:
         if degree == 1 and not vector:
             self.element = _lib.ELEMENT_P1
             nv = mesh.num_vertices()
+            if reorder:
+                node_permutation = node_reordering(mesh.cells(), nv)
             if node_permutation is None:
                 cd = mesh.cells().copy()
             else:
@@ -46,6 +73,8 @@ class FunctionSpace:
             nv = mesh.num_vertices()
             nn = nv + ne
             nodes = np.concatenate([mesh.cells(), nv + ce], axis=1).astype(np.int64)  # [nc, 10]
+            if reorder:
+                node_permutation = node_reordering(nodes, nn)
             if node_permutation is None:
                 cd = np.concatenate([c * nn + nodes for c in range(3)], axis=1)
             else:
@@ -83,8 +112,9 @@ class FunctionSpace:
         return f"FunctionSpace({kind}, dim={self._dim})"
 
 
-def VectorFunctionSpace(mesh, family="P", degree=2, node_permutation=None):
-    return FunctionSpace(mesh, family, degree, vector=True, node_permutation=node_permutation)
+def VectorFunctionSpace(mesh, family="P", degree=2, node_permutation=None, reorder=None):
+    return FunctionSpace(mesh, family, degree, vector=True, node_permutation=node_permutation,
+                         reorder=reorder)
 
 
 class Function:

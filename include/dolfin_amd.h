@@ -172,6 +172,25 @@ int dfa_plan_info(dfa_assembler h, int32_t scatter, int64_t* plan_bytes, int64_t
                   int64_t* cell_appearances, int32_t* lds_bytes);
 
 /*
+ * Reverse Cuthill-McKee renumbering of the nodes of a mesh's cells (two nodes are adjacent
+ * when they share a cell): old_to_new[node] receives the node's new index. This is the
+ * numbering DOLFIN gives the dofmap nodes with parameters["reorder_dofs_serial"] = true and
+ * the Boost ordering library: DofMapBuilder::compute_node_reordering (dolfin/fem/
+ * DofMapBuilder.cpp:1473-1569) builds this graph and BoostGraphOrdering::compute_cuthill_mckee
+ * (dolfin/graph/BoostGraphOrdering.cpp:34-75) orders it. Host computation, no device use.
+ */
+int dfa_node_reordering(int64_t num_cells, int32_t nodes_per_cell, const int32_t* cell_nodes,
+                        int64_t num_nodes, int32_t* old_to_new);
+
+/*
+ * The same ordering of a graph given as sorted adjacency lists without self loops
+ * (graph_ptr[num_nodes + 1], graph_adj): BoostGraphOrdering::compute_cuthill_mckee(graph,
+ * true) for a dolfin::Graph (BoostGraphOrdering.cpp:34-75). Host computation.
+ */
+int dfa_graph_reordering(int64_t num_nodes, const int32_t* graph_ptr, const int32_t* graph_adj,
+                         int32_t* old_to_new);
+
+/*
  * Exterior facets of a P1 handle (Facet::exterior: one incident cell), sorted by cell and
  * local facet index (local facet k lies opposite local vertex k). Writes their number and,
  * when the arrays are non-NULL, each facet's cell and local index -- the order d_facet_mask

@@ -365,3 +365,94 @@ def boundary_vertices(x):
     for P1, topological method)."""
     x = np.asarray(x)
     return np.any((np.abs(x) < 1e-14) | (np.abs(x - 1.0) < 1e-14), axis=1)
+
+
+# ---- node reordering (pure Python: small graphs only) -----------------------------------
+def node_graph(cell_nodes, num_nodes):
+    """Sorted adjacency lists of the node graph: two nodes are adjacent when they share a cell
+    (the graph DofMapBuilder::compute_node_reordering builds, DofMapBuilder.cpp:1530-1558)."""
+    nbr = [set() for _ in range(num_nodes)]
+    for c in np.asarray(cell_nodes):
+        for a in c:
+            for b in c:
+                if a != b:
+                    nbr[int(a)].add(int(b))
+    return [sorted(s) for s in nbr]
+
+
+def _bfs_levels(adj, s):
+    """Eccentricity of s and the first smallest-degree vertex of its last BFS level (the
+    rcm_queue bookkeeping Boost's pseudo_peripheral_pair reads)."""
+    seen = {s}
+    level = [s]
+    ecc = -1
+    last_min = s
+    while level:
+        last_min = level[0]
+        nxt = []
+        for u in level:
+            if len(adj[u]) < len(adj[last_min]):
+                last_min = u
+            for v in adj[u]:
+                if v not in seen:
+                    seen.add(v)
+                    nxt.append(v)
+        ecc += 1
+        level = nxt
+    return ecc, last_min
+
+
+def _stable_by_degree(adj, seq):
+    # Boost sorts each vertex's newly queued neighbours with std::sort by degree; for the
+    # short lists of these small test graphs (at most 16) libstdc++ uses insertion sort,
+    # which keeps equal degrees in queue order, as this stable sort does.
+    return sorted(seq, key=lambda v: len(adj[v]))
+
+
+def cuthill_mckee(adj, start):
+    """Cuthill-McKee order of start's component: breadth first, each vertex's unvisited
+    neighbours (ascending index) queued in ascending degree."""
+    order = [start]
+    seen = {start}
+    head = 0
+    while head < len(order):
+        u = order[head]
+        head += 1
+        new = [v for v in adj[u] if v not in seen]
+        seen.update(new)
+        order.extend(_stable_by_degree(adj, new))
+    return order
+
+
+def reverse_cuthill_mckee(adj):
+    """old_to_new of the reverse Cuthill-McKee numbering as BoostGraphOrdering::
+    compute_cuthill_mckee(graph, true) returns it (BoostGraphOrdering.cpp:34-75, with Boost's
+    cuthill_mckee_ordering): components in order of their lowest vertex, each started at the
+    pseudo-peripheral vertex of George and Liu (find_starting_node), the sequence reversed."""
+    n = len(adj)
+    if sum(len(a) for a in adj) == 0:
+        return np.arange(n, dtype=np.int32)
+    done = set()
+    order = []
+    for r0 in range(n):
+        if r0 in done:
+            continue
+        r = r0
+        ecc_r, x = _bfs_levels(adj, r)
+        ecc_x, y = _bfs_levels(adj, x)
+        while ecc_x > ecc_r:
+            r, ecc_r, x = x, ecc_x, y
+            ecc_x, y = _bfs_levels(adj, x)
+        comp = cuthill_mckee(adj, x)
+        done.update(comp)
+        order.extend(comp)
+    old_to_new = np.empty(n, dtype=np.int32)
+    for i, v in enumerate(order):
+        old_to_new[v] = n - 1 - i
+    return old_to_new
+
+
+def bandwidth(adj, old_to_new=None):
+    """max |new(u) - new(v)| over the edges."""
+    p = np.arange(len(adj)) if old_to_new is None else np.asarray(old_to_new)
+    return max((abs(int(p[u]) - int(p[v])) for u in range(len(adj)) for v in adj[u]), default=0)

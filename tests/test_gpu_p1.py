@@ -152,6 +152,31 @@ def test_keep_diagonal_and_permuted_dofs(oracle):
         assert rel(vals, ref) < TOL
 
 
+@pytest.mark.parametrize("scatter", SCATTERS)
+def test_rcm_reordered_system_vs_oracle(oracle, scatter):
+    # DOLFIN's default numbering (reorder_dofs_serial): reverse Cuthill-McKee node order
+    rng = np.random.default_rng(12)
+    mesh = dfa.UnitCubeMesh(8, 7, 6)
+    dfa.jitter_interior(mesh, 0.15 / 8, rng)
+    V = dfa.FunctionSpace(mesh, "P", 1, reorder=True)
+    kappa, f = rng.uniform(0.5, 2, V.dim()), rng.uniform(-1, 1, V.dim())
+    bc = dfa.DirichletBC(V, 0.0)
+    g = rng.uniform(-1, 1, V.dim())
+    bc = dfa.DirichletBC(V, g[bc.dofs], bc.dofs)
+    A, b = dfa.assemble_system(dfa.poisson_form(V, workloads.to_function(V, kappa)),
+                               dfa.load_form(V, workloads.to_function(V, f)), bc, scatter=scatter)
+    rp, col, vals = _check_structure(oracle, A, V)
+    mask = np.zeros(V.dim(), np.uint8)
+    mask[bc.dofs] = 1
+    gv = np.zeros(V.dim())
+    gv[bc.dofs] = g[bc.dofs]
+    ra, rb = oracle.assemble_system(oracle.P1_POISSON, oracle.P1_LOAD, mesh.coordinates(),
+                                    mesh.cells(), V.cell_dofs(), rp, col, mask, gv,
+                                    coeff_dofs=V.cell_dofs(), a_c0=kappa, L_c0=f)
+    assert rel(vals, ra) < TOL
+    assert rel(b.get_local(), rb) < TOL
+
+
 def test_blocked_equals_atomic_128_properties():
     """Full-size P1 128^3 (12.58M cells): size-independent properties -- blocked and atomic
     scatters agree, Poisson rows sum to zero (K 1 = 0), K is symmetric, sum(b) = int f."""

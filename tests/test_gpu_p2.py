@@ -49,6 +49,31 @@ def test_vp2_vs_oracle(oracle, n, block, scatter):
 
 
 @pytest.mark.parametrize("scatter", [dfa.SCATTER_ATOMIC, dfa.SCATTER_BLOCKED])
+def test_vp2_rcm_reordered_vs_oracle(oracle, scatter):
+    # reverse Cuthill-McKee node order, dofs 3 * node + component (DOLFIN after reordering)
+    m = dfa.UnitCubeMesh(4, 4, 4)
+    ce, ne = m.cell_edges()
+    nodes = np.concatenate([m.cells(), m.num_vertices() + ce], axis=1)
+    perm = dfa.node_reordering(nodes, m.num_vertices() + ne)
+    mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(4, seed=9, node_permutation=perm)
+    np.testing.assert_array_equal(V.cell_dofs(), dfa.VectorFunctionSpace(mesh, "P", 2, reorder=True).cell_dofs())
+    a, L = _forms(V, mu, lmbda, f3)
+    A = dfa.assemble(a, scatter=scatter)
+    b = dfa.assemble(L, scatter=scatter)
+    rp, col, vals = A.data()
+    orp, ocol = oracle.sparsity(V.cell_dofs(), V.dim())
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(col, ocol)
+    kd = V.coefficient_space().cell_dofs()
+    ref = oracle.assemble_matrix(oracle.VP2_ELASTICITY, mesh.coordinates(), mesh.cells(),
+                                 V.cell_dofs(), rp, col, coeff_dofs=kd, c0=mu, c1=lmbda)
+    bref = oracle.assemble_vector(oracle.VP2_BODY_FORCE, mesh.coordinates(), mesh.cells(),
+                                  V.cell_dofs(), V.dim(), f3=f3)
+    assert rel(vals, ref) < TOL
+    assert rel(b.get_local(), bref) < TOL
+
+
+@pytest.mark.parametrize("scatter", [dfa.SCATTER_ATOMIC, dfa.SCATTER_BLOCKED])
 def test_vp2_system_bc_vs_oracle(oracle, scatter):
     mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(3, seed=8)
     a, L = _forms(V, mu, lmbda, f3)
