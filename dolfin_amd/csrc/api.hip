@@ -3,6 +3,9 @@
 // (dolfin/fem/Assembler.cpp:52-103, SystemAssembler.cpp:138-301) for one (mesh, space).
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cstdio>
+
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -102,6 +105,26 @@ int dfa_set_plan_defaults(int32_t p1_block_rows, int32_t p1_threads, int32_t p2_
   return DFA_OK;
 }
 
+namespace {
+// Phase times of dfa_assembler_create on stderr when DFA_TIMINGS=1 (the role of DOLFIN's
+// "Build sparsity" / "Init tensor" timers, dolfin/common/Timer.h).
+struct PhaseTimer {
+  bool on;
+  std::chrono::steady_clock::time_point t;
+  PhaseTimer() {
+    const char* e = std::getenv("DFA_TIMINGS");
+    on = e && e[0] == '1';
+    t = std::chrono::steady_clock::now();
+  }
+  void mark(const char* what) {
+    if (!on) return;
+    const auto n = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[dfa] %-28s %8.3f s\n", what, std::chrono::duration<double>(n - t).count());
+    t = n;
+  }
+};
+}  // namespace
+
 int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t nc,
                          const int32_t* h_cells, int32_t element, int32_t nd,
                          const int32_t* h_cell_dofs, int64_t ndofs, const int32_t* h_coeff_dofs,
@@ -126,6 +149,7 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
       if (h_cells[i] < 0 || h_cells[i] >= nv || h_coeff_dofs[i] < 0 || h_coeff_dofs[i] >= nkdofs)
         return fail("cell vertex or coefficient dof out of range", DFA_ERR_ARG);
     hipStream_t s = static_cast<hipStream_t>(stream);
+    PhaseTimer timer;
     auto* h = new dfa_assembler_s();
     try {
       h->gdim = gdim;
@@ -136,20 +160,37 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
       h->nc = nc;
       h->ndofs = ndofs;
       h->nkdofs = nkdofs;
-      dfa::HostCSR csr;
-      dfa::build_csr(nc, nd, h_cell_dofs, ndofs, keep_diagonal != 0, csr);
-      h->nnz = static_cast<int64_t>(csr.col.size());
       {
         std::vector<uint8_t> seen(ndofs, 0);
-        for (int64_t i = 0; i < nc * nd; ++i) seen[h_cell_dofs[i]] = 1;
+        for (int64_t i = 0; i < nc * nd; ++i) {
+          const int32_t d = h_cell_dofs[i];
+          if (d < 0 || d >= ndofs) throw std::runtime_error("cell_dofs entry out of range");
+          seen[d] = 1;
+        }
         for (int64_t r = 0; r < ndofs; ++r)
           if (!seen[r]) h->rows_covered = false;
       }
       h->d_cells = h->base.upload(h_cells, static_cast<size_t>(nc * nvc), s);
       h->d_cdofs = h->base.upload(h_cell_dofs, static_cast<size_t>(nc * nd), s);
       h->d_kdofs = h->base.upload(h_coeff_dofs, static_cast<size_t>(nc * nvc), s);
-      h->d_row_ptr = h->base.upload(csr.row_ptr, s);
-      h->d_col = h->base.upload(csr.col, s);
+      timer.mark("check + upload mesh");
+      // CSR structure: built on the device (sort of the cell pairs) unless device memory is
+      // short or DFA_HOST_CSR=1, then on the host; both give the same arrays
+      dfa::HostCSR csr;
+      const char* host_csr = std::getenv("DFA_HOST_CSR");
+      int32_t* drp = nullptr;
+      int32_t* dcol = nullptr;
+      if (!(host_csr && host_csr[0] == '1') &&
+          dfa::build_csr_device(nc, nd, h->d_cdofs, ndofs, keep_diagonal != 0, csr, &drp, &dcol, s)) {
+        h->d_row_ptr = h->base.adopt(drp, static_cast<int64_t>(csr.row_ptr.size() * sizeof(int32_t)));
+        h->d_col = h->base.adopt(dcol, static_cast<int64_t>(csr.col.size() * sizeof(int32_t)));
+      } else {
+        dfa::build_csr(nc, nd, h_cell_dofs, ndofs, keep_diagonal != 0, csr);
+        h->d_row_ptr = h->base.upload(csr.row_ptr, s);
+        h->d_col = h->base.upload(csr.col, s);
+      }
+      h->nnz = static_cast<int64_t>(csr.col.size());
+      timer.mark("CSR structure");
       {
         const std::vector<int32_t> sb = dfa::build_spmv_blocks(csr.row_ptr, dfa::kSpmvBudget,
                                                                dfa::kSpmvMaxRows);
@@ -167,6 +208,7 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
             dfa::P1BlockPlan P;
             dfa::build_p1_block_plan(gdim, nv, nc, h_cells, h_cell_dofs, h_coeff_dofs, h_x, csr,
                                      default_block_rows(gdim), P);
+            timer.mark("P1 block plan");
             auto& t = h->btopo;
             t.num_blocks = P.num_blocks;
             t.max_nodes = P.max_nodes;
@@ -193,6 +235,7 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
             const char* ce = std::getenv("DFA_P1_CHAINS");
             if (gdim == 3 && !(ce && std::atoi(ce) == 0)) {
               dfa::build_p1_chain_plan(P, C);
+              timer.mark("P1 edge-chain plan");
               t.max_chains = C.max_chains;
               t.chain_threads = default_chain_threads();
               t.chain_ptr = h->plan.upload(C.chain_ptr, s);
@@ -222,12 +265,14 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
       } else {
         std::vector<uint8_t> rank;
         h->vp2_component_major = dfa::build_vp2_ranks(nc, h_cell_dofs, ndofs, csr, rank);
+        timer.mark("vector-P2 ranks");
         h->d_vp2_rank = h->base.upload(rank, s);
         if (h_x) {
           try {
             dfa::VP2BlockPlan P;
             dfa::build_vp2_block_plan(nc, h_cells, h_cell_dofs, h_coeff_dofs, h_x, csr, rank,
                                       h->vp2_component_major, default_p2_block_vals(), P);
+            timer.mark("vector-P2 block plan");
             auto& t = h->vtopo;
             t.num_blocks = P.num_blocks;
             t.max_verts = P.max_verts;
@@ -259,6 +304,7 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
         }
       }
       hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+      timer.mark("uploads");
     } catch (...) {
       h->plan.release();
       h->base.release();

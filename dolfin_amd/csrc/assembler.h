@@ -38,12 +38,27 @@ struct DeviceArena {
     if (n) hip_check(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync");
     return static_cast<T*>(d);
   }
+  // take ownership of a device allocation made elsewhere
+  template <class T>
+  T* adopt(T* d, int64_t nbytes) {
+    ptrs.push_back(d);
+    bytes += nbytes;
+    return d;
+  }
   void release() {
     for (void* p : ptrs) (void)hipFree(p);
     ptrs.clear();
     bytes = 0;
   }
 };
+
+// The CSR structure of build_csr, built on the device from device cell_dofs (csr_build.hip):
+// the arrays stay on the device (returned through d_row_ptr / d_col; the caller frees them)
+// and are also copied into `out`. Returns false, with nothing allocated, when device memory
+// is short.
+bool build_csr_device(int64_t num_cells, int32_t nd, const int32_t* d_cell_dofs,
+                      int64_t num_rows, bool keep_diagonal, HostCSR& out, int32_t** d_row_ptr,
+                      int32_t** d_col, hipStream_t stream);
 
 }  // namespace dfa
 

@@ -177,6 +177,56 @@ def test_rcm_reordered_system_vs_oracle(oracle, scatter):
     assert rel(b.get_local(), rb) < TOL
 
 
+class _PaddedP1:
+    """A P1 space with `extra` trailing dofs that belong to no cell (empty CSR rows)."""
+
+    element = dfa._lib.ELEMENT_P1
+
+    def __init__(self, mesh, extra):
+        self._mesh, self._extra = mesh, extra
+        self._W = dfa.FunctionSpace(mesh, "P", 1)
+
+    def mesh(self):
+        return self._mesh
+
+    def cell_dofs(self):
+        return self._mesh.cells()
+
+    def dim(self):
+        return self._mesh.num_vertices() + self._extra
+
+    def coefficient_space(self):
+        return self._W
+
+
+@pytest.mark.parametrize("keep_diagonal", [False, True])
+@pytest.mark.parametrize("where", ["device", "host"])
+def test_csr_structure_build_edge_rows(oracle, keep_diagonal, where, monkeypatch):
+    # the structure built on the device (radix sort of the cell pairs) and on the host give
+    # the oracle's sparsity, including rows in no cell (empty, or the diagonal alone)
+    if where == "host":
+        monkeypatch.setenv("DFA_HOST_CSR", "1")
+    mesh = dfa.UnitCubeMesh(3, 2, 2)
+    V = _PaddedP1(mesh, 5)
+    eng = dfa.fem.Engine(V, keep_diagonal=keep_diagonal)
+    rp, col = eng.structure()
+    torch.cuda.synchronize()
+    orp, ocol = oracle.sparsity(mesh.cells(), V.dim(), keep_diagonal=keep_diagonal)
+    np.testing.assert_array_equal(rp.cpu().numpy(), orp)
+    np.testing.assert_array_equal(col.cpu().numpy(), ocol)
+
+
+def test_csr_device_build_equals_host_vp2(monkeypatch):
+    mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(5, seed=2)
+    dev = dfa.fem.Engine(V)
+    monkeypatch.setenv("DFA_HOST_CSR", "1")
+    host = dfa.fem.Engine(V)
+    a, b = dev.structure(), host.structure()
+    torch.cuda.synchronize()
+    assert dev.nnz == host.nnz
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+
 def test_blocked_equals_atomic_128_properties():
     """Full-size P1 128^3 (12.58M cells): size-independent properties -- blocked and atomic
     scatters agree, Poisson rows sum to zero (K 1 = 0), K is symmetric, sum(b) = int f."""
