@@ -501,7 +501,9 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
   __syncthreads();
 
   // write-out, one owned node per wavefront: its three rows (3 deg values each) are contiguous
-  // in LDS and are three contiguous CSR ranges
+  // in LDS and are three contiguous CSR ranges. Streaming (non-temporal) stores: the values
+  // are not read again by this kernel, and keeping 4.4 GB of them out of L2 measured 10 %
+  // faster on the 64^3 workload.
   const int wave = tid >> 6, lane = tid & 63;
   for (int o = wave; o < no; o += kBT / 64) {
     const uint32_t info = sinfo[o];
@@ -512,7 +514,7 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
       for (int j = lane; j < 3 * len; j += 64) {
         const int comp = j >= len ? (j >= 2 * len ? 2 : 1) : 0;
         const int64_t base = comp == 0 ? b0 : (comp == 1 ? b1 : b2);
-        p.values[base + (j - comp * len)] = svals[off + j];
+        __builtin_nontemporal_store(svals[off + j], p.values + base + (j - comp * len));
       }
     }
     if (WANT_B && lane < 3) p.b[srows[3 * o + lane]] = sbacc[3 * o + lane];
