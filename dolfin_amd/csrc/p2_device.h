@@ -404,9 +404,10 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
       if (r >= ned) continue;
       k = nvx + r;
     }
-    const uint64_t w0 = t.task[3 * (t0 + k)];
-    const uint64_t w1 = t.task[3 * (t0 + k) + 1];
-    const uint64_t w2 = t.task[3 * (t0 + k) + 2];
+    // task words are read once: streaming loads keep L2 for the vertex gathers
+    const uint64_t w0 = __builtin_nontemporal_load(t.task + 3 * (t0 + k));
+    const uint64_t w1 = __builtin_nontemporal_load(t.task + 3 * (t0 + k) + 1);
+    const uint64_t w2 = __builtin_nontemporal_load(t.task + 3 * (t0 + k) + 2);
     const int o = static_cast<int>(w1 & 0xffffu);
     const int a = static_cast<int>((w1 >> 16) & 0xffu);
     double pc[4][3], mu[4], la[4];
