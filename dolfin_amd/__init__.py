@@ -12,8 +12,8 @@ from .function import (Function, FunctionSpace, VectorFunctionSpace, node_reorde
 from .la import Matrix, Vector, solve
 from .fem import (Assembler, DirichletBC, FacetForm, Form, FormSum, SystemAssembler, assemble,
                   assemble_system, body_force_form, elasticity_form, engine_for, exterior_facets,
-                  facet_load_form, facet_mass_form, load_form, mark_exterior_facets, mass_form,
-                  poisson_form)
+                  facet_load_form, facet_mass_form, facet_traction_form, load_form,
+                  mark_exterior_facets, mass_form, poisson_form)
 
 SCATTER_AUTO = _lib.SCATTER_AUTO
 SCATTER_ATOMIC = _lib.SCATTER_ATOMIC
@@ -21,10 +21,11 @@ SCATTER_BLOCKED = _lib.SCATTER_BLOCKED
 
 __all__ = [
     "Mesh", "UnitCubeMesh", "UnitIntervalMesh", "UnitSquareMesh", "jitter_interior",
-    "Function", "FunctionSpace", "VectorFunctionSpace", "node_reordering", "parameters", "Matrix", "Vector", "solve",
+    "Function", "FunctionSpace", "VectorFunctionSpace", "node_reordering", "parameters",
+    "Matrix", "Vector", "solve",
     "Assembler", "DirichletBC", "Form", "SystemAssembler", "assemble", "assemble_system",
     "body_force_form", "elasticity_form", "engine_for", "load_form", "mass_form", "poisson_form",
     "FacetForm", "FormSum", "exterior_facets", "facet_load_form", "facet_mass_form",
-    "mark_exterior_facets",
+    "facet_traction_form", "mark_exterior_facets",
     "SCATTER_AUTO", "SCATTER_ATOMIC", "SCATTER_BLOCKED",
 ]

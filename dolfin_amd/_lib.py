@@ -23,6 +23,7 @@ FORM_VP2_ELASTICITY = 3
 FORM_VP2_BODY_FORCE = 4
 FORM_P1_FACET_MASS = 7
 FORM_P1_FACET_LOAD = 8
+FORM_VP2_FACET_TRACTION = 9
 
 SCATTER_AUTO = 0
 SCATTER_ATOMIC = 1

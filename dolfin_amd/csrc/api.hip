@@ -554,7 +554,8 @@ int dfa_adjoint(dfa_assembler h, int32_t a_form, const double* d_x, const double
 namespace {
 void ensure_facets(dfa_assembler h, hipStream_t s) {
   if (h->has_facets) return;
-  if (h->element != DFA_ELEMENT_P1) throw std::runtime_error("exterior-facet forms need a P1 space");
+  if (h->element != DFA_ELEMENT_P1 && h->element != DFA_ELEMENT_VECTOR_P2)
+    throw std::runtime_error("exterior-facet forms need a P1 or vector-P2 space");
   std::vector<int32_t> cells(static_cast<size_t>(h->nc) * h->nvc);
   if (!cells.empty()) {
     hip_check(hipMemcpyAsync(cells.data(), h->d_cells, cells.size() * sizeof(int32_t),
@@ -631,10 +632,16 @@ int dfa_assemble_facets(dfa_assembler h, int32_t a_form, int32_t L_form, const d
                         double* d_b, void* stream) {
   return guarded([&]() -> int {
     if (!h || !d_x) return fail("NULL handle or coordinates", DFA_ERR_ARG);
-    if (a_form != -1 && a_form != DFA_FORM_P1_FACET_MASS)
-      return fail("facet bilinear form must be DFA_FORM_P1_FACET_MASS or -1", DFA_ERR_ARG);
-    if (L_form != -1 && L_form != DFA_FORM_P1_FACET_LOAD)
-      return fail("facet linear form must be DFA_FORM_P1_FACET_LOAD or -1", DFA_ERR_ARG);
+    const bool vp2 = h->element == DFA_ELEMENT_VECTOR_P2;
+    if (vp2) {
+      if (a_form != -1 || L_form != DFA_FORM_VP2_FACET_TRACTION)
+        return fail("a vector-P2 space takes a_form -1 and L_form DFA_FORM_VP2_FACET_TRACTION", DFA_ERR_ARG);
+    } else {
+      if (a_form != -1 && a_form != DFA_FORM_P1_FACET_MASS)
+        return fail("facet bilinear form must be DFA_FORM_P1_FACET_MASS or -1", DFA_ERR_ARG);
+      if (L_form != -1 && L_form != DFA_FORM_P1_FACET_LOAD)
+        return fail("facet linear form must be DFA_FORM_P1_FACET_LOAD or -1", DFA_ERR_ARG);
+    }
     if (L_form >= 0 && !d_g) return fail("g is NULL", DFA_ERR_ARG);
     if ((d_bc_mask == nullptr) != (d_bc_value == nullptr))
       return fail("Dirichlet mask and values go together", DFA_ERR_ARG);
@@ -662,6 +669,7 @@ int dfa_assemble_facets(dfa_assembler h, int32_t a_form, int32_t L_form, const d
     p.bc_value = d_bc_value;
     p.values = p.has_a ? d_values : nullptr;
     p.b = d_b;
+    p.vp2_traction = vp2;
     hip_check(dfa::launch_facets(t, p, s), "facet_kernel");
     return DFA_OK;
   });

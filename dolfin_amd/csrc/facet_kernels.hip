@@ -1,4 +1,5 @@
-// Exterior-facet integrals of the P1 forms alpha*u*v*ds and g*v*ds.
+// Exterior-facet integrals of the P1 forms alpha*u*v*ds and g*v*ds, and the vector-P2
+// traction inner(t, v)*ds.
 //
 // Assembler::assemble_exterior_facets (dolfin/fem/Assembler.cpp:196-298) tabulates, for every
 // facet with one incident cell, a cell-sized tensor that is nonzero on the facet's vertices and
@@ -93,6 +94,48 @@ __global__ __launch_bounds__(kFacetThreads) void facet_kernel(FacetTopo t, Facet
   }
 }
 
+// Vector-P2 traction inner(t, v)*ds, t constant: on a facet F the P2 basis integrates to 0 at
+// the vertex nodes and to |F| / 3 at the three edge nodes of F (the edges of the cell that do
+// not touch the opposite vertex). Only the load vector has terms, so a Dirichlet row is left
+// as the cell pass wrote it and nothing is lifted.
+template <bool BC>
+__global__ __launch_bounds__(kFacetThreads) void vp2_traction_kernel(FacetTopo t, FacetPass p) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kFacetThreads + threadIdx.x;
+  if (k >= t.num_facets) return;
+  if (p.mask && !p.mask[k]) return;
+  const int64_t c = t.facet_cell[k];
+  const int lf = t.facet_local[k];
+  double xv[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int64_t v = t.cells[c * 4 + j + (j >= lf ? 1 : 0)];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) xv[j][q] = p.x[v * 3 + q];
+  }
+  double e1[3], e2[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    e1[q] = xv[1][q] - xv[0][q];
+    e2[q] = xv[2][q] - xv[0][q];
+  }
+  const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+               cz = e1[0] * e2[1] - e1[1] * e2[0];
+  const double w = 0.5 * sqrt(cx * cx + cy * cy + cz * cz) / 3.0;
+  const double tr[3] = {p.g[0], p.g[1], p.g[2]};
+  // local edge e joins vertices (ea[e], eb[e]): nodes 4..9 of the cell
+  constexpr int ea[6] = {2, 1, 1, 0, 0, 0}, eb[6] = {3, 3, 2, 3, 2, 1};
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    if (ea[e] == lf || eb[e] == lf) continue;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int32_t d = t.cell_dofs[c * 30 + i * 10 + 4 + e];
+      if (BC && p.bc_mask[d]) continue;
+      p1dev::atomic_add_global(p.b + d, tr[i] * w);
+    }
+  }
+}
+
 template <int D>
 hipError_t launch_facets_d(const FacetTopo& t, const FacetPass& p, hipStream_t s) {
   const int64_t grid = (t.num_facets + kFacetThreads - 1) / kFacetThreads;
@@ -107,6 +150,16 @@ hipError_t launch_facets_d(const FacetTopo& t, const FacetPass& p, hipStream_t s
 
 hipError_t launch_facets(const FacetTopo& t, const FacetPass& p, hipStream_t s) {
   if (t.num_facets == 0) return hipSuccess;
+  if (p.vp2_traction) {
+    if (p.gdim != 3 || !p.g) return hipErrorInvalidValue;
+    if (!p.b) return hipSuccess;
+    const int64_t grid = (t.num_facets + kFacetThreads - 1) / kFacetThreads;
+    if (p.bc_mask)
+      hipLaunchKernelGGL((vp2_traction_kernel<true>), dim3(static_cast<unsigned>(grid)), dim3(kFacetThreads), 0, s, t, p);
+    else
+      hipLaunchKernelGGL((vp2_traction_kernel<false>), dim3(static_cast<unsigned>(grid)), dim3(kFacetThreads), 0, s, t, p);
+    return hipGetLastError();
+  }
   switch (p.gdim) {
     case 1: return launch_facets_d<1>(t, p, s);
     case 2: return launch_facets_d<2>(t, p, s);

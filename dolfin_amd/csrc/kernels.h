@@ -90,6 +90,7 @@ struct FacetPass {
   const double* bc_value = nullptr;
   double* values = nullptr;        // NULL: matrix not written (its Dirichlet lifting still is)
   double* b = nullptr;
+  bool vp2_traction = false;       // vector-P2 inner(t, v)*ds, g = the 3 traction components
 };
 hipError_t launch_facets(const FacetTopo& t, const FacetPass& p, hipStream_t s);
 

@@ -185,11 +185,18 @@ def mark_exterior_facets(V, inside, check_midpoint=True):
 
 
 class FacetForm:
-    """A P1 exterior-facet form: alpha*u*v*ds (rank 2) or g*v*ds (rank 1), optionally over
-    the marked facets only (an exterior-facet subdomain)."""
+    """An exterior-facet form, optionally over the marked facets only (an exterior-facet
+    subdomain): P1 alpha*u*v*ds (rank 2) or g*v*ds (rank 1); vector-P2 inner(t, v)*ds with a
+    constant traction t (rank 1)."""
 
-    def __init__(self, kind, rank, V, alpha=0.0, g=None, marker=None):
-        _p1(V)
+    def __init__(self, kind, rank, V, alpha=0.0, g=None, marker=None, traction=None):
+        if kind == _lib.FORM_VP2_FACET_TRACTION:
+            _vp2(V)
+            t3 = np.asarray(traction, dtype=np.float64).reshape(3)
+            self.traction = torch.from_numpy(t3.copy()).cuda()
+        else:
+            _p1(V)
+            self.traction = None
         self.kind, self.rank, self.V = kind, rank, V
         self.alpha = float(alpha)
         self.g = g
@@ -250,6 +257,12 @@ def facet_load_form(V, g, marker=None):
     return FacetForm(_lib.FORM_P1_FACET_LOAD, 1, V, g=g, marker=marker)
 
 
+def facet_traction_form(V, t3, marker=None):
+    """inner(t, v)*ds on a vector-P2 space, t a constant 3-vector (the Neumann term of
+    elasticity), over the exterior facets selected by marker."""
+    return FacetForm(_lib.FORM_VP2_FACET_TRACTION, 1, V, traction=t3, marker=marker)
+
+
 def _split(form):
     if isinstance(form, FormSum):
         return form.cell_parts(), form.facet_parts()
@@ -275,7 +288,8 @@ def _assemble_facets(eng, x, a_facets, L_facets, A_values, b_values, mask=None, 
         _lib.check(lib.dfa_assemble_facets(
             eng.handle, fa.kind if fa is not None else -1, fl.kind if fl is not None else -1,
             _lib.ptr(x), fa.alpha if fa is not None else 0.0,
-            _lib.ptr(fl.g.vector() if fl is not None else None), _lib.ptr(fmask), _lib.ptr(mask),
+            _lib.ptr(None if fl is None else (fl.traction if fl.traction is not None else fl.g.vector())),
+            _lib.ptr(fmask), _lib.ptr(mask),
             _lib.ptr(vals), _lib.ptr(A_values if fa is not None else None),
             _lib.ptr(b_values), _lib.stream()))
 
@@ -442,10 +456,10 @@ class SystemAssembler:
             L = L_cells[0]
         else:
             # a linear form of facet terms only: its cell part is f*v*dx with f = 0
-            if a.V.element != _lib.ELEMENT_P1:
-                raise ValueError("exterior-facet forms need a P1 space")
-            zero = Function(a.V)
-            L = load_form(a.V, zero)
+            if a.V.element == _lib.ELEMENT_P1:
+                L = load_form(a.V, Function(a.V))
+            else:
+                L = body_force_form(a.V, (0.0, 0.0, 0.0))
         self.a, self.L = a_cells[0], L
         self.bcs = [] if bcs is None else (list(bcs) if isinstance(bcs, (list, tuple)) else [bcs])
         self.scatter = scatter

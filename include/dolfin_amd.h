@@ -49,6 +49,7 @@ extern "C" {
 #define DFA_FORM_VP2_BODY_FORCE 4   /* inner(f, v)*dx, f a constant 3-vector              */
 #define DFA_FORM_P1_FACET_MASS 7    /* alpha*u*v*ds over exterior facets, alpha constant   */
 #define DFA_FORM_P1_FACET_LOAD 8    /* g*v*ds over exterior facets, g P1                   */
+#define DFA_FORM_VP2_FACET_TRACTION 9 /* inner(t, v)*ds, vector P2, t a constant 3-vector  */
 
 /* Scatter strategies (chosen per call, or by dfa_autotune). */
 #define DFA_SCATTER_AUTO 0
@@ -205,7 +206,9 @@ int dfa_exterior_facets(dfa_assembler h, int64_t* num_facets, int32_t* facet_cel
  * Assembler::assemble_exterior_facets (dolfin/fem/Assembler.cpp:196-298) and the facet terms
  * SystemAssembler merges into each cell tensor (dolfin/fem/SystemAssembler.cpp:421-481).
  * a_form: DFA_FORM_P1_FACET_MASS (alpha*u*v*ds) or -1; L_form: DFA_FORM_P1_FACET_LOAD
- * (g*v*ds, d_g on the coefficient dofs) or -1. d_facet_mask (NULL = every exterior facet)
+ * (g*v*ds, d_g on the coefficient dofs) or -1. A vector-P2 handle takes a_form -1 and
+ * L_form DFA_FORM_VP2_FACET_TRACTION (inner(t, v)*ds, the Neumann term of elasticity) with
+ * d_g pointing to the three traction components on the device. d_facet_mask (NULL = every exterior facet)
  * selects facets as an exterior-facet subdomain does. With d_bc_mask / d_bc_value the pass
  * completes dfa_assemble_system with the same Dirichlet data: Dirichlet rows are left as the
  * cell pass wrote them and Dirichlet columns are lifted into d_b. d_values may be NULL (the

@@ -193,11 +193,17 @@ class DeviceVector {
 // linear form, over the facets selected by a device mask in dfa_exterior_facets order (an
 // exterior-facet subdomain; NULL = all exterior facets).
 struct FacetTerms {
-  bool robin = false;                 // alpha*u*v*ds present
+  bool robin = false;                 // alpha*u*v*ds present (P1)
   double alpha = 0.0;
-  const double* g = nullptr;          // g on the coefficient dofs (device); NULL: no g*v*ds
+  const double* g = nullptr;          // g on the coefficient dofs (device); NULL: no g*v*ds (P1)
+  const double* traction = nullptr;   // 3 components (device): inner(t, v)*ds (vector P2)
   const uint8_t* facet_mask = nullptr;
-  bool empty() const { return !robin && !g; }
+  bool empty() const { return !robin && !g && !traction; }
+  int32_t a_form() const { return robin ? DFA_FORM_P1_FACET_MASS : -1; }
+  int32_t L_form() const {
+    return traction ? DFA_FORM_VP2_FACET_TRACTION : (g ? DFA_FORM_P1_FACET_LOAD : -1);
+  }
+  const double* L_data() const { return traction ? traction : g; }
 };
 
 // Exterior facets of a space: (cell, local facet) pairs, the order facet masks use.
@@ -258,8 +264,8 @@ class HipAssembler : public AssemblerFlags {
     finalize_tensor = false;
     assemble(b, L);
     finalize_tensor = fin;
-    if (ds.g)
-      check(dfa_assemble_facets(L.space->handle(), -1, DFA_FORM_P1_FACET_LOAD, L.x, 0.0, ds.g,
+    if (ds.L_form() >= 0)
+      check(dfa_assemble_facets(L.space->handle(), -1, ds.L_form(), L.x, 0.0, ds.L_data(),
                                 ds.facet_mask, nullptr, nullptr, nullptr, b.values(), stream_));
     if (finalize_tensor) b.apply("add");
   }
@@ -297,10 +303,9 @@ class HipSystemAssembler {
     // exterior-facet tensors join their cell's tensor before apply_bc
     // (dolfin/fem/SystemAssembler.cpp:421-481)
     if (!ds_.empty())
-      check(dfa_assemble_facets(a_.space->handle(), ds_.robin ? DFA_FORM_P1_FACET_MASS : -1,
-                                ds_.g ? DFA_FORM_P1_FACET_LOAD : -1, a_.x, ds_.alpha, ds_.g,
-                                ds_.facet_mask, mask_, value_, A ? A->values() : nullptr,
-                                b ? b->values() : nullptr, stream_));
+      check(dfa_assemble_facets(a_.space->handle(), ds_.a_form(), ds_.L_form(), a_.x, ds_.alpha,
+                                ds_.L_data(), ds_.facet_mask, mask_, value_,
+                                A ? A->values() : nullptr, b ? b->values() : nullptr, stream_));
   }
   Form a_, L_;
   const uint8_t* mask_;

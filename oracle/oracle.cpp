@@ -668,6 +668,16 @@ void orc_p1_facet_load(int32_t gdim, const double* xc, int32_t lf, const double*
   for (int a = 0; a < nv; ++a) b[a] = (a == lf) ? 0.0 : w * (g[a] + gs);
 }
 
+void orc_vp2_facet_traction(const double* xc, int32_t lf, const double* t3, double* b) {
+  static const int edge[6][2] = {{2, 3}, {1, 3}, {1, 2}, {0, 3}, {0, 2}, {0, 1}};
+  const double w = facet_measure(3, xc, lf) / 3.0;
+  for (int i = 0; i < 3; ++i) {
+    for (int a = 0; a < 4; ++a) b[i * 10 + a] = 0.0;
+    for (int e = 0; e < 6; ++e)
+      b[i * 10 + 4 + e] = (edge[e][0] != lf && edge[e][1] != lf) ? t3[i] * w : 0.0;
+  }
+}
+
 int32_t orc_assemble_facets(int32_t a_kind, int32_t L_kind, int32_t gdim, const int32_t* cells,
                             const double* x, const int32_t* cell_dofs, const int32_t* coeff_dofs,
                             int64_t num_facets, const int32_t* facet_cell,
@@ -676,6 +686,19 @@ int32_t orc_assemble_facets(int32_t a_kind, int32_t L_kind, int32_t gdim, const 
                             double* values, double* b) {
   const int nv = gdim + 1;
   double xc[12], gl[4], A[16], be[4];
+  if (L_kind == ORC_VP2_FACET_TRACTION) {
+    if (a_kind >= 0 || gdim != 3) return -1;
+    double bt[30];
+    for (int64_t k = 0; k < num_facets; ++k) {
+      if (facet_mask && !facet_mask[k]) continue;
+      const int64_t c = facet_cell[k];
+      gather_cell_coords(3, cells + c * 4, x, xc);
+      orc_vp2_facet_traction(xc, facet_local[k], g, bt);
+      if (b)
+        for (int j = 0; j < 30; ++j) b[cell_dofs[c * 30 + j]] += bt[j];
+    }
+    return 0;
+  }
   for (int64_t k = 0; k < num_facets; ++k) {
     if (facet_mask && !facet_mask[k]) continue;
     const int64_t c = facet_cell[k];

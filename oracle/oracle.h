@@ -54,7 +54,8 @@ enum {
   ORC_VP1_ELASTICITY = 5, /* vector P1 (or DG1) version of ORC_VP2_ELASTICITY     */
   ORC_VP1_BODY_FORCE = 6,
   ORC_P1_FACET_MASS = 7,  /* alpha*u*v*ds, alpha constant (Robin term)              */
-  ORC_P1_FACET_LOAD = 8   /* g*v*ds, g P1 (Neumann term)                            */
+  ORC_P1_FACET_LOAD = 8,  /* g*v*ds, g P1 (Neumann term)                            */
+  ORC_VP2_FACET_TRACTION = 9 /* inner(t, v)*ds, t constant vector, vector P2 (traction) */
 };
 
 /* ---- meshes ---------------------------------------------------------------- */
@@ -127,9 +128,15 @@ void orc_p1_facet_mass(int32_t gdim, const double* xc, int32_t local_facet, doub
                        double* A);
 void orc_p1_facet_load(int32_t gdim, const double* xc, int32_t local_facet, const double* g,
                        double* b);
+/* Vector-P2 traction inner(t, v)*ds on the facet opposite local vertex local_facet, in the
+   component-major cell order (30 entries): int_F phi = 0 for the vertex nodes and |F| / 3 for
+   the three edge nodes of the facet, times t_i. */
+void orc_vp2_facet_traction(const double* xc, int32_t local_facet, const double* t3, double* b);
 /* Assembler::assemble_exterior_facets (Assembler.cpp:196-298) for the P1 facet forms over
    the facets with facet_mask[k] != 0 (NULL: all), adding into values (a_kind) and b (L_kind,
-   g through coeff_dofs). a_kind / L_kind = -1 skips that tensor. */
+   g through coeff_dofs). a_kind / L_kind = -1 skips that tensor. L_kind ORC_VP2_FACET_TRACTION
+   (with a_kind -1) takes vector-P2 cell_dofs (30 per cell) and g = the 3 traction
+   components. */
 int32_t orc_assemble_facets(int32_t a_kind, int32_t L_kind, int32_t gdim, const int32_t* cells,
                             const double* x, const int32_t* cell_dofs, const int32_t* coeff_dofs,
                             int64_t num_facets, const int32_t* facet_cell,

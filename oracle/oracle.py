@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _SO = os.path.join(_HERE, "_build", "liboracle.so")
 
 (P1_POISSON, P1_LOAD, P1_MASS, VP2_ELASTICITY, VP2_BODY_FORCE, VP1_ELASTICITY, VP1_BODY_FORCE,
- P1_FACET_MASS, P1_FACET_LOAD) = range(9)
+ P1_FACET_MASS, P1_FACET_LOAD, VP2_FACET_TRACTION) = range(10)
 
 _lib = None
 
@@ -76,6 +76,7 @@ def _declare(L):
     L.orc_exterior_facets.restype = _i64
     L.orc_p1_facet_mass.argtypes = [_i32, _f64p, _i32, ctypes.c_double, _f64p]
     L.orc_p1_facet_load.argtypes = [_i32, _f64p, _i32, _f64p, _f64p]
+    L.orc_vp2_facet_traction.argtypes = [_f64p, _i32, _f64p, _f64p]
     L.orc_assemble_facets.argtypes = [_i32, _i32, _i32, _i32p, _f64p, _i32p, _i32p, _i64, _i32p,
                                       _i32p, _vp, ctypes.c_double, _vp, _i32p, _i32p, _vp, _vp]
     L.orc_assemble_facets.restype = _i32
@@ -278,6 +279,13 @@ def p1_facet_load(xc, local_facet, g):
     gdim = xc.shape[1]
     b = np.empty(gdim + 1)
     lib().orc_p1_facet_load(gdim, xc.ravel(), int(local_facet), _c(g, np.float64), b)
+    return b
+
+
+def vp2_facet_traction(xc, local_facet, t3):
+    """inner(t, v)*ds on one facet of a tetrahedron, vector-P2 component-major (30 entries)."""
+    b = np.empty(30)
+    lib().orc_vp2_facet_traction(_c(xc, np.float64).ravel(), int(local_facet), _c(t3, np.float64), b)
     return b
 
 

@@ -138,3 +138,56 @@ def test_facet_elimination_decomposition(oracle, kind):
     b2 = b0 + np.where(bcm == 0, bf - lift, 0.0)
     assert np.linalg.norm(A1 - A2) <= 1e-13 * np.linalg.norm(A1)
     assert np.linalg.norm(b1 - b2) <= 1e-13 * np.linalg.norm(b1)
+
+
+# ---- vector-P2 traction inner(t, v)*ds ---------------------------------------------------
+_EDGES = [(2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1)]
+
+
+def _p2_basis(lam):
+    """The ten P2 basis functions of a tetrahedron at barycentric point lam (vertex nodes
+    lambda(2 lambda - 1), edge nodes 4 lambda_a lambda_b)."""
+    return np.array([l * (2 * l - 1) for l in lam] + [4 * lam[a] * lam[b] for a, b in _EDGES])
+
+
+def test_vp2_traction_oracle_vs_midpoint_quadrature(oracle):
+    # the edge-midpoint rule on a triangle is exact for quadratics: int_F phi =
+    # |F| / 3 sum over the facet's edge midpoints of phi there
+    rng = np.random.default_rng(11)
+    t3 = np.array([0.3, -1.2, 2.5])
+    for trial in range(5):
+        xc = rng.uniform(-1, 1, (4, 3))
+        for lf in range(4):
+            fv = [a for a in range(4) if a != lf]
+            e1, e2 = xc[fv[1]] - xc[fv[0]], xc[fv[2]] - xc[fv[0]]
+            area = 0.5 * np.linalg.norm(np.cross(e1, e2))
+            integ = np.zeros(10)
+            for a, b in [(fv[0], fv[1]), (fv[0], fv[2]), (fv[1], fv[2])]:
+                lam = np.zeros(4)
+                lam[a] = lam[b] = 0.5
+                integ += area / 3 * _p2_basis(lam)
+            ref = np.concatenate([t3[i] * integ for i in range(3)])
+            np.testing.assert_allclose(oracle.vp2_facet_traction(xc, lf, t3), ref, rtol=0, atol=1e-14)
+
+
+def test_vp2_traction_boundary_known_answer(oracle):
+    # partition of unity on every facet: the components of b sum to t_i times the boundary
+    # area (6 for the unit cube), and to t_i on the marked face x = 1
+    x, cells = oracle.box_mesh(3, 3, 3)
+    ce, ne = oracle.tet_edges(cells)
+    cd = oracle.vp2_dofmap(cells, ce, x.shape[0], ne)
+    nd = 3 * (x.shape[0] + ne)
+    facets = oracle.exterior_facets(cells)
+    t3 = np.array([1.0, -2.0, 0.5])
+    _, b = oracle.assemble_facets(-1, oracle.VP2_FACET_TRACTION, x, cells, cd, facets, ndofs=nd, g=t3)
+    comp = np.zeros(nd, int)
+    for i in range(3):
+        comp[cd[:, i * 10:(i + 1) * 10].ravel()] = i
+    for i in range(3):
+        assert abs(b[comp == i].sum() - 6.0 * t3[i]) < 1e-12
+    mesh = dfa.UnitCubeMesh(3, 3, 3)
+    m = _facet_mask(mesh, facets, lambda p: p[:, 0] > 1 - 1e-12)
+    _, bm = oracle.assemble_facets(-1, oracle.VP2_FACET_TRACTION, x, cells, cd, facets, ndofs=nd,
+                                   g=t3, facet_mask=m)
+    for i in range(3):
+        assert abs(bm[comp == i].sum() - t3[i]) < 1e-12

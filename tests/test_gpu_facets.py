@@ -128,3 +128,68 @@ def test_system_with_facets_and_dirichlet_vs_oracle(oracle, kind, n):
                                              bcm, bcv, coeff_dofs=cd, a_c0=kappa,
                                              L_c0=np.zeros(V.dim()), alpha=1.7, g=g, facet_mask=mask)
     assert _rel(b2.values.cpu().numpy(), b2ref) <= TOL
+
+
+# ---- vector-P2 traction inner(t, v)*ds ---------------------------------------------------
+def _comp_of(V):
+    cd = V.cell_dofs()
+    comp = np.zeros(V.dim(), int)
+    for i in range(3):
+        comp[cd[:, i * 10:(i + 1) * 10].ravel()] = i
+    return comp
+
+
+def test_vp2_traction_vs_oracle_and_known_answer(oracle):
+    from dolfin_amd import workloads
+    mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(3, seed=4)
+    t3 = np.array([0.7, -1.1, 2.0])
+    facets = dfa.exterior_facets(V)
+    comp = _comp_of(V)
+    for marker in (None, lambda p: p[:, 0] > 1 - 1e-12):
+        L = dfa.facet_traction_form(V, t3, marker=marker)
+        b = dfa.assemble(L).get_local()
+        mask = None if marker is None else dfa.mark_exterior_facets(V, marker)
+        _, ref = oracle.assemble_facets(-1, oracle.VP2_FACET_TRACTION, mesh.coordinates(),
+                                        mesh.cells(), V.cell_dofs(), facets, ndofs=V.dim(),
+                                        g=t3, facet_mask=mask)
+        assert _rel(b, ref) < TOL
+        area = 6.0 if marker is None else 1.0
+        for i in range(3):
+            assert abs(b[comp == i].sum() - area * t3[i]) < 1e-12
+
+
+def test_vp2_system_with_traction_and_dirichlet_vs_oracle(oracle):
+    # body force + traction on x = 1, clamped on x = 0: the facet term has no matrix part, so
+    # merging it into the cell tensors before apply_bc adds it to the non-Dirichlet rows only
+    from dolfin_amd import workloads
+    mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(3, seed=6)
+    Q = V.coefficient_space()
+    a = dfa.elasticity_form(V, workloads.to_function(Q, mu), workloads.to_function(Q, lmbda))
+    t3 = np.array([0.0, 0.0, -3.0])
+    right = lambda p: p[:, 0] > 1 - 1e-12
+    L = dfa.body_force_form(V, f3) + dfa.facet_traction_form(V, t3, marker=right)
+    bdofs = dfa.fem.boundary_facet_nodes(V)
+    xd = np.empty((V.dim(), 3))
+    cd = V.cell_dofs()
+    x = mesh.coordinates()
+    for i in range(3):
+        for k in range(10):
+            xd[cd[:, i * 10 + k]] = x[mesh.cells()[:, k]] if k < 4 else 0.5 * (
+                x[mesh.cells()[:, [2, 1, 1, 0, 0, 0][k - 4]]] + x[mesh.cells()[:, [3, 3, 2, 3, 2, 1][k - 4]]])
+    left = bdofs[np.abs(xd[bdofs, 0]) < 1e-12]
+    bc = dfa.DirichletBC(V, 0.0, left)
+    A, b = dfa.assemble_system(a, L, bc)
+    rp, col, vals = A.data()
+    mask = np.zeros(V.dim(), np.uint8)
+    mask[left] = 1
+    ra, rb = oracle.assemble_system(oracle.VP2_ELASTICITY, oracle.VP2_BODY_FORCE,
+                                    mesh.coordinates(), mesh.cells(), cd, rp, col, mask,
+                                    np.zeros(V.dim()), coeff_dofs=Q.cell_dofs(), a_c0=mu,
+                                    a_c1=lmbda, f3=f3)
+    facets = dfa.exterior_facets(V)
+    fm = dfa.mark_exterior_facets(V, right)
+    _, bt = oracle.assemble_facets(-1, oracle.VP2_FACET_TRACTION, mesh.coordinates(),
+                                   mesh.cells(), cd, facets, ndofs=V.dim(), g=t3, facet_mask=fm)
+    bt[mask == 1] = 0.0
+    assert _rel(vals, ra) < TOL
+    assert _rel(b.get_local(), rb + bt) < TOL
