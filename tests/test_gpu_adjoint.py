@@ -86,3 +86,50 @@ def test_p1_adjoint_fd_64():
         worst = max(worst, err)
     mesh._device = {}
     assert worst < 1e-6, worst
+
+
+def _forward_energy_vp2(V, mesh, x, mu, lmbda, f3, u):
+    mesh._device = {"cuda": x}
+    Q = V.coefficient_space()
+    A = dfa.assemble(dfa.elasticity_form(V, dfa.Function(Q, mu), dfa.Function(Q, lmbda)))
+    b = dfa.assemble(dfa.body_force_form(V, f3))
+    rp, col = A.structure()
+    rows = torch.repeat_interleave(torch.arange(V.dim(), device="cuda"), (rp[1:] - rp[:-1]).long())
+    return (torch.sum(A.values * u[rows] * u[col.long()]) - torch.dot(b.values, u)).item()
+
+
+def test_vp2_adjoint_fd_32():
+    """UnitCubeMesh 32^3 vector P2 (196,608 cells, 823,875 dofs): directional derivatives along
+    32 random directions (12 in mu, 10 in lmbda with entries U[0,1], 10 in x with entries
+    U[-1,1]) agree with central differences of the forward assembly (h = 1e-6) to 1e-6."""
+    mesh, V, mu, lmbda, f3, u = workloads.vp2_elasticity_3d(32, with_u=True)
+    print("case vp2 adjoint fd 32", flush=True)
+    m0, l0, ut = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (mu, lmbda, u))
+    x0 = torch.from_numpy(mesh.coordinates().copy()).cuda()
+    mm, ll, xx = (t.clone().requires_grad_(True) for t in (m0, l0, x0))
+    J = adjoint.elastic_energy(V, mm, ll, xx, f3, ut)
+    J.backward()
+    J0 = _forward_energy_vp2(V, mesh, x0, m0, l0, f3, ut)
+    assert abs(J.item() - J0) < 1e-11 * abs(J0)
+    g = torch.Generator(device="cpu").manual_seed(1)
+    h = 1e-6
+    worst = 0.0
+    for i in range(32):
+        if i < 12:
+            d = torch.rand(m0.shape, generator=g, dtype=torch.float64).cuda()
+            fd = (_forward_energy_vp2(V, mesh, x0, m0 + h * d, l0, f3, ut)
+                  - _forward_energy_vp2(V, mesh, x0, m0 - h * d, l0, f3, ut)) / (2 * h)
+            an = torch.dot(mm.grad, d).item()
+        elif i < 22:
+            d = torch.rand(l0.shape, generator=g, dtype=torch.float64).cuda()
+            fd = (_forward_energy_vp2(V, mesh, x0, m0, l0 + h * d, f3, ut)
+                  - _forward_energy_vp2(V, mesh, x0, m0, l0 - h * d, f3, ut)) / (2 * h)
+            an = torch.dot(ll.grad, d).item()
+        else:
+            d = (2 * torch.rand(x0.shape, generator=g, dtype=torch.float64) - 1).cuda()
+            fd = (_forward_energy_vp2(V, mesh, x0 + h * d, m0, l0, f3, ut)
+                  - _forward_energy_vp2(V, mesh, x0 - h * d, m0, l0, f3, ut)) / (2 * h)
+            an = torch.sum(xx.grad * d).item()
+        worst = max(worst, abs(fd - an) / abs(an))
+    mesh._device = {}
+    assert worst < 1e-6, worst
