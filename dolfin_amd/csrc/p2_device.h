@@ -386,23 +386,23 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
   __syncthreads();
 
   // Work is laid out in wavefront-aligned segments so that every wavefront runs one code path:
-  // vertex-node items, one thread each for all ten node pairs b (one gradient factor); then
-  // edge-node items split over two threads, b = 0..4 (half 0) and b = 5..9 (half 1).
+  // the vertex-node items (one gradient factor), then the edge-node items; each item (a cell
+  // and one of its owned nodes) is one thread over all ten node pairs b. (Splitting edge items
+  // over two threads, five pairs each, made a block take five wavefront passes for four
+  // wavefronts: 7 % slower on the 64^3 workload.)
   const int t0 = t.task_ptr[blk], nt = t.task_ptr[blk + 1] - t0;
   const int nvx = t.task_nvert[blk], ned = nt - nvx;
-  const int pv = (nvx + 63) & ~63, pe = (ned + 63) & ~63;
-  for (int kk = tid; kk < pv + 2 * pe; kk += kBT) {
-    int k, half;  // half 2: a vertex item, all b
+  const int pv = (nvx + 63) & ~63;
+  for (int kk = tid; kk < pv + ned; kk += kBT) {
+    int k;
+    bool vert;
     if (kk < pv) {
       if (kk >= nvx) continue;
       k = kk;
-      half = 2;
+      vert = true;
     } else {
-      const int q = kk - pv;
-      half = q >= pe ? 1 : 0;
-      const int r = q - half * pe;
-      if (r >= ned) continue;
-      k = nvx + r;
+      k = nvx + (kk - pv);
+      vert = false;
     }
     // task words are read once: streaming loads keep L2 for the vertex gathers
     const uint64_t w0 = __builtin_nontemporal_load(t.task + 3 * (t0 + k));
@@ -463,39 +463,31 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
             if constexpr (WANT_A) unsafeAtomicAdd(base + i * 3 * deg + j * s2, v);
           });
         };
-        if (VERT || half == 0) {
-          one(std::integral_constant<int, 0>{});
-          one(std::integral_constant<int, 1>{});
-          one(std::integral_constant<int, 2>{});
-          one(std::integral_constant<int, 3>{});
-          one(std::integral_constant<int, 4>{});
-        }
-        if (VERT || half == 1) {
-          one(std::integral_constant<int, 5>{});
-          one(std::integral_constant<int, 6>{});
-          one(std::integral_constant<int, 7>{});
-          one(std::integral_constant<int, 8>{});
-          one(std::integral_constant<int, 9>{});
-        }
+        one(std::integral_constant<int, 0>{});
+        one(std::integral_constant<int, 1>{});
+        one(std::integral_constant<int, 2>{});
+        one(std::integral_constant<int, 3>{});
+        one(std::integral_constant<int, 4>{});
+        one(std::integral_constant<int, 5>{});
+        one(std::integral_constant<int, 6>{});
+        one(std::integral_constant<int, 7>{});
+        one(std::integral_constant<int, 8>{});
+        one(std::integral_constant<int, 9>{});
       };
       // the segment decides the path: uniform over the wavefront
-      if (half == 2) run(std::integral_constant<bool, true>{});
+      if (vert) run(std::integral_constant<bool, true>{});
       else run(std::integral_constant<bool, false>{});
     }
-    if (WANT_B && half != 1) {
+    if (WANT_B) {
       // body force f3 . phi_a: -vol/20 at vertex nodes, vol/5 at edge nodes
 #pragma unroll
       for (int i = 0; i < 3; ++i) be[i] += p.f3[i] * (a < 4 ? -c.vol / 20.0 : c.vol / 5.0);
     }
     if constexpr (WANT_B) {
-      // the load vector and a Dirichlet row's value come from the thread that is not the
-      // second half of an edge item (once per item); the second half adds only its lifting
-      if (half != 1 || BC) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          const double v = (BC && (bcrow >> i & 1u)) ? (half != 1 ? growv[i] : 0.0) : be[i];
-          unsafeAtomicAdd(sbacc + 3 * o + i, v);
-        }
+      for (int i = 0; i < 3; ++i) {
+        const double v = (BC && (bcrow >> i & 1u)) ? growv[i] : be[i];
+        unsafeAtomicAdd(sbacc + 3 * o + i, v);
       }
     }
   }
