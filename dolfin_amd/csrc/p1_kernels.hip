@@ -97,8 +97,7 @@ void p1_chain_kernel(P1BlockTopo t, P1Pass p, LdsLayout L) {
 template <int D>
 __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTopo t, AdjointP1 a,
                                                                       int lds_node, int lds_f,
-                                                                      int lds_u, int lds_own,
-                                                                      int lds_acc) {
+                                                                      int lds_u, int lds_acc) {
   constexpr int N = D + 1;
   constexpr int NS = node_stride(D);
   extern __shared__ __align__(16) unsigned char lds[];
@@ -360,13 +359,12 @@ hipError_t launch_adjoint_blocked_t(const P1BlockTopo& t, const AdjointP1& a, hi
   const int node = o; o += align16(t.max_nodes * NS * 8);
   const int f = o;    o += align16(t.max_nodes * 8);
   const int u = o;    o += align16(t.max_nodes * 8);
-  const int own = o;
   const int acc = o;  o += align16(t.max_rows * (1 + D) * 8);
   auto kern = p1_adjoint_blocked_kernel<D>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, o);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(kern, dim3(t.num_blocks), dim3(kThreads), o, s, t, a, node, f, u, own, acc);
+  hipLaunchKernelGGL(kern, dim3(t.num_blocks), dim3(kThreads), o, s, t, a, node, f, u, acc);
   return hipGetLastError();
 }
 }  // namespace

@@ -451,7 +451,9 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
         node_w<VERT>(c, a, fa);
         auto one = [&](auto bconst) {
           constexpr int b = decltype(bconst)::value;
-          const int rk = static_cast<int>(b < 5 ? (w1 >> (24 + 8 * b)) & 0xffu : (w2 >> (8 * (b - 5))) & 0xffu);
+          int rk;
+          if constexpr (b < 5) rk = static_cast<int>((w1 >> (24 + 8 * b)) & 0xffu);
+          else rk = static_cast<int>((w2 >> (8 * (b - 5))) & 0xffu);
           const int s2 = t.component_major ? deg : 1;
           double* base = svals + off + (t.component_major ? rk : 3 * rk);
           node_block_emit<b, VERT>(c, fa, [&](int i, int j, double v) {
