@@ -231,6 +231,7 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
               rec[2 * k + 1] = P.cell_rank[k];
             }
             t.cell_rec = h->plan.upload(rec, s);
+            if (!P.cell_rank_wide.empty()) t.cell_rank_wide = h->plan.upload(P.cell_rank_wide, s);
             dfa::P1ChainPlan C;
             const char* ce = std::getenv("DFA_P1_CHAINS");
             if (gdim == 3 && !(ce && std::atoi(ce) == 0)) {

@@ -64,6 +64,9 @@ struct P1BlockPlan {
   std::vector<int32_t> cell_ptr;        // [nb+1]
   std::vector<uint64_t> cell_ids;       // 4 x uint16 local node ids per cell appearance
   std::vector<uint64_t> cell_rank;      // 16 x 4-bit column ranks (row a, col b) per appearance
+  // when some CSR row has more than kMaxRank entries: 16 x 8-bit ranks per appearance, two
+  // words (byte a * 4 + b), and cell_rank is zero
+  std::vector<uint64_t> cell_rank_wide;
   std::vector<int32_t> row_ptr;         // [nb+1] owned rows per block
   std::vector<int32_t> row_dof;         // global row (dof) of each owned row
   std::vector<uint32_t> row_info;       // (local node id) | (LDS value offset << 16)

@@ -51,6 +51,7 @@ struct P1BlockTopo {
   const int32_t* cell_ptr = nullptr;
   // per cell appearance, interleaved: [2k] 4 x 16-bit local node ids, [2k+1] 16 x 4-bit ranks
   const uint64_t* cell_rec = nullptr;
+  const uint64_t* cell_rank_wide = nullptr;  // 8-bit ranks, 2 words per appearance (rows > 16)
   const int32_t* row_ptr_b = nullptr;
   const int32_t* row_dof = nullptr;
   const uint32_t* row_info = nullptr;

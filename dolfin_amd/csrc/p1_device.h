@@ -326,6 +326,10 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
   // cell records: issued first so that their latency overlaps the node gather
   const int c0 = t.cell_ptr[blk], ncb = t.cell_ptr[blk + 1] - c0;
   const ulonglong2* __restrict__ crec = reinterpret_cast<const ulonglong2*>(t.cell_rec) + c0;
+  // rows longer than 16 entries (the layout without strided rows): 8-bit ranks, two words
+  static_assert(STRIDED || RR == 0, "the layout for long rows streams its cell records");
+  const ulonglong2* __restrict__ wrec =
+      STRIDED ? nullptr : reinterpret_cast<const ulonglong2*>(t.cell_rank_wide) + c0;
   ulonglong2 rreg[RR > 0 ? RR : 1];
   if constexpr (RR > 0) {
 #pragma unroll
@@ -366,7 +370,7 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
   }
   __syncthreads();
 
-  auto process = [&](uint64_t ids, uint64_t rk) {
+  auto process = [&](uint64_t ids, uint64_t rk, ulonglong2 wr) {
     int l[N];
     double pc[N][D], kap[N], fv[N], gv[N];
 #pragma unroll
@@ -422,9 +426,11 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
       }
       if (do_A) {
         const uint32_t ra = static_cast<uint32_t>(rk >> (16 * a));
+        const uint64_t wa = (a < 2) ? wr.x : wr.y;
 #pragma unroll
         for (int b = 0; b < N; ++b) {
-          const int slot = static_cast<int>((ra >> (4 * b)) & 15u);
+          const int slot = STRIDED ? static_cast<int>((ra >> (4 * b)) & 15u)
+                                   : static_cast<int>((wa >> (8 * ((a & 1) * 4 + b))) & 0xffu);
           if constexpr (ROWSUM_DIAG) {
             if (b == a) continue;  // recovered from the row sum in the write-out
           }
@@ -438,14 +444,18 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
     // all of this thread's cell records were loaded into registers in the prologue
 #pragma unroll
     for (int j = 0; j < RR; ++j)
-      if (tid + j * T < ncb) process(rreg[j].x, rreg[j].y);
+      if (tid + j * T < ncb) process(rreg[j].x, rreg[j].y, make_ulonglong2(0, 0));
   } else {
     // streaming: records prefetched one iteration ahead
     ulonglong2 nxt = tid < ncb ? crec[tid] : make_ulonglong2(0, 0);
+    ulonglong2 wnxt = (!STRIDED && tid < ncb) ? wrec[tid] : make_ulonglong2(0, 0);
     for (int k = tid; k < ncb; k += T) {
-      const ulonglong2 cur = nxt;
-      if (k + T < ncb) nxt = crec[k + T];
-      process(cur.x, cur.y);
+      const ulonglong2 cur = nxt, wcur = wnxt;
+      if (k + T < ncb) {
+        nxt = crec[k + T];
+        if constexpr (!STRIDED) wnxt = wrec[k + T];
+      }
+      process(cur.x, cur.y, wcur);
     }
   }
   __syncthreads();

@@ -306,6 +306,10 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
   std::vector<std::vector<uint8_t>> bdiag(nb);
   P.cell_ids.resize(app.size());
   P.cell_rank.resize(app.size());
+  bool wide_ranks = false;  // a row longer than the 4-bit ranks reach
+  for (int64_t r = 0; r < ndofs && !wide_ranks; ++r)
+    wide_ranks = csr.row_ptr[r + 1] - csr.row_ptr[r] > kMaxRank;
+  if (wide_ranks) P.cell_rank_wide.assign(2 * app.size(), 0);
   P.blk_nnz.assign(nb, 0);
   std::atomic<int> failed{0};
   std::string fail_msg;
@@ -440,25 +444,31 @@ void build_p1_block_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* ce
           ln[a] = local(cells[c * nvc + a]);
           idw |= static_cast<uint64_t>(ln[a]) << (16 * a);
         }
+        uint64_t wr[2] = {0, 0};
         for (int a = 0; a < nvc; ++a) {
           if (ln[a] >= nown) continue;  // not an owned row
           for (int bb = 0; bb < nvc; ++bb) {
             const uint32_t r = rankof[ln[a] * nn + ln[bb]];
-            if (r >= static_cast<uint32_t>(kMaxRank)) {
+            if (r >= (wide_ranks ? 256u : static_cast<uint32_t>(kMaxRank))) {
               failed = 3;
               continue;
             }
-            rk |= static_cast<uint64_t>(r) << (4 * (a * 4 + bb));
+            if (wide_ranks) wr[a >> 1] |= static_cast<uint64_t>(r) << (8 * ((a & 1) * 4 + bb));
+            else rk |= static_cast<uint64_t>(r) << (4 * (a * 4 + bb));
           }
         }
         P.cell_ids[k] = idw;
         P.cell_rank[k] = rk;
+        if (wide_ranks) {
+          P.cell_rank_wide[2 * static_cast<size_t>(k)] = wr[0];
+          P.cell_rank_wide[2 * static_cast<size_t>(k) + 1] = wr[1];
+        }
       }
     }
   });
   if (failed == 1) throw std::runtime_error("blocked plan: too many nodes in a block");
   if (failed == 2) throw std::runtime_error("blocked plan: too many nonzeros in a block");
-  if (failed == 3) throw std::runtime_error("blocked plan: a CSR row has more than 16 entries");
+  if (failed == 3) throw std::runtime_error("blocked plan: a CSR row has more than 256 entries");
 
   P.node_ptr.assign(nb + 1, 0);
   P.row_ptr.assign(nb + 1, 0);

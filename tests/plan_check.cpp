@@ -78,7 +78,10 @@ extern "C" int plan_check_p1(int32_t gdim, int64_t nv, int64_t nc, const int32_t
           const int32_t rdof = P.row_dof[r0 + l[a]];
           const int rlen = csr.row_ptr[rdof + 1] - csr.row_ptr[rdof];
           for (int bb = 0; bb < nvc; ++bb) {
-            const int rank = static_cast<int>((rk >> (4 * (a * 4 + bb))) & 15u);
+            const bool wide = !P.cell_rank_wide.empty();
+            const int rank = wide ? static_cast<int>((P.cell_rank_wide[2 * (c0 + k) + (a >> 1)] >>
+                                                      (8 * ((a & 1) * 4 + bb))) & 0xffu)
+                                  : static_cast<int>((rk >> (4 * (a * 4 + bb))) & 15u);
             if (rank >= rlen || (rlen <= 16 && l[a] * 17 + rank >= nr * 17)) return -14;
             const int slot = own[l[a]] + rank;
             if (slot >= nnzb) return -7;

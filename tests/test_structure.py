@@ -90,13 +90,20 @@ def _plan_check(plan_lib, V, target_rows):
 
 
 @pytest.mark.parametrize("case,target", [("square", 64), ("square", 512), ("cube", 32),
-                                         ("cube", 256), ("cube_perm", 100), ("interval", 8)])
+                                         ("cube", 256), ("cube_perm", 100), ("interval", 8),
+                                         ("delaunay", 64), ("delaunay", 160)])
 def test_blocked_plan_walk(oracle, plan_lib, case, target):
     rng = np.random.default_rng(2)
     if case == "square":
         mesh = dfa.UnitSquareMesh(32, 32)
     elif case == "interval":
         mesh = dfa.UnitIntervalMesh(50)
+    elif case == "delaunay":
+        # unstructured tetrahedra: rows longer than 16 entries, 8-bit ranks
+        import scipy.spatial
+        pts = np.vstack([rng.uniform(0, 1, (600, 3)),
+                         np.array(np.meshgrid([0., 1.], [0., 1.], [0., 1.])).reshape(3, -1).T])
+        mesh = dfa.Mesh(pts, np.sort(scipy.spatial.Delaunay(pts).simplices, axis=1))
     else:
         mesh = dfa.UnitCubeMesh(6, 5, 7)
         dfa.jitter_interior(mesh, 0.2 / 7, rng)
