@@ -1,8 +1,8 @@
 """Autotuner for the blocked-scatter plans and the batched launch.
 
 The tunables are the plan granularity (P1 rows per block, P1 workgroup size, vector-P2 LDS
-values per block -- i.e. cells per workgroup) and the number of resident workgroups per CU
-of the batched persistent kernel. Candidates are timed with HIP events on a sample of the
+values per block -- i.e. cells per workgroup) and the launch mode of the batched kernels
+(one workgroup per row block, or a number of persistent workgroups per CU). Candidates are timed with HIP events on a sample of the
 workload and the fastest is returned; results are cached per (kind, size class).
 """
 import ctypes
@@ -68,7 +68,8 @@ def tune_forms(make_forms, candidates=None, reps=5):
 
 
 def tune_batch_grid(make_batch, grids=(1, 2, 4, 0), reps=5):
-    """Time the batched launch for several resident-workgroups-per-CU limits."""
+    """Time the batched launch for several persistent-workgroups-per-CU counts (0: one
+    workgroup per row block)."""
     times = {}
     for g in grids:
         b = make_batch(g)

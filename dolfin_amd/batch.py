@@ -1,9 +1,11 @@
-"""Batched assembly of many independent forms in one persistent, graph-captured launch.
+"""Batched assembly of many independent forms in one graph-captured launch.
 
 Each member is a bilinear form (P1 kappa-Poisson on a tetrahedral mesh, or vector-P2
-elasticity) with its own function space, coefficients and CSR tensor. The library builds a
-unit table over all members' row blocks and captures one persistent kernel launch in a
-hipGraph; assemble() replays it. Coefficient vectors may be updated in place between calls.
+elasticity) with its own function space, coefficients and CSR tensor. The library builds
+unit tables over all members' row blocks and captures two kernels (vector-P2 members, then
+P1 members, each at its own occupancy) in one hipGraph; assemble() replays it. grid_per_cu=0
+launches one workgroup per row block, grid_per_cu>0 that many persistent workgroups per CU.
+Coefficient vectors may be updated in place between calls.
 """
 import ctypes
 

@@ -150,11 +150,12 @@ int dfa_set_plan_defaults(int32_t p1_block_rows, int32_t p1_threads, int32_t p2_
 /*
  * Batched assembly (no counterpart in the reference, which assembles one form per
  * Assembler::assemble call): n independent assemblers, each with one bilinear form
- * (DFA_FORM_P1_POISSON on tetrahedra or DFA_FORM_VP2_ELASTICITY), assembled by ONE
- * persistent kernel whose launch is captured once in a hipGraph. Every member needs its
- * blocked plan (built at creation when coordinates are given). d_x, d_c0, d_c1 (lmbda, P2
- * only) and d_values are per member and are read/written at every launch, so coefficients
- * may change between launches. grid_per_cu limits resident workgroups per CU (0 = maximum).
+ * (DFA_FORM_P1_POISSON on tetrahedra or DFA_FORM_VP2_ELASTICITY), assembled by two kernels
+ * (vector-P2 members, then P1 members) whose launches are captured once in a hipGraph. Every
+ * member needs its blocked plan (built at creation when coordinates are given). d_x, d_c0,
+ * d_c1 (lmbda, P2 only) and d_values are per member and are read/written at every launch, so
+ * coefficients may change between launches. grid_per_cu = 0 launches one workgroup per row
+ * block; grid_per_cu > 0 launches persistent workgroups, at most that many per CU.
  */
 typedef struct dfa_batch_s* dfa_batch;
 int dfa_batch_create(dfa_batch* out, int32_t n, const dfa_assembler* hs, const int32_t* forms,
