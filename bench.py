@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=128, help="P1 mesh size (cubes per side)")
     ap.add_argument("--p2-n", type=int, default=64, help="vector P2 mesh size (0 = skip)")
-    ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "blocked"])
+    ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "blocked", "colored"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--batch", type=int, default=256,
@@ -133,7 +133,7 @@ def main():
     from dolfin_amd import adjoint, workloads
 
     scatter = {"auto": dfa.SCATTER_AUTO, "atomic": dfa.SCATTER_ATOMIC,
-               "blocked": dfa.SCATTER_BLOCKED}[args.scatter]
+               "blocked": dfa.SCATTER_BLOCKED, "colored": dfa.SCATTER_COLORED}[args.scatter]
     dev = torch.cuda.get_device_properties(0)
 
     # ---- headline: P1 128^3 fused A + b --------------------------------------------------
@@ -161,7 +161,8 @@ def main():
     alg_bytes = workloads.p1_algorithmic_bytes(nc, 4, mesh.num_vertices(), 3, eng.nnz, V.dim(),
                                                n_coeff=1, with_vector=True)
     achieved = alg_bytes / (ev_ms * 1e-3) / 1e9
-    used_scatter = "blocked" if info["num_blocks"] > 0 else "atomic"
+    used_scatter = ("colored" if scatter == dfa.SCATTER_COLORED else
+                    "blocked" if info["num_blocks"] > 0 else "atomic")
     traffic = load_traffic(f"p1_{used_scatter}_{args.n}")
     result = {
         "metric": METRIC,
@@ -248,6 +249,29 @@ def main():
                           "traffic": load_traffic(f"p1_spmv_{args.n}"),
                           "workload": f"y = A x on the assembled P1 {args.n}^3 matrix"}
         log("spmv timed")
+        # the other two scatter strategies on the same A+b workload (the choice between
+        # owner-computes, graph-coloured read-add-write and global atomics)
+        strat = {}
+        for name, sc in (("atomic", dfa.SCATTER_ATOMIC), ("colored", dfa.SCATTER_COLORED)):
+            if sc == scatter:
+                continue
+            asm_s = dfa.SystemAssembler(a, L, scatter=sc)
+            t = time.perf_counter()
+            asm_s.assemble(A, b)
+            torch.cuda.synchronize()
+            t_s = time.perf_counter() - t
+            w_s, ev_s = timed_steps(lambda: asm_s.assemble(A, b), 10, 2)
+            strat[name] = {"ms": float(np.mean(ev_s)), "first_call_s": t_s,
+                           "hbm_frac": alg_bytes / (float(np.mean(ev_s)) * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            if sc == dfa.SCATTER_COLORED:
+                strat[name]["colors"] = len(eng.color_ptr()) - 1
+                strat[name]["traffic"] = load_traffic(f"p1_colored_{args.n}")
+            else:
+                strat[name]["traffic"] = load_traffic(f"p1_atomic_{args.n}")
+            log(f"{name} scatter timed")
+        asm.assemble(A, b)
+        sec["p1_scatter_strategies"] = dict(
+            strat, workload=f"P1 {args.n}^3 A+b, same inputs as the headline, other scatters")
         # the same workload with DOLFIN's default numbering (reverse Cuthill-McKee nodes)
         t = time.perf_counter()
         Vr = dfa.FunctionSpace(mesh, "P", 1, reorder=True)

@@ -6,7 +6,7 @@ The compute path is libdolfin_amd.so (HIP for gfx950, C ABI in include/dolfin_am
 torch provides device memory and streams.
 """
 from . import _lib
-from .mesh import Mesh, UnitCubeMesh, UnitIntervalMesh, UnitSquareMesh, jitter_interior
+from .mesh import Mesh, MeshColoring, UnitCubeMesh, UnitIntervalMesh, UnitSquareMesh, jitter_interior
 from .function import (Function, FunctionSpace, VectorFunctionSpace, node_reordering,
                        parameters)
 from .la import Matrix, Vector, solve
@@ -18,14 +18,15 @@ from .fem import (Assembler, DirichletBC, FacetForm, Form, FormSum, SystemAssemb
 SCATTER_AUTO = _lib.SCATTER_AUTO
 SCATTER_ATOMIC = _lib.SCATTER_ATOMIC
 SCATTER_BLOCKED = _lib.SCATTER_BLOCKED
+SCATTER_COLORED = _lib.SCATTER_COLORED
 
 __all__ = [
-    "Mesh", "UnitCubeMesh", "UnitIntervalMesh", "UnitSquareMesh", "jitter_interior",
+    "Mesh", "MeshColoring", "UnitCubeMesh", "UnitIntervalMesh", "UnitSquareMesh", "jitter_interior",
     "Function", "FunctionSpace", "VectorFunctionSpace", "node_reordering", "parameters",
     "Matrix", "Vector", "solve",
     "Assembler", "DirichletBC", "Form", "SystemAssembler", "assemble", "assemble_system",
     "body_force_form", "elasticity_form", "engine_for", "load_form", "mass_form", "poisson_form",
     "FacetForm", "FormSum", "exterior_facets", "facet_load_form", "facet_mass_form",
     "facet_traction_form", "mark_exterior_facets",
-    "SCATTER_AUTO", "SCATTER_ATOMIC", "SCATTER_BLOCKED",
+    "SCATTER_AUTO", "SCATTER_ATOMIC", "SCATTER_BLOCKED", "SCATTER_COLORED",
 ]

@@ -321,6 +321,8 @@ int dfa_assembler_destroy(dfa_assembler h) {
   if (!h) return DFA_OK;
   h->plan.release();
   h->base.release();
+  h->colors.release();
+  h->facets.release();
   delete h;
   return DFA_OK;
 }
@@ -351,16 +353,81 @@ static int resolve_scatter(dfa_assembler h, int32_t scatter) {
     if (scatter == DFA_SCATTER_AUTO) return h->has_plan ? DFA_SCATTER_BLOCKED : DFA_SCATTER_ATOMIC;
     if (scatter == DFA_SCATTER_BLOCKED && !h->has_plan)
       throw std::runtime_error("blocked scatter unavailable for this mesh: " + h->plan_error);
-    if (scatter != DFA_SCATTER_ATOMIC && scatter != DFA_SCATTER_BLOCKED)
+    if (scatter != DFA_SCATTER_ATOMIC && scatter != DFA_SCATTER_BLOCKED &&
+        scatter != DFA_SCATTER_COLORED)
       throw std::runtime_error("unknown scatter strategy");
     return scatter;
   }
+  if (scatter == DFA_SCATTER_COLORED)
+    throw std::runtime_error("the coloured scatter is implemented for P1 elements");
   if (scatter == DFA_SCATTER_AUTO) return h->has_vp2_plan ? DFA_SCATTER_BLOCKED : DFA_SCATTER_ATOMIC;
   if (scatter == DFA_SCATTER_BLOCKED && !h->has_vp2_plan)
     throw std::runtime_error("blocked scatter unavailable for this mesh: " + h->plan_error);
   if (scatter != DFA_SCATTER_ATOMIC && scatter != DFA_SCATTER_BLOCKED)
     throw std::runtime_error("unknown scatter strategy");
   return scatter;
+}
+
+// Colour-ordered cell arrays of the coloured scatter (P1), built on first use: a greedy
+// colouring of the cells in which two cells sharing a dof differ (for the vertex dofmap this
+// is MeshColoring::color_cells(mesh, "vertex"), dolfin/mesh/MeshColoring.cpp:45-56), cells
+// stably sorted by colour, column ranks found once on the device.
+static void ensure_colors(dfa_assembler h, hipStream_t s) {
+  if (h->has_colors) return;
+  const int32_t nvc = h->nvc;
+  const size_t n = static_cast<size_t>(h->nc) * nvc;
+  std::vector<int32_t> cd(n), kd(n), rp(static_cast<size_t>(h->ndofs) + 1);
+  hip_check(hipMemcpyAsync(cd.data(), h->d_cdofs, n * sizeof(int32_t), hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+  hip_check(hipMemcpyAsync(kd.data(), h->d_kdofs, n * sizeof(int32_t), hipMemcpyDeviceToHost, s),
+            "hipMemcpyAsync");
+  hip_check(hipMemcpyAsync(rp.data(), h->d_row_ptr, rp.size() * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  for (int64_t r = 0; r < h->ndofs; ++r)
+    if (rp[r + 1] - rp[r] > 256)
+      throw std::runtime_error("coloured scatter: a CSR row holds more than 256 entries");
+  std::vector<int32_t> color(static_cast<size_t>(h->nc));
+  const int32_t ncol = dfa::greedy_cell_coloring(h->nc, nvc, cd.data(), h->ndofs, 1, color.data());
+  std::vector<int64_t> ptr(static_cast<size_t>(ncol) + 1, 0);
+  for (int32_t c : color) ++ptr[c + 1];
+  for (int32_t k = 0; k < ncol; ++k) ptr[k + 1] += ptr[k];
+  std::vector<int32_t> perm(static_cast<size_t>(h->nc));
+  {
+    std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t c = 0; c < h->nc; ++c) perm[fill[color[c]]++] = static_cast<int32_t>(c);
+  }
+  const bool same = cd == kd;
+  dfa::DeviceArena tmp;
+  try {
+    const int32_t* d_perm = tmp.upload(perm, s);
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, n * sizeof(int32_t)), "hipMalloc");
+    h->d_ccells = h->colors.adopt(static_cast<int32_t*>(p), n * sizeof(int32_t));
+    hip_check(hipMalloc(&p, n * sizeof(int32_t)), "hipMalloc");
+    h->d_cdofs_c = h->colors.adopt(static_cast<int32_t*>(p), n * sizeof(int32_t));
+    if (same) {
+      h->d_kdofs_c = h->d_cdofs_c;
+    } else {
+      hip_check(hipMalloc(&p, n * sizeof(int32_t)), "hipMalloc");
+      h->d_kdofs_c = h->colors.adopt(static_cast<int32_t*>(p), n * sizeof(int32_t));
+    }
+    hip_check(hipMalloc(&p, n * nvc), "hipMalloc");
+    h->d_crank = h->colors.adopt(static_cast<uint8_t*>(p), n * nvc);
+    hip_check(dfa::launch_p1_color_gather(h->nc, nvc, d_perm, h->d_cells, h->d_cdofs,
+                                          same ? nullptr : h->d_kdofs, h->d_row_ptr, h->d_col,
+                                          h->d_ccells, h->d_cdofs_c, same ? nullptr : h->d_kdofs_c,
+                                          h->d_crank, s),
+              "p1_color_gather_kernel");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  } catch (...) {
+    tmp.release();
+    h->colors.release();
+    throw;
+  }
+  tmp.release();
+  h->color_ptr = std::move(ptr);
+  h->has_colors = true;
 }
 
 // One fused pass over the cells (Assembler::assemble_cells / SystemAssembler::cell_wise_assembly).
@@ -407,6 +474,21 @@ static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const doubl
       t.row_ptr = h->d_row_ptr;
       t.col = h->d_col;
       hip_check(dfa::launch_p1_atomic(t, p, s), "p1_atomic_kernel");
+    } else if (sc == DFA_SCATTER_COLORED) {
+      ensure_colors(h, s);
+      if (!add && values) hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
+      if (!add && b) hip_check(dfa::launch_fill_zero(b, h->ndofs, s), "fill_zero");
+      dfa::P1ColorTopo t;
+      t.cells = h->d_ccells;
+      t.cell_dofs = h->d_cdofs_c;
+      t.coeff_dofs = h->d_kdofs_c;
+      t.rank = h->d_crank;
+      t.row_ptr = h->d_row_ptr;
+      for (size_t k = 0; k + 1 < h->color_ptr.size(); ++k) {
+        t.begin = h->color_ptr[k];
+        t.end = h->color_ptr[k + 1];
+        hip_check(dfa::launch_p1_colored(t, p, s), "p1_colored_kernel");
+      }
     } else {
       if (!add && !h->rows_covered) {
         if (values) hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
@@ -727,12 +809,43 @@ int dfa_plan_info(dfa_assembler h, int32_t scatter, int64_t* plan_bytes, int64_t
       if (num_blocks) *num_blocks = h->btopo.num_blocks;
       if (cell_appearances) *cell_appearances = h->appearances;
       if (lds_bytes) *lds_bytes = static_cast<int32_t>(dfa::p1_blocked_lds_bytes(h->btopo, p));
+    } else if (sc == DFA_SCATTER_COLORED) {
+      ensure_colors(h, nullptr);
+      if (plan_bytes) *plan_bytes = h->colors.bytes;
+      if (num_blocks) *num_blocks = static_cast<int64_t>(h->color_ptr.size()) - 1;
+      if (cell_appearances) *cell_appearances = h->nc;
+      if (lds_bytes) *lds_bytes = 0;
     } else {
       if (plan_bytes) *plan_bytes = h->element == DFA_ELEMENT_VECTOR_P2 ? h->nc * 100 : 0;
       if (num_blocks) *num_blocks = 0;
       if (cell_appearances) *cell_appearances = h->nc;
       if (lds_bytes) *lds_bytes = 0;
     }
+    return DFA_OK;
+  });
+}
+
+int dfa_mesh_coloring(int64_t num_cells, int32_t nvc, const int32_t* h_cells, int64_t num_vertices,
+                      int32_t entity_dim, int32_t* h_colors, int32_t* num_colors) {
+  return guarded([&]() -> int {
+    if (num_cells < 0 || num_vertices < 0 || nvc < 1 || entity_dim < 0 || entity_dim >= nvc)
+      return fail("invalid mesh colouring arguments", DFA_ERR_ARG);
+    if (num_cells > 0 && (!h_cells || !h_colors)) return fail("NULL array", DFA_ERR_ARG);
+    const int32_t n = dfa::greedy_cell_coloring(num_cells, nvc, h_cells, num_vertices,
+                                                entity_dim + 1, h_colors);
+    if (num_colors) *num_colors = n;
+    return DFA_OK;
+  });
+}
+
+int dfa_color_info(dfa_assembler h, int32_t* num_colors, int64_t* h_color_ptr, void* stream) {
+  return guarded([&]() -> int {
+    if (!h) return fail("NULL handle", DFA_ERR_ARG);
+    if (h->element != DFA_ELEMENT_P1)
+      return fail("the coloured scatter is implemented for P1 elements", DFA_ERR_ARG);
+    ensure_colors(h, static_cast<hipStream_t>(stream));
+    if (num_colors) *num_colors = static_cast<int32_t>(h->color_ptr.size()) - 1;
+    if (h_color_ptr) std::copy(h->color_ptr.begin(), h->color_ptr.end(), h_color_ptr);
     return DFA_OK;
   });
 }

@@ -88,6 +88,14 @@ struct dfa_assembler_s {
   bool vp2_component_major = true;
   bool has_vp2_plan = false;
   dfa::VP2BlockTopo vtopo;
+  // graph-coloured scatter (P1), built on first use: cells in colour order
+  bool has_colors = false;
+  std::vector<int64_t> color_ptr;      // [ncolors + 1]
+  int32_t* d_ccells = nullptr;
+  int32_t* d_cdofs_c = nullptr;
+  int32_t* d_kdofs_c = nullptr;        // aliases d_cdofs_c when the dofmaps are equal
+  uint8_t* d_crank = nullptr;
+  dfa::DeviceArena colors;
   // exterior facets (P1), built on first use
   bool has_facets = false;
   std::vector<int32_t> facet_cell;

@@ -108,6 +108,12 @@ void build_exterior_facets(int32_t gdim, int64_t num_cells, const int32_t* cells
 // unvisited neighbours appended in ascending degree, the whole sequence reversed -- the
 // ordering BoostGraphOrdering::compute_cuthill_mckee(graph, true) returns.
 void rcm_ordering(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int32_t* old_to_new);
+
+// Greedy colouring of cells in index order: cell c takes the smallest colour held by none of
+// its neighbours of lower index, two cells being neighbours when they share at least
+// `min_shared` of their `nvc` entity ids. Returns the number of colours.
+int32_t greedy_cell_coloring(int64_t num_cells, int32_t nvc, const int32_t* cell_entities,
+                             int64_t num_entities, int32_t min_shared, int32_t* colors);
 // Same from a start vertex given by the caller (one component).
 void cuthill_mckee_from(int64_t n, const int32_t* adj_ptr, const int32_t* adj, int32_t start,
                         std::vector<int32_t>& order);

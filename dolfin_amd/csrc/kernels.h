@@ -32,6 +32,18 @@ struct P1AtomicTopo {
   const int32_t* col = nullptr;
 };
 
+// Graph-coloured scatter: cells stored in colour order (cells of one colour share no dof),
+// each with its vertex ids, dofs, coefficient dofs and the rank of column d[b] in row d[a]
+// (one byte per pair). One launch per colour over [color_ptr[k], color_ptr[k+1]).
+struct P1ColorTopo {
+  int64_t begin = 0, end = 0;          // cell range of the colour being launched
+  const int32_t* cells = nullptr;      // [nc x nvc], colour order
+  const int32_t* cell_dofs = nullptr;  // [nc x nvc], colour order
+  const int32_t* coeff_dofs = nullptr; // [nc x nvc], colour order (may alias cell_dofs)
+  const uint8_t* rank = nullptr;       // [nc x nvc x nvc]
+  const int32_t* row_ptr = nullptr;
+};
+
 struct P1BlockTopo {
   int32_t num_blocks = 0;
   int32_t max_nodes = 0, max_rows = 0, max_nnz = 0, max_row_len = 0;
@@ -63,6 +75,15 @@ struct P1BlockTopo {
 };
 
 hipError_t launch_p1_atomic(const P1AtomicTopo& t, const P1Pass& p, hipStream_t s);
+hipError_t launch_p1_colored(const P1ColorTopo& t, const P1Pass& p, hipStream_t s);
+// Colour-order copies of a handle's cell arrays and the column ranks of the coloured scatter:
+// for i in [0, nc): cell perm[i]'s vertex ids, dofs and coefficient dofs go to position i.
+// Fails (hipErrorInvalidValue) if a row holds more than 256 entries.
+hipError_t launch_p1_color_gather(int64_t num_cells, int32_t nvc, const int32_t* perm,
+                                  const int32_t* cells, const int32_t* cell_dofs,
+                                  const int32_t* coeff_dofs, const int32_t* row_ptr,
+                                  const int32_t* col, int32_t* out_cells, int32_t* out_dofs,
+                                  int32_t* out_kdofs, uint8_t* out_rank, hipStream_t s);
 hipError_t launch_p1_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t s);
 size_t p1_blocked_lds_bytes(const P1BlockTopo& t, const P1Pass& p);
 hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);

@@ -9,8 +9,11 @@
 // cell needs one division. Pinned in tests by python/test/unit/fem/test_assembler.py:351-390
 // (reference tetrahedron K, M) and the assembled norms of test_system_assembler.py:73-118.
 //
-// Two scatter strategies (dolfin/la/EigenMatrix.cpp:156-166 is the serial original):
+// Three scatter strategies (dolfin/la/EigenMatrix.cpp:156-166 is the serial original):
 //   atomic  : one thread per cell, binary search of the column in the CSR row, FP64 atomic add;
+//   colored : cells grouped by a greedy colouring in which cells of one colour share no dof
+//             (dolfin/mesh/MeshColoring.cpp:57-121), one launch per colour, thread per cell,
+//             plain read-add-write of the CSR entries at precomputed column ranks;
 //   blocked : one workgroup per row block (owner computes): the block's nodes are gathered
 //             into LDS, every cell touching the block is evaluated from LDS, owned rows are
 //             accumulated with LDS atomics at precomputed 4-bit column ranks, and each CSR
@@ -73,6 +76,81 @@ __global__ __launch_bounds__(kThreads) void p1_atomic_kernel(P1AtomicTopo t, P1P
         atomic_add_global(p.values + find_slot(t.row_ptr, t.col, d[a], d[b]), A[a][b]);
     }
     if (WANT_B && p.want_b) atomic_add_global(p.b + d[a], be[a]);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Graph-coloured scatter: thread per cell of one colour. No two cells of a colour share a
+// dof, so the CSR rows a thread updates are its own during the launch: plain
+// read-add-write, no atomics, one launch per colour (stream order separates the colours).
+// ---------------------------------------------------------------------------------------
+template <int D, int AFORM, bool WANT_B, bool BC>
+__global__ __launch_bounds__(kThreads) void p1_colored_kernel(P1ColorTopo t, P1Pass p) {
+  constexpr int N = D + 1;
+  constexpr bool WANT_A = AFORM >= 0;
+  const int64_t c = t.begin + static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (c >= t.end) return;
+  int32_t v[N], d[N], k[N];
+#pragma unroll
+  for (int a = 0; a < N; ++a) {
+    v[a] = t.cells[c * N + a];
+    d[a] = t.cell_dofs[c * N + a];
+    k[a] = t.coeff_dofs[c * N + a];
+  }
+  double pc[N][D], kap[N], fv[N], A[N][N], be[N];
+#pragma unroll
+  for (int a = 0; a < N; ++a) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) pc[a][i] = p.x[static_cast<int64_t>(v[a]) * D + i];
+    kap[a] = (AFORM == DFA_FORM_P1_POISSON) ? p.kappa[k[a]] : 0.0;
+    fv[a] = WANT_B ? p.f[k[a]] : 0.0;
+    be[a] = 0.0;
+  }
+  p1_element<D, AFORM, WANT_B>(pc, kap, fv, A, be);
+  if constexpr (BC) {
+    bool isbc[N];
+    double gv[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+      isbc[a] = p.bc_mask[d[a]] != 0;
+      gv[a] = isbc[a] ? p.bc_value[d[a]] : 0.0;
+    }
+    apply_bc<N>(A, be, isbc, gv);
+  }
+  const uint8_t* rk = t.rank + c * (N * N);
+#pragma unroll
+  for (int a = 0; a < N; ++a) {
+    if (WANT_A && p.want_A) {
+      double* row = p.values + t.row_ptr[d[a]];
+#pragma unroll
+      for (int b = 0; b < N; ++b) row[rk[a * N + b]] += A[a][b];
+    }
+    if (WANT_B && p.want_b) p.b[d[a]] += be[a];
+  }
+}
+
+// Colour-order copies and column ranks (rank of d[b] in row d[a] by binary search).
+__global__ void p1_color_gather_kernel(int64_t nc, int32_t nvc, const int32_t* __restrict__ perm,
+                                       const int32_t* __restrict__ cells,
+                                       const int32_t* __restrict__ cdofs,
+                                       const int32_t* __restrict__ kdofs,
+                                       const int32_t* __restrict__ row_ptr,
+                                       const int32_t* __restrict__ col, int32_t* __restrict__ oc,
+                                       int32_t* __restrict__ od, int32_t* __restrict__ ok,
+                                       uint8_t* __restrict__ orank) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  const int64_t c = perm[i];
+  for (int a = 0; a < nvc; ++a) {
+    oc[i * nvc + a] = cells[c * nvc + a];
+    od[i * nvc + a] = cdofs[c * nvc + a];
+    if (ok) ok[i * nvc + a] = kdofs[c * nvc + a];
+  }
+  for (int a = 0; a < nvc; ++a) {
+    const int32_t r = cdofs[c * nvc + a];
+    for (int b = 0; b < nvc; ++b)
+      orank[(i * nvc + a) * nvc + b] =
+          static_cast<uint8_t>(find_slot(row_ptr, col, r, cdofs[c * nvc + b]) - row_ptr[r]);
   }
 }
 
@@ -222,6 +300,15 @@ hipError_t launch_atomic_t(const P1AtomicTopo& t, const P1Pass& p, hipStream_t s
   return hipGetLastError();
 }
 
+template <int D, int AFORM, bool WANT_B, bool BC>
+hipError_t launch_colored_t(const P1ColorTopo& t, const P1Pass& p, hipStream_t s) {
+  if (t.end <= t.begin) return hipSuccess;
+  const int64_t grid = (t.end - t.begin + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL((p1_colored_kernel<D, AFORM, WANT_B, BC>), dim3(static_cast<unsigned>(grid)),
+                     dim3(kThreads), 0, s, t, p);
+  return hipGetLastError();
+}
+
 template <int D, int AFORM, bool WANT_B, bool BC, bool STRIDED, int T, bool WIDE, int RR>
 hipError_t launch_blocked_tt(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
   const LdsLayout L = lds_layout(D, t.max_nodes, t.max_rows, t.max_nnz, WANT_B, BC, STRIDED, WIDE);
@@ -307,6 +394,25 @@ hipError_t dispatch_atomic(const P1AtomicTopo& t, const P1Pass& p, hipStream_t s
 }
 
 template <int D>
+hipError_t dispatch_colored(const P1ColorTopo& t, const P1Pass& p, hipStream_t s) {
+#define DFA_C(AF, WB, BCv) return launch_colored_t<D, AF, WB, BCv>(t, p, s)
+#define DFA_CBC(AF, WB) \
+  if (p.bc) DFA_C(AF, WB, true); else DFA_C(AF, WB, false)
+#define DFA_CWB(AF) \
+  if (p.want_b) { DFA_CBC(AF, true); } else { DFA_CBC(AF, false); }
+  switch (p.a_form) {
+    case DFA_FORM_P1_POISSON: DFA_CWB(DFA_FORM_P1_POISSON);
+    case DFA_FORM_P1_MASS: DFA_CWB(DFA_FORM_P1_MASS);
+    default:
+      if (!p.want_b) return hipErrorInvalidValue;
+      DFA_CBC(-1, true);
+  }
+#undef DFA_C
+#undef DFA_CBC
+#undef DFA_CWB
+}
+
+template <int D>
 hipError_t dispatch_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {
 #define DFA_B(AF, WB, BCv) return launch_blocked_t<D, AF, WB, BCv>(t, p, s)
 #define DFA_BBC(AF, WB) \
@@ -334,6 +440,27 @@ hipError_t launch_p1_atomic(const P1AtomicTopo& t, const P1Pass& p, hipStream_t 
     case 3: return dispatch_atomic<3>(t, p, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_p1_colored(const P1ColorTopo& t, const P1Pass& p, hipStream_t s) {
+  switch (p.gdim) {
+    case 1: return dispatch_colored<1>(t, p, s);
+    case 2: return dispatch_colored<2>(t, p, s);
+    case 3: return dispatch_colored<3>(t, p, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_p1_color_gather(int64_t nc, int32_t nvc, const int32_t* perm,
+                                  const int32_t* cells, const int32_t* cdofs,
+                                  const int32_t* kdofs, const int32_t* row_ptr,
+                                  const int32_t* col, int32_t* oc, int32_t* od, int32_t* ok,
+                                  uint8_t* orank, hipStream_t s) {
+  if (nc == 0) return hipSuccess;
+  const int64_t grid = (nc + 255) / 256;
+  hipLaunchKernelGGL(p1_color_gather_kernel, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s,
+                     nc, nvc, perm, cells, cdofs, kdofs, row_ptr, col, oc, od, ok, orank);
+  return hipGetLastError();
 }
 
 hipError_t launch_p1_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {

@@ -1171,3 +1171,62 @@ void build_vp2_block_plan(int64_t nc, const int32_t* cells, const int32_t* cd, c
 }
 
 }  // namespace dfa
+
+namespace dfa {
+
+// MeshColoring::color for the cell graphs (tdim, dim, tdim) (dolfin/mesh/MeshColoring.cpp:
+// 57-121): GraphBuilder::local_graph(mesh, tdim, dim) (dolfin/graph/GraphBuilder.cpp:124-153)
+// links two cells through a shared entity of dimension dim, and Boost's
+// sequential_vertex_coloring (called from dolfin/graph/BoostGraphColoring.h:87-113) visits
+// the cells in index order and gives each the smallest colour not taken by an already
+// coloured neighbour. In a simplicial mesh two cells share a vertex, an edge or a facet
+// exactly when they share at least 1, 2 or tdim vertices, so the entity graph reduces to
+// counting shared vertex ids through a vertex -> cells table. Sequential by definition.
+int32_t greedy_cell_coloring(int64_t nc, int32_t nvc, const int32_t* ce, int64_t ne,
+                             int32_t min_shared, int32_t* colors) {
+  if (nc == 0) return 0;
+  if (nvc < 1 || min_shared < 1) throw std::runtime_error("invalid colouring arguments");
+  std::vector<int64_t> ptr(static_cast<size_t>(ne) + 1, 0);
+  for (int64_t i = 0; i < nc * nvc; ++i) {
+    const int32_t e = ce[i];
+    if (e < 0 || e >= ne) throw std::runtime_error("cell entity id out of range");
+    ++ptr[static_cast<size_t>(e) + 1];
+  }
+  for (int64_t e = 0; e < ne; ++e) ptr[e + 1] += ptr[e];
+  std::vector<int32_t> adj(static_cast<size_t>(ptr[ne]));
+  {
+    std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t c = 0; c < nc; ++c)  // cells enter each list in ascending order
+      for (int32_t a = 0; a < nvc; ++a) adj[fill[ce[c * nvc + a]]++] = static_cast<int32_t>(c);
+  }
+  std::vector<int32_t> shared(static_cast<size_t>(nc), 0);  // count per candidate, reset after use
+  std::vector<int64_t> used;                                 // colour -> stamp (c + 1)
+  std::vector<int32_t> cand;
+  int32_t ncolors = 0;
+  for (int64_t c = 0; c < nc; ++c) {
+    cand.clear();
+    for (int32_t a = 0; a < nvc; ++a) {
+      const int32_t e = ce[c * nvc + a];
+      for (int64_t k = ptr[e]; k < ptr[e + 1]; ++k) {
+        const int32_t n = adj[k];
+        if (n >= c) break;  // lists are ascending: only coloured (lower) cells matter
+        if (shared[n]++ == 0) cand.push_back(n);
+      }
+    }
+    for (int32_t n : cand) {
+      if (shared[n] >= min_shared) {
+        const int32_t col = colors[n];
+        if (static_cast<size_t>(col) >= used.size()) used.resize(col + 1, 0);
+        used[col] = c + 1;
+      }
+      shared[n] = 0;
+    }
+    int32_t k = 0;
+    while (static_cast<size_t>(k) < used.size() && used[k] == c + 1) ++k;
+    colors[c] = k;
+    ncolors = std::max(ncolors, k + 1);
+  }
+  return ncolors;
+}
+
+}  // namespace dfa

@@ -75,6 +75,15 @@ class Engine:
         return {"plan_bytes": pb.value, "num_blocks": nb.value, "cell_appearances": ca.value,
                 "lds_bytes": lds.value}
 
+    def color_ptr(self):
+        """Offsets of each colour's cells in the coloured scatter (P1; built on first use)."""
+        L = _lib.load()
+        n = ctypes.c_int32()
+        _lib.check(L.dfa_color_info(self._h, ctypes.byref(n), None, _lib.stream()))
+        ptr = np.empty(n.value + 1, dtype=np.int64)
+        _lib.check(L.dfa_color_info(self._h, ctypes.byref(n), _lib.ptr(ptr), _lib.stream()))
+        return ptr
+
 
 def engine_for(V, keep_diagonal=False):
     key = bool(keep_diagonal)

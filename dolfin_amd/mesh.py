@@ -24,6 +24,7 @@ class Mesh:
             raise ValueError("cells must have gdim+1 vertices (simplices with tdim == gdim)")
         self._edges = None
         self._device = {}
+        self._coloring = {}
 
     def coordinates(self):
         return self._x
@@ -68,6 +69,11 @@ class Mesh:
             self._edges = (inv.reshape(-1, 6).astype(np.int32), int(uniq.shape[0]))
         return self._edges
 
+    def color(self, coloring_type="vertex"):
+        """Mesh::color (dolfin/mesh/Mesh.cpp:311-340): cell colours for a type name
+        ("vertex", "edge", "facet") or a tuple (tdim, dim, tdim); computed once and kept."""
+        return MeshColoring.color(self, MeshColoring.coloring_tuple(self, coloring_type))
+
     def coordinates_device(self, device="cuda"):
         import torch
         t = self._device.get(device)
@@ -79,6 +85,83 @@ class Mesh:
     def set_coordinates(self, x):
         self._x = np.ascontiguousarray(x, dtype=np.float64).reshape(self._x.shape)
         self._device = {}
+
+
+class MeshColoring:
+    """Greedy cell colouring, dolfin::MeshColoring (dolfin/mesh/MeshColoring.cpp:45-189).
+
+    Two cells are neighbours when they share an entity of dimension dim (the graph of
+    GraphBuilder::local_graph(mesh, tdim, dim), dolfin/graph/GraphBuilder.cpp:124-153); cells
+    are visited in index order and each takes the smallest colour no coloured neighbour holds
+    (Boost's sequential_vertex_coloring, dolfin/graph/BoostGraphColoring.h:87-113). Computed by
+    dfa_mesh_coloring in libdolfin_amd.so (host code; there is no Python fallback).
+    """
+
+    @staticmethod
+    def type_to_dim(coloring_type, mesh):
+        """MeshColoring::type_to_dim (MeshColoring.cpp:170-189)."""
+        if coloring_type not in ("vertex", "edge", "facet"):
+            raise RuntimeError(f'Unknown coloring type ("{coloring_type}"). Known options are '
+                               '"vertex", "edge" and "facet"')
+        return {"vertex": 0, "edge": 1}.get(coloring_type, mesh.gdim() - 1)
+
+    @staticmethod
+    def coloring_tuple(mesh, coloring_type):
+        if isinstance(coloring_type, str):
+            d = MeshColoring.type_to_dim(coloring_type, mesh)
+            return (mesh.gdim(), d, mesh.gdim())
+        t = tuple(int(v) for v in coloring_type)
+        if t[0] != t[-1]:
+            raise RuntimeError("Mesh coloring does not support dim i - j coloring")
+        if len(t) != 3 or t[0] != mesh.gdim() or not 0 <= t[1] <= mesh.gdim():
+            raise RuntimeError("cell colourings of type (tdim, dim, tdim) are implemented")
+        return t
+
+    @staticmethod
+    def compute_colors(mesh, coloring_type):
+        """Colours of the cells and their number (MeshColoring::compute_colors, :100-121)."""
+        from . import _lib
+        import ctypes
+        t = MeshColoring.coloring_tuple(mesh, coloring_type)
+        cells = np.ascontiguousarray(mesh.cells(), dtype=np.int32)
+        colors = np.empty(mesh.num_cells(), dtype=np.int32)
+        n = ctypes.c_int32(0)
+        _lib.check(_lib.load().dfa_mesh_coloring(mesh.num_cells(), cells.shape[1], _lib.ptr(cells),
+                                                 mesh.num_vertices(), t[1], _lib.ptr(colors),
+                                                 ctypes.byref(n)))
+        return colors.astype(np.int64), n.value
+
+    @staticmethod
+    def color(mesh, coloring_type):
+        """MeshColoring::color (:57-98): computes, stores on the mesh and returns the colours."""
+        t = MeshColoring.coloring_tuple(mesh, coloring_type)
+        if t not in mesh._coloring:
+            colors, n = MeshColoring.compute_colors(mesh, t)
+            order = np.argsort(colors, kind="stable")
+            bounds = np.searchsorted(colors[order], np.arange(n + 1))
+            mesh._coloring[t] = (colors, [order[bounds[k]:bounds[k + 1]] for k in range(n)])
+        return mesh._coloring[t][0]
+
+    @staticmethod
+    def color_cells(mesh, coloring_type="vertex"):
+        """MeshColoring::color_cells (:45-56)."""
+        return MeshColoring.color(mesh, MeshColoring.coloring_tuple(mesh, coloring_type))
+
+    @staticmethod
+    def entities_of_color(mesh, coloring_type):
+        """Cells of each colour, ascending (the second member of the colouring data kept in
+        MeshTopology::coloring, MeshColoring.cpp:88-95)."""
+        t = MeshColoring.coloring_tuple(mesh, coloring_type)
+        MeshColoring.color(mesh, t)
+        return mesh._coloring[t][1]
+
+    @staticmethod
+    def cell_colors(mesh, coloring_type):
+        """MeshColoring::cell_colors (:126-168): the stored colours; raises if not computed."""
+        t = MeshColoring.coloring_tuple(mesh, coloring_type)
+        if t not in mesh._coloring:
+            raise RuntimeError("Requested coloring has not been computed")
+        return mesh._coloring[t][0].copy()
 
 
 def _lin(n):

@@ -56,6 +56,9 @@ extern "C" {
 #define DFA_SCATTER_ATOMIC 1   /* one thread per cell, FP64 atomics into CSR           */
 #define DFA_SCATTER_BLOCKED 2  /* row blocks owned by workgroups, LDS accumulation,
                                   each CSR value written once, no global atomics       */
+#define DFA_SCATTER_COLORED 3  /* P1 only: cells in colours that share no dof (greedy
+                                  colouring as MeshColoring::color), one launch per
+                                  colour, thread per cell, CSR read-add-write, no atomics */
 
 /* Flags of AssemblerBase (dolfin/fem/AssemblerBase.h:46-67). */
 #define DFA_ADD_VALUES 1       /* do not zero the tensor before assembly               */
@@ -231,6 +234,27 @@ int dfa_csr_spmv(dfa_assembler h, const double* d_values, double alpha, const do
 /* The diagonal of that matrix (0 where the pattern has no diagonal entry): the Jacobi
    preconditioner of EigenKrylovSolver (dolfin/la/EigenKrylovSolver.cpp:52-56). */
 int dfa_csr_diagonal(dfa_assembler h, const double* d_values, double* d_diag, void* stream);
+
+/*
+ * Greedy colouring of mesh cells: MeshColoring::color(mesh, {tdim, entity_dim, tdim}) /
+ * MeshColoring::color_cells(mesh, "vertex" | "edge" | "facet") (dolfin/mesh/
+ * MeshColoring.cpp:45-121). Two cells are neighbours when they share an entity of dimension
+ * entity_dim (GraphBuilder::local_graph(mesh, tdim, entity_dim), dolfin/graph/
+ * GraphBuilder.cpp:124-153); cells are visited in index order and each takes the smallest
+ * colour no coloured neighbour holds (Boost's sequential_vertex_coloring, called from
+ * dolfin/graph/BoostGraphColoring.h:87-113). Simplicial meshes: h_cells [num_cells x nvc]
+ * vertex ids, nvc = tdim + 1, 0 <= entity_dim <= tdim. Writes h_colors [num_cells] and the
+ * number of colours. Host computation.
+ */
+int dfa_mesh_coloring(int64_t num_cells, int32_t nvc, const int32_t* h_cells, int64_t num_vertices,
+                      int32_t entity_dim, int32_t* h_colors, int32_t* num_colors);
+
+/*
+ * Colours of a P1 handle's coloured scatter (DFA_SCATTER_COLORED; built on first use): cells
+ * sharing a dof get different colours. Writes the number of colours and, when h_color_ptr is
+ * non-NULL (room for num_colors + 1 entries), the offsets of each colour's cells.
+ */
+int dfa_color_info(dfa_assembler h, int32_t* num_colors, int64_t* h_color_ptr, void* stream);
 
 #ifdef __cplusplus
 }

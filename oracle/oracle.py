@@ -464,3 +464,45 @@ def bandwidth(adj, old_to_new=None):
     """max |new(u) - new(v)| over the edges."""
     p = np.arange(len(adj)) if old_to_new is None else np.asarray(old_to_new)
     return max((abs(int(p[u]) - int(p[v])) for u in range(len(adj)) for v in adj[u]), default=0)
+
+
+# ---- cell colouring (pure Python; small meshes) ---------------------------------------------
+def cell_entity_graph(cells, dim):
+    """GraphBuilder::local_graph(mesh, tdim, dim) (dolfin/graph/GraphBuilder.cpp:124-153) for a
+    simplicial mesh: the entities of dimension dim of a cell are its (dim+1)-vertex subsets
+    (as sorted tuples, unique per vertex set as in TopologyComputation), and a cell's
+    neighbours are the other cells holding one of its entities. Returns a list of sets."""
+    from itertools import combinations
+    cells = np.asarray(cells)
+    owners = {}
+    for c, vs in enumerate(cells.tolist()):
+        for e in combinations(sorted(vs), dim + 1):
+            owners.setdefault(e, []).append(c)
+    graph = [set() for _ in range(len(cells))]
+    for c, vs in enumerate(cells.tolist()):
+        for e in combinations(sorted(vs), dim + 1):
+            graph[c].update(n for n in owners[e] if n != c)
+    return graph
+
+
+def sequential_vertex_coloring(graph):
+    """Boost's sequential_vertex_coloring (boost/graph/sequential_vertex_coloring.hpp, Boost
+    1.56+; called from dolfin/graph/BoostGraphColoring.h:112-113), restated from its published
+    description: every vertex starts with the colour V - 1, vertices are visited in index
+    order, the colours of all neighbours are marked, and the vertex takes the smallest
+    unmarked colour below the current number of colours, or opens a new one. Returns the
+    colours and their number."""
+    V = len(graph)
+    color = [V - 1] * V
+    mark = [-1] * max(V, 1)
+    max_color = 0
+    for i in range(V):
+        for u in graph[i]:
+            mark[color[u]] = i
+        k = 0
+        while k < max_color and mark[k] == i:
+            k += 1
+        if k == max_color:
+            max_color += 1
+        color[i] = k
+    return np.array(color, dtype=np.int64), max_color

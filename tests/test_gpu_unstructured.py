@@ -30,7 +30,7 @@ def _delaunay(gdim, n, seed):
 
 
 @pytest.mark.parametrize("gdim,n", [(2, 300), (3, 500)])
-@pytest.mark.parametrize("scatter", [dfa.SCATTER_ATOMIC, dfa.SCATTER_BLOCKED])
+@pytest.mark.parametrize("scatter", [dfa.SCATTER_ATOMIC, dfa.SCATTER_BLOCKED, dfa.SCATTER_COLORED])
 def test_p1_unstructured_vs_oracle(oracle, gdim, n, scatter):
     print(f"case unstructured p1 gdim={gdim} n={n} scatter={scatter}", flush=True)
     mesh, rng = _delaunay(gdim, n, seed=gdim * 100 + n)
