@@ -117,6 +117,20 @@ def bench_batch(nmembers, log, seed=5):
             "grid_sweep_ms": {str(k): v for k, v in gtimes.items()}, "setup_s": t_setup}
 
 
+def device_info(dev):
+    """Name, CUs, clocks, memory bus and sizes from hipDeviceGetAttribute (through the
+    library), beside the HBM peak the roofline uses: MI355X's published 8 TB/s, i.e. the
+    8192-bit bus at 8 Gb/s per pin."""
+    import ctypes
+    from dolfin_amd import _lib
+    v = (ctypes.c_int64 * 6)()
+    _lib.check(_lib.load().dfa_device_info(0, ctypes.cast(v, ctypes.c_void_p)))
+    return {"name": dev.name, "arch": getattr(dev, "gcnArchName", ""), "compute_units": v[0],
+            "engine_clock_mhz": v[1] / 1e3, "memory_clock_mhz": v[2] / 1e3,
+            "memory_bus_bits": v[3], "l2_bytes": v[4], "memory_bytes": v[5],
+            "hbm_peak_used_GBs": HBM_PEAK_GBS}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -181,10 +195,11 @@ def main():
                                f"{args.n}^3 tets ({nc} cells, {V.dim()} dofs, nnz {eng.nnz}), "
                                f"fused A+b re-assembly, {used_scatter} scatter",
                    "cells": nc, "dofs": V.dim(), "nnz": eng.nnz, "scatter": used_scatter,
-                   "plan": info, "device": dev.name},
+                   "plan": info, "device": device_info(dev)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": alg_bytes, "kernel_ms": ev_ms},
+                     "algorithmic_bytes": alg_bytes, "kernel_ms": ev_ms,
+                     "kernel_ms_median": float(np.median(ev))},
         "first_assembly_s": t_first,
     }
 
@@ -270,8 +285,14 @@ def main():
                 strat[name]["traffic"] = load_traffic(f"p1_atomic_{args.n}")
             log(f"{name} scatter timed")
         asm.assemble(A, b)
+        from dolfin_amd import autotune
+        pick, ttimes = autotune.tune_scatter([a], reps=3)
+        names = {dfa.SCATTER_BLOCKED: "blocked", dfa.SCATTER_COLORED: "colored",
+                 dfa.SCATTER_ATOMIC: "atomic"}
         sec["p1_scatter_strategies"] = dict(
-            strat, workload=f"P1 {args.n}^3 A+b, same inputs as the headline, other scatters")
+            strat, workload=f"P1 {args.n}^3 A+b, same inputs as the headline, other scatters",
+            autotuned={"choice": names[pick],
+                       "matrix_ms": {names[k]: v for k, v in ttimes.items()}})
         # the same workload with DOLFIN's default numbering (reverse Cuthill-McKee nodes)
         t = time.perf_counter()
         Vr = dfa.FunctionSpace(mesh, "P", 1, reorder=True)

@@ -37,7 +37,8 @@ EXPORTS = ["dfa_last_error", "dfa_version", "dfa_assembler_create", "dfa_assembl
            "dfa_assemble_system", "dfa_adjoint", "dfa_plan_info", "dfa_set_plan_defaults",
            "dfa_batch_create", "dfa_batch_launch", "dfa_batch_info", "dfa_batch_destroy",
            "dfa_exterior_facets", "dfa_assemble_facets", "dfa_csr_spmv", "dfa_csr_diagonal",
-           "dfa_node_reordering", "dfa_graph_reordering", "dfa_mesh_coloring", "dfa_color_info"]
+           "dfa_node_reordering", "dfa_graph_reordering", "dfa_mesh_coloring", "dfa_color_info",
+           "dfa_device_info"]
 
 _lib = None
 
@@ -83,6 +84,7 @@ def load():
     L.dfa_graph_reordering.argtypes = [_i64, _vp, _vp, _vp]
     L.dfa_mesh_coloring.argtypes = [_i64, _i32, _vp, _i64, _i32, _vp, ctypes.POINTER(_i32)]
     L.dfa_color_info.argtypes = [_vp, ctypes.POINTER(_i32), _vp, _vp]
+    L.dfa_device_info.argtypes = [_i32, _vp]
     L.dfa_assemble_facets.argtypes = [_vp, _i32, _i32, _vp, ctypes.c_double, _vp, _vp, _vp, _vp,
                                       _vp, _vp, _vp]
     for name in EXPORTS:

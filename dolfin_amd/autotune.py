@@ -1,8 +1,9 @@
 """Autotuner for the blocked-scatter plans and the batched launch.
 
-The tunables are the plan granularity (P1 rows per block, P1 workgroup size, vector-P2 LDS
-values per block -- i.e. cells per workgroup) and the launch mode of the batched kernels
-(one workgroup per row block, or a number of persistent workgroups per CU). Candidates are timed with HIP events on a sample of the
+The tunables are the scatter strategy (owner-computes blocks, graph colouring, global
+atomics), the plan granularity (P1 rows per block, P1 workgroup size, vector-P2 LDS values
+per block -- i.e. cells per workgroup) and the launch mode of the batched kernels (one
+workgroup per row block, or a number of persistent workgroups per CU). Candidates are timed with HIP events on a sample of the
 workload and the fastest is returned; results are cached per (kind, size class).
 """
 import ctypes
@@ -63,6 +64,29 @@ def tune_forms(make_forms, candidates=None, reps=5):
             times[cand] = _time(run, reps)
         finally:
             set_plan_defaults(0, 0, 0)
+    best = min(times, key=times.get)
+    return best, times
+
+
+def tune_scatter(forms, scatters=None, reps=5):
+    """Time the assembly of `forms` (rank-2 forms, each into its own matrix) with every
+    scatter strategy the element supports and return (fastest, {strategy: ms}). Plans are
+    built by an untimed first call."""
+    from .fem import Assembler
+    from .la import Matrix
+    if scatters is None:
+        p1 = all(f.V.element == _lib.ELEMENT_P1 for f in forms)
+        scatters = ((_lib.SCATTER_BLOCKED, _lib.SCATTER_COLORED, _lib.SCATTER_ATOMIC) if p1
+                    else (_lib.SCATTER_BLOCKED, _lib.SCATTER_ATOMIC))
+    times = {}
+    for sc in scatters:
+        tensors = [Matrix() for _ in forms]
+        asm = Assembler(sc)
+
+        def run():
+            for f, A in zip(forms, tensors):
+                asm.assemble(A, f)
+        times[sc] = _time(run, reps)
     best = min(times, key=times.get)
     return best, times
 

@@ -825,6 +825,25 @@ int dfa_plan_info(dfa_assembler h, int32_t scatter, int64_t* plan_bytes, int64_t
   });
 }
 
+int dfa_device_info(int32_t device, int64_t* info) {
+  return guarded([&]() -> int {
+    if (!info) return fail("info is NULL", DFA_ERR_ARG);
+    const hipDeviceAttribute_t attrs[5] = {
+        hipDeviceAttributeMultiprocessorCount, hipDeviceAttributeClockRate,
+        hipDeviceAttributeMemoryClockRate, hipDeviceAttributeMemoryBusWidth,
+        hipDeviceAttributeL2CacheSize};
+    for (int i = 0; i < 5; ++i) {
+      int v = 0;
+      hip_check(hipDeviceGetAttribute(&v, attrs[i], device), "hipDeviceGetAttribute");
+      info[i] = v;
+    }
+    size_t mem = 0;
+    hip_check(hipDeviceTotalMem(&mem, device), "hipDeviceTotalMem");
+    info[5] = static_cast<int64_t>(mem);
+    return DFA_OK;
+  });
+}
+
 int dfa_mesh_coloring(int64_t num_cells, int32_t nvc, const int32_t* h_cells, int64_t num_vertices,
                       int32_t entity_dim, int32_t* h_colors, int32_t* num_colors) {
   return guarded([&]() -> int {

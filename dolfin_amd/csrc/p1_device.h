@@ -217,7 +217,9 @@ __device__ __forceinline__ int xcd_block(int nb) {
 }
 
 // Prologue shared by the assembly and adjoint kernels: gather the block's nodes into LDS.
-template <int D, int T, bool NEED_K, bool NEED_F, bool BC, bool OWN, bool WIDE>
+// ZF (wide tetrahedral records without Dirichlet data): x, y, z, kappa, z, f, so that z and
+// f come with one 128-bit read.
+template <int D, int T, bool NEED_K, bool NEED_F, bool BC, bool OWN, bool WIDE, bool ZF = false>
 __device__ __forceinline__ void gather_nodes(const P1BlockTopo& t, const double* __restrict__ x,
                                              const double* __restrict__ kappa,
                                              const double* __restrict__ f,
@@ -237,7 +239,8 @@ __device__ __forceinline__ void gather_nodes(const P1BlockTopo& t, const double*
       const int32_t kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + i];
       if constexpr (NEED_K) rec[D] = kappa[kd];
       if constexpr (NEED_F) {
-        if constexpr (WIDE) rec[D + 1] = f[kd];
+        if constexpr (ZF) rec[D + 2] = f[kd];
+        else if constexpr (WIDE) rec[D + 1] = f[kd];
         else sf[i] = f[kd];
       }
     }
@@ -249,6 +252,7 @@ __device__ __forceinline__ void gather_nodes(const P1BlockTopo& t, const double*
       if constexpr (WIDE) rec[D + 2] = g;
       else sbcv[i] = g;
     }
+    if constexpr (ZF) rec[D + 1] = rec[D - 1];
 #pragma unroll
     for (int k = 0; k < NS; k += 2)
       *reinterpret_cast<double2*>(snode + i * NS + k) = make_double2(rec[k], rec[k + 1]);
@@ -589,45 +593,71 @@ __device__ __forceinline__ void chain_walk_matrix(const uint32_t (&d)[kChainWord
   emit(lp, lq, rank_byte(20), apq);
 }
 
-// The load-vector walk of one chain: b_a += |det| / 120 (f_a + sum of the cell's f). The
-// contributions to p and q are kept as sum(w F) + f_{p,q} sum(w) until the chain ends; those
-// to r_k are complete when r_k leaves the window. A second, light pass over the same chain
-// keeps the matrix walk's register footprint (and so the workgroups per CU) unchanged.
-__device__ __forceinline__ void chain_walk_vector(const uint32_t (&d)[kChainWords],
-                                                  const double* snode, double* sbv, uint32_t nr) {
+// Matrix and load vector in one walk: the same node reads serve both (x, y | z, kappa | z, f
+// records: three 128-bit reads per node instead of four over two walks), and |det| is
+// computed once per cell.
+__device__ __forceinline__ void chain_walk_fused(const uint32_t (&d)[kChainWords],
+                                                 const double* snode, double* svals, double* sbv,
+                                                 uint32_t nr) {
   const uint32_t lp = d[0] & 0xffffu, lq = d[0] >> 16;
   const int m = static_cast<int>(d[4] >> 16);
   auto rid = [&](int k) { return (d[1 + k / 2] >> (16 * (k & 1))) & 0xffffu; };
+  auto rank_byte = [&](int j) { return (d[5 + j / 4] >> (8 * (j % 4))) & 0xffu; };
+  auto emit = [&](uint32_t u, uint32_t v, uint32_t byte, double w) {
+    if (u < nr) atomic_add_lds(svals + u * kRowStride + (byte & 15u), w);
+    if (v < nr) atomic_add_lds(svals + v * kRowStride + (byte >> 4), w);
+  };
   auto emit_b = [&](uint32_t u, double w) {
     if (u < nr) atomic_add_lds(sbv + u, w);
   };
   auto xy = [&](uint32_t l) { return *reinterpret_cast<const double2*>(snode + l * 6); };
-  auto zc = [&](uint32_t l) { return snode[l * 6 + 2]; };
-  auto fv = [&](uint32_t l) { return snode[l * 6 + 4]; };
-  const double2 pxy = xy(lp), qxy = xy(lq);
-  const double px = pxy.x, py = pxy.y, pz = zc(lp);
-  const double e1x = qxy.x - px, e1y = qxy.y - py, e1z = zc(lq) - pz;
-  const double fp = fv(lp), fq = fv(lq), fpq = fp + fq;
+  auto zk = [&](uint32_t l) { return *reinterpret_cast<const double2*>(snode + l * 6 + 2); };
+  auto fv = [&](uint32_t l) { return snode[l * 6 + 5]; };
+
+  const double2 pxy = xy(lp), pzk = zk(lp), qxy = xy(lq), qzk = zk(lq);
+  const double px = pxy.x, py = pxy.y, pz = pzk.x;
+  const double e1x = qxy.x - px, e1y = qxy.y - py, e1z = qzk.x - pz;
+  const double kpq = pzk.y + qzk.y;
+  const double fpq = fv(lp) + fv(lq);
   uint32_t cur = rid(0);
-  double e2x, e2y, e2z, fr;
+  double e2x, e2y, e2z, kr, fr;
   {
-    const double2 rxy = xy(cur);
+    const double2 rxy = xy(cur), rzk = zk(cur);
     e2x = rxy.x - px;
     e2y = rxy.y - py;
-    e2z = zc(cur) - pz;
+    e2z = rzk.x - pz;
+    kr = rzk.y;
     fr = fv(cur);
   }
-  double swf = 0.0, sw = 0.0, cb = 0.0;
+  double apq = 0.0, cp = 0.0, cq = 0.0, swf = 0.0, sw = 0.0, cb = 0.0;
 #pragma unroll
   for (int k = 0; k < kChainLen; ++k) {
     if (k < m) {
       const uint32_t ln = rid(k + 1);
-      const double2 nxy = xy(ln);
-      const double e3x = nxy.x - px, e3y = nxy.y - py, e3z = zc(ln) - pz;
+      const double2 nxy = xy(ln), nzk = zk(ln);
       const double fn = fv(ln);
-      const double det = e1x * (e2y * e3z - e2z * e3y) + e1y * (e2z * e3x - e2x * e3z) +
-                         e1z * (e2x * e3y - e2y * e3x);
-      const double wb = fabs(det) * (1.0 / 120.0);
+      const double e3x = nxy.x - px, e3y = nxy.y - py, e3z = nzk.x - pz;
+      const double g1x = e2y * e3z - e2z * e3y, g1y = e2z * e3x - e2x * e3z, g1z = e2x * e3y - e2y * e3x;
+      const double g2x = e3y * e1z - e3z * e1y, g2y = e3z * e1x - e3x * e1z, g2z = e3x * e1y - e3y * e1x;
+      const double g3x = e1y * e2z - e1z * e2y, g3y = e1z * e2x - e1x * e2z, g3z = e1x * e2y - e1y * e2x;
+      const double adet = fabs(e1x * g1x + e1y * g1y + e1z * g1z);
+      const double w = (kpq + kr + nzk.y) * (1.0 / 24.0) * recip(adet);
+      const double g11 = fma(g1x, g1x, fma(g1y, g1y, g1z * g1z));
+      const double g22 = fma(g2x, g2x, fma(g2y, g2y, g2z * g2z));
+      const double g33 = fma(g3x, g3x, fma(g3y, g3y, g3z * g3z));
+      const double g12 = fma(g1x, g2x, fma(g1y, g2y, g1z * g2z));
+      const double g13 = fma(g1x, g3x, fma(g1y, g3y, g1z * g3z));
+      const double g23 = fma(g2x, g3x, fma(g2y, g3y, g2z * g3z));
+      apq -= w * (g11 + g12 + g13);
+      cp -= w * (g12 + g22 + g23);
+      cq += w * g12;
+      emit(lp, cur, rank_byte(3 * k), cp);
+      emit(lq, cur, rank_byte(3 * k + 1), cq);
+      emit(cur, ln, rank_byte(3 * k + 2), w * g23);
+      cp = -w * (g13 + g23 + g33);
+      cq = w * g13;
+      // load vector: b_a += |det| / 120 (f_a + sum of the cell's f)
+      const double wb = adet * (1.0 / 120.0);
       const double F = fpq + fr + fn;
       swf = fma(wb, F, swf);
       sw += wb;
@@ -636,13 +666,17 @@ __device__ __forceinline__ void chain_walk_vector(const uint32_t (&d)[kChainWord
       e2x = e3x;
       e2y = e3y;
       e2z = e3z;
+      kr = nzk.y;
       fr = fn;
       cur = ln;
     }
   }
+  emit(lp, cur, rank_byte(18), cp);
+  emit(lq, cur, rank_byte(19), cq);
+  emit(lp, lq, rank_byte(20), apq);
   emit_b(cur, cb);
-  emit_b(lp, fma(fp, sw, swf));
-  emit_b(lq, fma(fq, sw, swf));
+  emit_b(lp, fma(fv(lp), sw, swf));
+  emit_b(lq, fma(fv(lq), sw, swf));
 }
 
 template <bool WANT_B, int T, int RR>
@@ -653,7 +687,6 @@ __device__ __forceinline__ void p1_chain_body(const P1BlockTopo& t, const P1Pass
   double* sbv = reinterpret_cast<double*>(lds + L.bv);
   int32_t* srdof = reinterpret_cast<int32_t*>(lds + L.rdof);
   int32_t* srbase = reinterpret_cast<int32_t*>(lds + L.rbase);
-  int8_t* sdiag = reinterpret_cast<int8_t*>(lds + L.diag);
   const int tid = threadIdx.x;
   // chain records first, so that their latency overlaps the node gather
   const int c0 = t.chain_ptr[blk], nch = t.chain_ptr[blk + 1] - c0;
@@ -671,37 +704,38 @@ __device__ __forceinline__ void p1_chain_body(const P1BlockTopo& t, const P1Pass
       d[j][4 * h + 3] = w.w;
     }
   }
-  gather_nodes<3, T, true, WANT_B, false, false, true>(t, p.x, p.kappa, p.f, nullptr, nullptr, blk,
-                                                       snode, nullptr, nullptr, nullptr, nullptr);
+  gather_nodes<3, T, true, WANT_B, false, false, true, true>(t, p.x, p.kappa, p.f, nullptr, nullptr,
+                                                             blk, snode, nullptr, nullptr, nullptr,
+                                                             nullptr);
   const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
   for (int i = tid; i < (nr * kRowStride + 1) / 2; i += T)
     reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
+  // one 8-byte word per owned row: CSR start, row length | diagonal position << 8
+  int2* srow = reinterpret_cast<int2*>(srbase);
   for (int i = tid; i < nr; i += T) {
     const int32_t dof = t.row_dof[r0 + i];
     srdof[i] = dof;
     const int32_t b0 = t.csr_row_ptr[dof];
-    srbase[i] = b0;
-    srbase[nr + i] = t.csr_row_ptr[dof + 1] - b0;
-    sdiag[i] = static_cast<int8_t>(t.row_diag[r0 + i]);
+    srow[i] = make_int2(b0, (t.csr_row_ptr[dof + 1] - b0) | (static_cast<int>(t.row_diag[r0 + i]) << 8));
     if constexpr (WANT_B) sbv[i] = 0.0;
   }
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < RR; ++j)
     if (tid + j * T < nch) {
-      chain_walk_matrix(d[j], snode, svals, static_cast<uint32_t>(nr));
-      if constexpr (WANT_B) chain_walk_vector(d[j], snode, sbv, static_cast<uint32_t>(nr));
+      if constexpr (WANT_B) chain_walk_fused(d[j], snode, svals, sbv, static_cast<uint32_t>(nr));
+      else chain_walk_matrix(d[j], snode, svals, static_cast<uint32_t>(nr));
     }
   __syncthreads();
 
   // write-out: 16 lanes per row, diagonal = minus the row sum
   const int grp = tid >> 4, ln = tid & 15;
   for (int i = grp; i < nr; i += T / 16) {
-    const int base = srbase[i];
-    const int len = srbase[nr + i];
+    const int2 rw = srow[i];
+    const int base = rw.x, len = rw.y & 0xff;
     double v = ln < len ? svals[i * kRowStride + ln] : 0.0;
     const double rs = row16_sum(v);
-    if (ln == sdiag[i]) v = -rs;
+    if (ln == (rw.y >> 8)) v = -rs;
     if (ln < len) {
       if (p.add) v += p.values[base + ln];
       p.values[base + ln] = v;

@@ -161,7 +161,7 @@ __global__ __launch_bounds__(T) void p1_blocked_kernel(P1BlockTopo t, P1Pass p, 
 }
 
 // 512 threads, one chain each: six waves per SIMD, i.e. three resident workgroups per CU (at
-// most 80 VGPRs; the load vector takes a second, light walk for that reason).
+// most 80 VGPRs, matrix and load vector walked together).
 template <bool WANT_B, int T, int RR>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(RR == 1 && T == 512 ? 6 : 1)))
 void p1_chain_kernel(P1BlockTopo t, P1Pass p, LdsLayout L) {

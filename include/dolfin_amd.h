@@ -235,6 +235,11 @@ int dfa_csr_spmv(dfa_assembler h, const double* d_values, double alpha, const do
    preconditioner of EigenKrylovSolver (dolfin/la/EigenKrylovSolver.cpp:52-56). */
 int dfa_csr_diagonal(dfa_assembler h, const double* d_values, double* d_diag, void* stream);
 
+/* Device properties the roofline figures rest on (no counterpart in the reference):
+   info[0] compute units, [1] peak engine clock (kHz), [2] peak memory clock (kHz),
+   [3] memory bus width (bits), [4] L2 size (bytes), [5] device memory (bytes). */
+int dfa_device_info(int32_t device, int64_t* info);
+
 /*
  * Greedy colouring of mesh cells: MeshColoring::color(mesh, {tdim, entity_dim, tdim}) /
  * MeshColoring::color_cells(mesh, "vertex" | "edge" | "facet") (dolfin/mesh/

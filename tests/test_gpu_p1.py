@@ -263,3 +263,19 @@ def test_blocked_equals_atomic_128_properties():
     ptr = dfa.engine_for(V).color_ptr()
     assert ptr[0] == 0 and ptr[-1] == mesh.num_cells() and np.all(np.diff(ptr) > 0)
     assert 24 <= len(ptr) - 1 <= 64  # at least the 24 cells around an interior vertex
+
+
+def test_scatter_autotuner_picks_a_strategy():
+    """autotune.tune_scatter times every P1 strategy on the forms and returns the fastest;
+    the three strategies give the same matrix."""
+    from dolfin_amd import autotune
+    mesh, V, kappa, f = workloads.p1_poisson_3d(16, seed=3)
+    print("case scatter autotuner n=16", flush=True)
+    a = dfa.poisson_form(V, workloads.to_function(V, kappa))
+    best, times = autotune.tune_scatter([a], reps=2)
+    assert set(times) == set(SCATTERS) and best in SCATTERS
+    assert times[best] == min(times.values()) and all(t > 0 for t in times.values())
+    ref = dfa.assemble(a, scatter=dfa.SCATTER_ATOMIC).values
+    for sc in SCATTERS:
+        v = dfa.assemble(a, scatter=sc).values
+        assert (torch.linalg.norm(v - ref) / torch.linalg.norm(ref)).item() < TOL
