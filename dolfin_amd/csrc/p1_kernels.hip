@@ -177,7 +177,9 @@ __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTop
                                                                       int lds_node, int lds_f,
                                                                       int lds_u, int lds_acc) {
   constexpr int N = D + 1;
-  constexpr int NS = node_stride(D);
+  // tetrahedra: 48-byte node records x, y | z, kappa | f, u (three 128-bit reads per node)
+  constexpr bool W = D == 3;
+  constexpr int NS = node_stride(D, W);
   extern __shared__ __align__(16) unsigned char lds[];
   double* snode = reinterpret_cast<double*>(lds + lds_node);
   double* sf = reinterpret_cast<double*>(lds + lds_f);
@@ -187,12 +189,16 @@ __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTop
 
   const int blk = xcd_block(t.num_blocks);
   const int tid = threadIdx.x;
-  gather_nodes<D, kThreads, true, true, false, false, false>(t, a.x, a.kappa, a.f, nullptr,
-                                                             nullptr, blk, snode, sf, nullptr,
-                                                             nullptr, nullptr);
+  gather_nodes<D, kThreads, true, true, false, false, W>(t, a.x, a.kappa, a.f, nullptr, nullptr,
+                                                         blk, snode, sf, nullptr, nullptr, nullptr);
   {
+    // same node-to-thread mapping as the gather: each thread completes its own records
     const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
-    for (int i = tid; i < nn; i += kThreads) su[i] = a.u[t.node_dof[n0 + i]];
+    for (int i = tid; i < nn; i += kThreads) {
+      const double uv = a.u[t.node_dof[n0 + i]];
+      if constexpr (W) snode[i * NS + 5] = uv;
+      else su[i] = uv;
+    }
   }
   const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
   for (int i = tid; i < nr * (1 + D); i += kThreads) sacc[i] = 0.0;
@@ -203,11 +209,19 @@ __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTop
   for (int k = tid; k < ncb; k += kThreads) {
     const uint64_t ids = t.cell_rec[2 * (c0 + k)];
     int l[N];
-    double pc[N][D], kap[N];
+    double pc[N][D], kap[N], un[N], fn[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) {
       l[q] = static_cast<int>((ids >> (16 * q)) & 0xffffu);
       load_node<D>(snode + l[q] * NS, pc[q], kap[q]);
+      if constexpr (W) {
+        const double2 fu = *reinterpret_cast<const double2*>(snode + l[q] * NS + 4);
+        fn[q] = fu.x;
+        un[q] = fu.y;
+      } else {
+        fn[q] = sf[l[q]];
+        un[q] = su[l[q]];
+      }
     }
     double g[N][D], absdet;
     p1_geometry<D>(pc, g, absdet);
@@ -235,7 +249,7 @@ __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTop
     double gu[D] = {}, ks = 0.0, us = 0.0, fs = 0.0, uf = 0.0;
 #pragma unroll
     for (int q = 0; q < N; ++q) {
-      const double uq = su[l[q]], fq = sf[l[q]];
+      const double uq = un[q], fq = fn[q];
 #pragma unroll
       for (int i = 0; i < D; ++i) gu[i] += uq * G[q][i];
       ks += kap[q];
@@ -481,11 +495,12 @@ size_t p1_blocked_lds_bytes(const P1BlockTopo& t, const P1Pass& p) {
 namespace {
 template <int D>
 hipError_t launch_adjoint_blocked_t(const P1BlockTopo& t, const AdjointP1& a, hipStream_t s) {
-  constexpr int NS = node_stride(D);
+  constexpr bool W = D == 3;
+  constexpr int NS = node_stride(D, W);
   int o = 0;
   const int node = o; o += align16(t.max_nodes * NS * 8);
-  const int f = o;    o += align16(t.max_nodes * 8);
-  const int u = o;    o += align16(t.max_nodes * 8);
+  const int f = o;    o += W ? 0 : align16(t.max_nodes * 8);
+  const int u = o;    o += W ? 0 : align16(t.max_nodes * 8);
   const int acc = o;  o += align16(t.max_rows * (1 + D) * 8);
   auto kern = p1_adjoint_blocked_kernel<D>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
