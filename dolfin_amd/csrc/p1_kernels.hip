@@ -299,6 +299,160 @@ __global__ __launch_bounds__(kThreads) void p1_adjoint_blocked_kernel(P1BlockTop
   }
 }
 
+// The adjoint over the edge chains of the forward kernel (tetrahedra): a lane walks the cells
+// (p, q, r_k, r_{k+1}) around its edge, keeps the running dJ/dkappa and dJ/dx of p and q in
+// registers for the whole chain and those of r_k for its two cells, and adds each finished
+// sum once to the owned node's LDS accumulator (4 LDS atomics per node instead of 4 per cell
+// and vertex). Node records: x, y | z, kappa | f, u. J counts a cell once over all blocks by
+// weighting it with (owned vertices of the cell) / 4 in every block it touches.
+constexpr int kAdjChainThreads = 512;
+__global__ __launch_bounds__(kAdjChainThreads) void p1_adjoint_chain_kernel(P1BlockTopo t,
+                                                                            AdjointP1 a,
+                                                                            int lds_acc) {
+  constexpr int T = kAdjChainThreads;
+  extern __shared__ __align__(16) unsigned char lds[];
+  double* snode = reinterpret_cast<double*>(lds);
+  double* sacc = reinterpret_cast<double*>(lds + lds_acc);  // [rows][4]: dJ/dkappa, dJ/dx
+  __shared__ double sJ[T / 64];
+  const int blk = xcd_block(t.num_blocks);
+  const int tid = threadIdx.x;
+  gather_nodes<3, T, true, true, false, false, true>(t, a.x, a.kappa, a.f, nullptr, nullptr, blk,
+                                                     snode, nullptr, nullptr, nullptr, nullptr);
+  {
+    const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
+    for (int i = tid; i < nn; i += T) snode[i * 6 + 5] = a.u[t.node_dof[n0 + i]];
+  }
+  const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
+  for (int i = tid; i < nr * 4; i += T) sacc[i] = 0.0;
+  __syncthreads();
+
+  auto rec = [&](uint32_t l, double (&x)[3], double& k, double& f, double& u) {
+    const double2 a0 = *reinterpret_cast<const double2*>(snode + l * 6);
+    const double2 a1 = *reinterpret_cast<const double2*>(snode + l * 6 + 2);
+    const double2 a2 = *reinterpret_cast<const double2*>(snode + l * 6 + 4);
+    x[0] = a0.x;
+    x[1] = a0.y;
+    x[2] = a1.x;
+    k = a1.y;
+    f = a2.x;
+    u = a2.y;
+  };
+  auto emit = [&](uint32_t l, double dk, const double (&dx)[3]) {
+    if (l < static_cast<uint32_t>(nr)) {
+      atomic_add_lds(sacc + 4 * l, dk);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) atomic_add_lds(sacc + 4 * l + 1 + i, dx[i]);
+    }
+  };
+  double Jl = 0.0;
+  const int c0 = t.chain_ptr[blk], nch = t.chain_ptr[blk + 1] - c0;
+  for (int c = tid; c < nch; c += T) {
+    const uint32_t* w = t.chain_rec + static_cast<int64_t>(c0 + c) * kChainWords;
+    const uint4 w0 = *reinterpret_cast<const uint4*>(w);
+    const uint32_t w4 = w[4];
+    const uint32_t lp = w0.x & 0xffffu, lq = w0.x >> 16;
+    const int m = static_cast<int>(w4 >> 16);
+    auto rid = [&](int k) {
+      const uint32_t word = k < 2 ? w0.y : (k < 4 ? w0.z : (k < 6 ? w0.w : w4));
+      return (word >> (16 * (k & 1))) & 0xffffu;
+    };
+    double xp[3], xq[3], xr[3], kp, kq, kr, fp, fq, fr, up, uq, ur;
+    rec(lp, xp, kp, fp, up);
+    rec(lq, xq, kq, fq, uq);
+    uint32_t cur = rid(0);
+    rec(cur, xr, kr, fr, ur);
+    const double e1[3] = {xq[0] - xp[0], xq[1] - xp[1], xq[2] - xp[2]};
+    const double own_pq = (lp < static_cast<uint32_t>(nr) ? 1.0 : 0.0) +
+                          (lq < static_cast<uint32_t>(nr) ? 1.0 : 0.0);
+    double dkp = 0.0, dkq = 0.0, dkr = 0.0;
+    double dxp[3] = {0.0, 0.0, 0.0}, dxq[3] = {0.0, 0.0, 0.0}, dxr[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < m; ++k) {
+      const uint32_t ln = rid(k + 1);
+      double xs[3], ks, fs, us;
+      rec(ln, xs, ks, fs, us);
+      const double e2[3] = {xr[0] - xp[0], xr[1] - xp[1], xr[2] - xp[2]};
+      const double e3[3] = {xs[0] - xp[0], xs[1] - xp[1], xs[2] - xp[2]};
+      // cofactor gradients of q, r, s and the signed determinant; G_a = grad lambda_a
+      double g1[3], g2[3], g3[3];
+      g1[0] = e2[1] * e3[2] - e2[2] * e3[1];
+      g1[1] = e2[2] * e3[0] - e2[0] * e3[2];
+      g1[2] = e2[0] * e3[1] - e2[1] * e3[0];
+      g2[0] = e3[1] * e1[2] - e3[2] * e1[1];
+      g2[1] = e3[2] * e1[0] - e3[0] * e1[2];
+      g2[2] = e3[0] * e1[1] - e3[1] * e1[0];
+      g3[0] = e1[1] * e2[2] - e1[2] * e2[1];
+      g3[1] = e1[2] * e2[0] - e1[0] * e2[2];
+      g3[2] = e1[0] * e2[1] - e1[1] * e2[0];
+      const double det = e1[0] * g1[0] + e1[1] * g1[1] + e1[2] * g1[2];
+      const double inv = 1.0 / det;
+      const double vol = fabs(det) * (1.0 / 6.0);
+      double Gq[3], Gr[3], Gs[3], Gp[3], gu[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        Gq[i] = g1[i] * inv;
+        Gr[i] = g2[i] * inv;
+        Gs[i] = g3[i] * inv;
+        Gp[i] = -(Gq[i] + Gr[i] + Gs[i]);
+        gu[i] = (uq - up) * Gq[i] + (ur - up) * Gr[i] + (us - up) * Gs[i];
+      }
+      const double g2u = gu[0] * gu[0] + gu[1] * gu[1] + gu[2] * gu[2];
+      const double kb = 0.25 * (kp + kq + kr + ks);
+      const double E = kb * vol * g2u;
+      const double Gl = vol * (1.0 / 20.0) * (up * fp + uq * fq + ur * fr + us * fs +
+                                              (up + uq + ur + us) * (fp + fq + fr + fs));
+      const double own = own_pq + (cur < static_cast<uint32_t>(nr) ? 1.0 : 0.0) +
+                         (ln < static_cast<uint32_t>(nr) ? 1.0 : 0.0);
+      Jl += 0.25 * own * (E - Gl);
+      const double dk = 0.25 * vol * g2u;
+      const double kv = kb * vol;
+      // dJ/dx_a = kb vol (|gu|^2 G_a - 2 gu (gu . G_a)) - Gl G_a
+      auto dxa = [&](const double (&G)[3], double (&acc)[3]) {
+        const double gd = gu[0] * G[0] + gu[1] * G[1] + gu[2] * G[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[i] += kv * (g2u * G[i] - 2.0 * gu[i] * gd) - Gl * G[i];
+      };
+      dkp += dk;
+      dkq += dk;
+      dkr += dk;
+      dxa(Gp, dxp);
+      dxa(Gq, dxq);
+      dxa(Gr, dxr);
+      // r_k is in no later cell: emit it; s = r_{k+1} starts with this cell's share
+      emit(cur, dkr, dxr);
+      dkr = dk;
+      dxr[0] = dxr[1] = dxr[2] = 0.0;
+      dxa(Gs, dxr);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) xr[i] = xs[i];
+      kr = ks;
+      fr = fs;
+      ur = us;
+      cur = ln;
+    }
+    emit(cur, dkr, dxr);
+    emit(lp, dkp, dxp);
+    emit(lq, dkq, dxq);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) Jl += __shfl_down(Jl, off, 64);
+  if ((tid & 63) == 0) sJ[tid >> 6] = Jl;
+  __syncthreads();
+  if (tid == 0) {
+    double sum = 0.0;
+#pragma unroll
+    for (int w = 0; w < T / 64; ++w) sum += sJ[w];
+    unsafeAtomicAdd(a.J, sum);
+  }
+  const int n0 = t.node_ptr[blk];
+  for (int i = tid; i < nr; i += T) {
+    const int32_t v = t.node_vertex[n0 + i];
+    const int32_t kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + i];
+    a.dJdk[kd] = sacc[4 * i];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) a.dJdx[static_cast<int64_t>(v) * 3 + q] = sacc[4 * i + 1 + q];
+  }
+}
+
 __global__ void fill_zero_kernel(double* p, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -513,6 +667,16 @@ hipError_t launch_adjoint_blocked_t(const P1BlockTopo& t, const AdjointP1& a, hi
 
 hipError_t launch_p1_adjoint_blocked(const P1BlockTopo& t, const AdjointP1& a, hipStream_t s) {
   if (t.num_blocks == 0) return hipSuccess;
+  if (a.gdim == 3 && t.chain_rec) {
+    const int acc = align16(t.max_nodes * 6 * 8);
+    const int total = acc + align16(t.max_rows * 4 * 8);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(p1_adjoint_chain_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, total);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(p1_adjoint_chain_kernel, dim3(t.num_blocks), dim3(kAdjChainThreads), total, s,
+                       t, a, acc);
+    return hipGetLastError();
+  }
   switch (a.gdim) {
     case 1: return launch_adjoint_blocked_t<1>(t, a, s);
     case 2: return launch_adjoint_blocked_t<2>(t, a, s);

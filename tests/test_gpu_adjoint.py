@@ -26,6 +26,40 @@ def test_p1_adjoint_vs_oracle(oracle):
     assert rel(x.grad.cpu().numpy(), dxo) < 1e-12
 
 
+@pytest.mark.parametrize("n,perm", [(9, False), (16, True)])
+def test_p1_adjoint_chain_vs_cell_kernel(oracle, monkeypatch, n, perm):
+    """The edge-chain adjoint (tetrahedra with chain plans) against the per-cell blocked
+    adjoint (plans built without chains, DFA_P1_CHAINS=0) and the oracle, with vertex and
+    with permuted dof numbering."""
+    print(f"case adjoint chain vs cell n={n} perm={perm}", flush=True)
+    rng = np.random.default_rng(n)
+    mesh = dfa.UnitCubeMesh(n, n + 1, n - 1)
+    dfa.jitter_interior(mesh, 0.2 / n, rng)
+    node_perm = rng.permutation(mesh.num_vertices()) if perm else None
+    kappa = rng.uniform(0.5, 2.0, mesh.num_vertices())
+    f = rng.uniform(-1.0, 1.0, mesh.num_vertices())
+    u = rng.uniform(-1.0, 1.0, mesh.num_vertices())
+    out = []
+    for chains in ("1", "0"):
+        monkeypatch.setenv("DFA_P1_CHAINS", chains)
+        V = dfa.FunctionSpace(mesh, "P", 1, node_permutation=node_perm)
+        kd, fd, ud = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (kappa, f, u))
+        if perm:  # coefficients and state are indexed by dof
+            inv = torch.from_numpy(np.ascontiguousarray(node_perm)).cuda().long()
+            kd, fd, ud = (torch.empty_like(a).index_copy_(0, inv, a) for a in (kd, fd, ud))
+        x = mesh.coordinates_device()
+        J, dk, _, dx = adjoint._run(V, dfa._lib.FORM_P1_POISSON, x, kd, None, fd, None, ud)
+        out.append((J.item(), dk.cpu().numpy(), dx.cpu().numpy(), V))
+    (Jc, dkc, dxc, V), (Jb, dkb, dxb, _) = out
+    assert abs(Jc - Jb) < 1e-12 * abs(Jb)
+    assert rel(dkc, dkb) < 1e-12 and rel(dxc, dxb) < 1e-12
+    if not perm:
+        Jo, dko, dxo = oracle.adjoint_p1(mesh.coordinates(), mesh.cells(), V.cell_dofs(), kappa,
+                                         f, u)
+        assert abs(Jc - Jo) < 1e-12 * abs(Jo)
+        assert rel(dkc, dko) < 1e-12 and rel(dxc, dxo) < 1e-12
+
+
 def test_vp2_adjoint_vs_oracle(oracle):
     mesh, V, mu, lmbda, f3, u = workloads.vp2_elasticity_3d(2, seed=6, with_u=True)
     Q = V.coefficient_space()
