@@ -95,3 +95,23 @@ def test_vp2_unstructured_vs_oracle(oracle, scatter):
                                   V.cell_dofs(), V.dim(), f3=f3)
     assert _rel(vals, ref) < TOL
     assert _rel(b.get_local(), bref) < TOL
+
+
+@pytest.mark.parametrize("gdim,n", [(2, 300), (3, 500)])
+def test_p1_adjoint_unstructured_vs_oracle(oracle, gdim, n):
+    """dJ/dkappa and dJ/dx on Delaunay meshes (in 3D through the edge-chain adjoint, whose
+    chains also cover rows longer than 16 entries) against the oracle's closed forms."""
+    from dolfin_amd import adjoint
+    print(f"case unstructured adjoint gdim={gdim} n={n}", flush=True)
+    mesh, rng = _delaunay(gdim, n, seed=gdim * 1000 + n)
+    V = dfa.FunctionSpace(mesh, "P", 1)
+    kappa = rng.uniform(0.5, 2.0, V.dim())
+    f = rng.uniform(-1.0, 1.0, V.dim())
+    u = rng.uniform(-1.0, 1.0, V.dim())
+    kd, fd, ud = (torch.from_numpy(a).cuda() for a in (kappa, f, u))
+    J, dk, _, dx = adjoint._run(V, dfa._lib.FORM_P1_POISSON, mesh.coordinates_device(), kd, None,
+                                fd, None, ud)
+    Jo, dko, dxo = oracle.adjoint_p1(mesh.coordinates(), mesh.cells(), V.cell_dofs(), kappa, f, u)
+    assert abs(J.item() - Jo) < TOL * abs(Jo)
+    assert _rel(dk.cpu().numpy(), dko) < TOL
+    assert _rel(dx.cpu().numpy(), dxo) < TOL
