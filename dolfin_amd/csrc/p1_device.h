@@ -704,21 +704,65 @@ __device__ __forceinline__ void p1_chain_body(const P1BlockTopo& t, const P1Pass
       d[j][4 * h + 3] = w.w;
     }
   }
-  gather_nodes<3, T, true, WANT_B, false, false, true, true>(t, p.x, p.kappa, p.f, nullptr, nullptr,
-                                                             blk, snode, nullptr, nullptr, nullptr,
-                                                             nullptr);
+  // The first node and the first row of each thread are fetched in two rounds of independent
+  // loads (ids, then coordinates / coefficients and CSR row starts) so that the node and row
+  // chains of dependent loads overlap; later nodes and rows, if any, follow in loops.
+  const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
   const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
-  for (int i = tid; i < (nr * kRowStride + 1) / 2; i += T)
-    reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
+  const bool has_node = tid < nn, has_row = tid < nr;
+  int32_t v = 0, kd = 0, dof = 0, dg = 0;
+  if (has_node) {
+    v = t.node_vertex[n0 + tid];
+    kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + tid];
+  }
+  if (has_row) {
+    dof = t.row_dof[r0 + tid];
+    dg = t.row_diag[r0 + tid];
+  }
+  double nx = 0.0, ny = 0.0, nz = 0.0, nk = 0.0, nf = 0.0;
+  if (has_node) {
+    nx = p.x[3 * static_cast<int64_t>(v)];
+    ny = p.x[3 * static_cast<int64_t>(v) + 1];
+    nz = p.x[3 * static_cast<int64_t>(v) + 2];
+    nk = p.kappa[kd];
+    if constexpr (WANT_B) nf = p.f[kd];
+  }
+  int32_t rb0 = 0, rb1 = 0;
+  if (has_row) {
+    rb0 = t.csr_row_ptr[dof];
+    rb1 = t.csr_row_ptr[dof + 1];
+  }
   // one 8-byte word per owned row: CSR start, row length | diagonal position << 8
   int2* srow = reinterpret_cast<int2*>(srbase);
-  for (int i = tid; i < nr; i += T) {
-    const int32_t dof = t.row_dof[r0 + i];
-    srdof[i] = dof;
-    const int32_t b0 = t.csr_row_ptr[dof];
-    srow[i] = make_int2(b0, (t.csr_row_ptr[dof + 1] - b0) | (static_cast<int>(t.row_diag[r0 + i]) << 8));
+  if (has_node) {
+    double2* rec = reinterpret_cast<double2*>(snode + tid * 6);
+    rec[0] = make_double2(nx, ny);
+    rec[1] = make_double2(nz, nk);
+    rec[2] = make_double2(nz, nf);
+  }
+  if (has_row) {
+    srdof[tid] = dof;
+    srow[tid] = make_int2(rb0, (rb1 - rb0) | (dg << 8));
+    if constexpr (WANT_B) sbv[tid] = 0.0;
+  }
+  for (int i = tid + T; i < nn; i += T) {
+    const int32_t vi = t.node_vertex[n0 + i];
+    const int32_t ki = t.kdof_is_vertex ? vi : t.node_kdof[n0 + i];
+    const double zi = p.x[3 * static_cast<int64_t>(vi) + 2];
+    double2* rec = reinterpret_cast<double2*>(snode + i * 6);
+    rec[0] = make_double2(p.x[3 * static_cast<int64_t>(vi)], p.x[3 * static_cast<int64_t>(vi) + 1]);
+    rec[1] = make_double2(zi, p.kappa[ki]);
+    rec[2] = make_double2(zi, WANT_B ? p.f[ki] : 0.0);
+  }
+  for (int i = tid + T; i < nr; i += T) {
+    const int32_t di = t.row_dof[r0 + i];
+    srdof[i] = di;
+    const int32_t b0 = t.csr_row_ptr[di];
+    srow[i] = make_int2(b0, (t.csr_row_ptr[di + 1] - b0) | (static_cast<int>(t.row_diag[r0 + i]) << 8));
     if constexpr (WANT_B) sbv[i] = 0.0;
   }
+  for (int i = tid; i < (nr * kRowStride + 1) / 2; i += T)
+    reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < RR; ++j)
