@@ -316,11 +316,18 @@ __global__ __launch_bounds__(kAdjChainThreads) void p1_adjoint_chain_kernel(P1Bl
   __shared__ double sJ[T / 64];
   const int blk = xcd_block(t.num_blocks);
   const int tid = threadIdx.x;
-  gather_nodes<3, T, true, true, false, false, true>(t, a.x, a.kappa, a.f, nullptr, nullptr, blk,
-                                                     snode, nullptr, nullptr, nullptr, nullptr);
   {
+    // node ids and dofs, then coordinates, kappa, f and u: two rounds of independent loads
     const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
-    for (int i = tid; i < nn; i += T) snode[i * 6 + 5] = a.u[t.node_dof[n0 + i]];
+    for (int i = tid; i < nn; i += T) {
+      const int64_t v = t.node_vertex[n0 + i];
+      const int32_t kd = t.kdof_is_vertex ? static_cast<int32_t>(v) : t.node_kdof[n0 + i];
+      const int32_t dd = t.node_dof[n0 + i];
+      double2* rec = reinterpret_cast<double2*>(snode + i * 6);
+      rec[0] = make_double2(a.x[3 * v], a.x[3 * v + 1]);
+      rec[1] = make_double2(a.x[3 * v + 2], a.kappa[kd]);
+      rec[2] = make_double2(a.f[kd], a.u[dd]);
+    }
   }
   const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
   for (int i = tid; i < nr * 4; i += T) sacc[i] = 0.0;
