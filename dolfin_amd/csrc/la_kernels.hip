@@ -58,8 +58,9 @@ __global__ __launch_bounds__(kLaThreads) void spmv_stream_kernel(
 #pragma unroll
   for (int j = 0; j < kNpt; ++j) {
     const int k = base + j * kLaThreads + tid;
-    c[j] = k < end ? col[k] : 0;
-    v[j] = k < end ? vals[k] : 0.0;
+    // read once: streaming loads keep L2 for the x gathers
+    c[j] = k < end ? __builtin_nontemporal_load(col + k) : 0;
+    v[j] = k < end ? __builtin_nontemporal_load(vals + k) : 0.0;
   }
 #pragma unroll
   for (int j = 0; j < kNpt; ++j) {
