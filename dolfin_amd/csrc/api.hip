@@ -588,8 +588,14 @@ int dfa_adjoint(dfa_assembler h, int32_t a_form, const double* d_x, const double
     if (!h || !d_x || !d_u || !d_J || !d_dJdc0 || !d_dJdx) return fail("NULL argument", DFA_ERR_ARG);
     hipStream_t s = static_cast<hipStream_t>(stream);
     hip_check(dfa::launch_fill_zero(d_J, 1, s), "fill_zero");
-    hip_check(dfa::launch_fill_zero(d_dJdc0, h->nkdofs, s), "fill_zero");
-    hip_check(dfa::launch_fill_zero(d_dJdx, h->nv * h->gdim, s), "fill_zero");
+    // the blocked P1 adjoint writes every owned node's gradients once; when every vertex is a
+    // row of some block (one dof and one coefficient dof per vertex, each in a cell), nothing
+    // is left for a fill
+    const bool p1_blocked = h->element == DFA_ELEMENT_P1 && h->has_plan && h->kdof_injective;
+    if (!(p1_blocked && h->rows_covered && h->ndofs == h->nv && h->nkdofs == h->nv)) {
+      hip_check(dfa::launch_fill_zero(d_dJdc0, h->nkdofs, s), "fill_zero");
+      hip_check(dfa::launch_fill_zero(d_dJdx, h->nv * h->gdim, s), "fill_zero");
+    }
     if (h->element == DFA_ELEMENT_P1) {
       if (a_form != DFA_FORM_P1_POISSON) return fail("P1 adjoint supports the kappa-Poisson form", DFA_ERR_ARG);
       if (!d_c0 || !d_f) return fail("kappa or f is NULL", DFA_ERR_ARG);
@@ -606,7 +612,7 @@ int dfa_adjoint(dfa_assembler h, int32_t a_form, const double* d_x, const double
       a.J = d_J;
       a.dJdk = d_dJdc0;
       a.dJdx = d_dJdx;
-      if (h->has_plan && h->kdof_injective)
+      if (p1_blocked)
         hip_check(dfa::launch_p1_adjoint_blocked(h->btopo, a, s), "p1_adjoint_blocked_kernel");
       else
         hip_check(dfa::launch_adjoint_p1(a, s), "adjoint_p1_kernel");
