@@ -223,7 +223,7 @@ def main():
         t0 = time.perf_counter()
         O.sparsity(cd, V.dim())
         cpu_struct = time.perf_counter() - t0
-        t1 = [cpu_pass(0) for _ in range(3)]
+        t1 = [cpu_pass(0) for _ in range(5)]
         log(f"cpu baseline: values, {nthreads} threads")
         cpu_pass(nthreads)  # warm-up
         tn = [cpu_pass(nthreads) for _ in range(5)]
@@ -235,7 +235,7 @@ def main():
                       f"threads over contiguous cell ranges with atomic scatter, median of 5"}
         result["cpu_baseline_1thread"] = {
             "value": nc / s1, "unit": "cells/s", "cores": 1, "kind": "port",
-            "sample": f"the same loop serial (DOLFIN's per-process semantics), median of 3",
+            "sample": f"the same loop serial (DOLFIN's per-process semantics), median of 5",
             "first_assembly_s": cpu_struct + t1[0], "structure_s": cpu_struct}
         result["cpu_baseline_host"] = {"nproc": os.cpu_count()}
 
