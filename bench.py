@@ -82,21 +82,10 @@ def bench_batch(nmembers, log, seed=5):
     import dolfin_amd as dfa
     from dolfin_amd import autotune, workloads
     from dolfin_amd.batch import BatchAssembler
-    rng = np.random.default_rng(seed)
-    sizes = rng.integers(8, 49, nmembers)
     t = time.perf_counter()
-    forms, alg, cells = [], 0, 0
-    for i, n in enumerate(sizes):
-        if i % 2 == 0:
-            mesh, V, kappa, f = workloads.p1_poisson_3d(int(n), seed=int(rng.integers(1 << 30)))
-            forms.append(dfa.poisson_form(V, workloads.to_function(V, kappa)))
-        else:
-            mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(int(n), seed=int(rng.integers(1 << 30)))
-            Q = V.coefficient_space()
-            forms.append(dfa.elasticity_form(V, workloads.to_function(Q, mu), workloads.to_function(Q, lmbda)))
-        cells += mesh.num_cells()
-        if (i + 1) % 32 == 0:
-            log(f"batch: {i + 1}/{nmembers} members built ({time.perf_counter() - t:.0f} s)")
+    forms = workloads.batch_members(nmembers, seed, log=log)
+    alg = 0
+    cells = sum(a.V.mesh().num_cells() for a in forms)
     best, gtimes = autotune.tune_batch_grid(lambda g: BatchAssembler(forms, grid_per_cu=g),
                                             grids=(1, 2, 0), reps=3)
     batch = BatchAssembler(forms, grid_per_cu=best)
@@ -180,6 +169,7 @@ def main():
     used_scatter = ("colored" if scatter == dfa.SCATTER_COLORED else
                     "blocked" if info["num_blocks"] > 0 else "atomic")
     traffic = load_traffic(f"p1_{used_scatter}_{args.n}")
+    kernel = eng.pass_kernel(dfa._lib.FORM_P1_POISSON, dfa._lib.FORM_P1_LOAD, scatter=scatter)
     result = {
         "metric": METRIC,
         "value": nc / (ms * 1e-3),
@@ -197,6 +187,7 @@ def main():
                                f"{args.n}^3 tets ({nc} cells, {V.dim()} dofs, nnz {eng.nnz}), "
                                f"fused A+b re-assembly, {used_scatter} scatter",
                    "cells": nc, "dofs": V.dim(), "nnz": eng.nnz, "scatter": used_scatter,
+                   "kernel": kernel,
                    "plan": info, "device": device_info(dev)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

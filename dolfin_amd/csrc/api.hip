@@ -323,6 +323,7 @@ int dfa_assembler_destroy(dfa_assembler h) {
   h->base.release();
   h->colors.release();
   h->facets.release();
+  h->x0buf.release();
   delete h;
   return DFA_OK;
 }
@@ -431,10 +432,13 @@ static void ensure_colors(dfa_assembler h, hipStream_t s) {
 }
 
 // One fused pass over the cells (Assembler::assemble_cells / SystemAssembler::cell_wise_assembly).
+// With name_only non-NULL nothing is launched: the name of the kernel the pass would run is
+// written there instead (pointer arguments are then only tested for NULL).
 static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const double* x,
                     const double* a_c0, const double* a_c1, const double* L_c0, const double* f3,
                     const uint8_t* bcm, const double* bcv, double* values, double* b,
-                    int32_t scatter, int32_t flags, hipStream_t s) {
+                    int32_t scatter, int32_t flags, hipStream_t s,
+                    std::string* name_only = nullptr) {
   if (!x) throw std::runtime_error("coordinates are NULL");
   const bool add = (flags & DFA_ADD_VALUES) != 0;
   const bool bc = bcm != nullptr;
@@ -463,6 +467,12 @@ static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const doubl
     p.values = values;
     p.b = b;
     const int sc = resolve_scatter(h, scatter);
+    if (name_only) {
+      if (sc == DFA_SCATTER_ATOMIC) *name_only = "p1_atomic_kernel";
+      else if (sc == DFA_SCATTER_COLORED) *name_only = "p1_colored_kernel";
+      else *name_only = dfa::p1_blocked_kernel_name(h->btopo, p);
+      return DFA_OK;
+    }
     if (sc == DFA_SCATTER_ATOMIC) {
       if (!add && values) hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
       if (!add && b) hip_check(dfa::launch_fill_zero(b, h->ndofs, s), "fill_zero");
@@ -520,9 +530,13 @@ static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const doubl
   p.bc_value = bcv;
   p.values = values;
   p.b = b;
+  p.add = add;
+  if (name_only) {
+    *name_only = dfa::vp2_kernel_name(sc == DFA_SCATTER_BLOCKED, p);
+    return DFA_OK;
+  }
   if (sc == DFA_SCATTER_BLOCKED) {
-    if (add) throw std::runtime_error("add_values is implemented by the atomic scatter for vector P2");
-    if (!h->rows_covered) {
+    if (!add && !h->rows_covered) {
       if (values) hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
       if (b) hip_check(dfa::launch_fill_zero(b, h->ndofs, s), "fill_zero");
     }
@@ -578,6 +592,66 @@ int dfa_assemble_system(dfa_assembler h, int32_t a_form, int32_t L_form, const d
     if (L_form == DFA_FORM_VP2_BODY_FORCE && !h_f3) return fail("f3 is NULL", DFA_ERR_ARG);
     return run_pass(h, a_form, L_form, d_x, d_a_c0, d_a_c1, d_L_c0, h_f3, d_bc_mask, d_bc_value,
                     d_values, d_b, scatter, flags, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dfa_dirichlet_x0(int64_t n, const uint8_t* d_bc_mask, const double* d_bc_value,
+                     const double* d_x0, double* d_out, void* stream) {
+  return guarded([&]() -> int {
+    if (n < 0 || (n > 0 && (!d_bc_mask || !d_bc_value || !d_x0 || !d_out)))
+      return fail("invalid arguments", DFA_ERR_ARG);
+    hip_check(dfa::launch_dirichlet_x0(n, d_bc_mask, d_bc_value, d_x0, d_out,
+                                       static_cast<hipStream_t>(stream)),
+              "dirichlet_x0_kernel");
+    return DFA_OK;
+  });
+}
+
+int dfa_assemble_system_x0(dfa_assembler h, int32_t a_form, int32_t L_form, const double* d_x,
+                           const double* d_a_c0, const double* d_a_c1, const double* d_L_c0,
+                           const double* h_f3, const uint8_t* d_bc_mask, const double* d_bc_value,
+                           const double* d_x0, double* d_values, double* d_b, int32_t scatter,
+                           int32_t flags, void* stream) {
+  return guarded([&]() -> int {
+    if (!h) return fail("NULL handle", DFA_ERR_ARG);
+    if (!d_x0) return fail("x0 is NULL (use dfa_assemble_system)", DFA_ERR_ARG);
+    if (!d_values && !d_b) return fail("nothing to assemble (A and b are NULL)", DFA_ERR_ARG);
+    if (a_form < 0 || L_form < 0) return fail("assemble_system needs a bilinear and a linear form", DFA_ERR_ARG);
+    if (L_form == DFA_FORM_VP2_BODY_FORCE && !h_f3) return fail("f3 is NULL", DFA_ERR_ARG);
+    if ((d_bc_mask == nullptr) != (d_bc_value == nullptr))
+      return fail("Dirichlet mask and values go together", DFA_ERR_ARG);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const double* bcv = d_bc_value;
+    if (d_bc_mask) {
+      // SystemAssembler.cpp:252-275: every boundary value g becomes x0[dof] - g
+      if (!h->d_bc_x0) {
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, std::max<int64_t>(h->ndofs, 1) * sizeof(double)), "hipMalloc");
+        h->d_bc_x0 = h->x0buf.adopt(static_cast<double*>(p), h->ndofs * sizeof(double));
+      }
+      hip_check(dfa::launch_dirichlet_x0(h->ndofs, d_bc_mask, d_bc_value, d_x0, h->d_bc_x0, s),
+                "dirichlet_x0_kernel");
+      bcv = h->d_bc_x0;
+    }
+    return run_pass(h, a_form, L_form, d_x, d_a_c0, d_a_c1, d_L_c0, h_f3, d_bc_mask, bcv,
+                    d_values, d_b, scatter, flags, s);
+  });
+}
+
+int dfa_pass_kernel(dfa_assembler h, int32_t a_form, int32_t L_form, int32_t has_bc,
+                    int32_t scatter, char* name, int32_t len) {
+  return guarded([&]() -> int {
+    if (!h || !name || len <= 0) return fail("NULL handle or name buffer", DFA_ERR_ARG);
+    // stand-in non-NULL pointers: only their presence selects the kernel; nothing is read
+    static const double dummy[3] = {0, 0, 0};
+    static const uint8_t dmask = 0;
+    double* w = const_cast<double*>(dummy);
+    std::string out;
+    run_pass(h, a_form, L_form, dummy, dummy, dummy, dummy, dummy, has_bc ? &dmask : nullptr,
+             has_bc ? dummy : nullptr, a_form >= 0 ? w : nullptr, L_form >= 0 ? w : nullptr,
+             scatter, 0, nullptr, &out);
+    std::snprintf(name, static_cast<size_t>(len), "%s", out.c_str());
+    return DFA_OK;
   });
 }
 
@@ -778,6 +852,72 @@ int dfa_csr_spmv(dfa_assembler h, const double* d_values, double alpha, const do
     A.nblocks = h->spmv_nblocks;
     hip_check(dfa::launch_spmv(A, alpha, d_x, beta, d_y, static_cast<hipStream_t>(stream)),
               "spmv_kernel");
+    return DFA_OK;
+  });
+}
+
+int dfa_csr_add_local(dfa_assembler h, double* d_values, int32_t m, const int32_t* h_rows,
+                      int32_t n, const int32_t* h_cols, const double* h_block, void* stream) {
+  return guarded([&]() -> int {
+    if (!h || !d_values || m < 0 || n < 0) return fail("invalid arguments", DFA_ERR_ARG);
+    const int64_t nk = static_cast<int64_t>(m) * n;
+    if (nk == 0) return DFA_OK;
+    if (!h_rows || !h_cols || !h_block) return fail("NULL block or index array", DFA_ERR_ARG);
+    for (int32_t i = 0; i < m; ++i)
+      if (h_rows[i] < 0 || h_rows[i] >= h->ndofs) return fail("row index out of range", DFA_ERR_ARG);
+    for (int32_t j = 0; j < n; ++j)
+      if (h_cols[j] < 0 || h_cols[j] >= h->ndofs) return fail("column index out of range", DFA_ERR_ARG);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dfa::DeviceArena tmp;
+    try {
+      const int32_t* d_rows = tmp.upload(h_rows, static_cast<size_t>(m), s);
+      const int32_t* d_cols = tmp.upload(h_cols, static_cast<size_t>(n), s);
+      const double* d_block = tmp.upload(h_block, static_cast<size_t>(nk), s);
+      std::vector<int32_t> slot(static_cast<size_t>(nk));
+      int32_t* d_slot = tmp.upload(slot, s);
+      hip_check(dfa::launch_block_slots(m, d_rows, n, d_cols, h->d_row_ptr, h->d_col, d_slot, s),
+                "block_slots_kernel");
+      hip_check(hipMemcpyAsync(slot.data(), d_slot, nk * sizeof(int32_t), hipMemcpyDeviceToHost, s),
+                "hipMemcpyAsync");
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+      for (int64_t k = 0; k < nk; ++k)
+        if (slot[k] < 0) {
+          tmp.release();
+          return fail("entry (" + std::to_string(h_rows[k / n]) + ", " + std::to_string(h_cols[k % n]) +
+                          ") is not in the sparsity pattern",
+                      DFA_ERR_ARG);
+        }
+      hip_check(dfa::launch_block_add(nk, d_slot, d_block, d_values, s), "block_add_kernel");
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    } catch (...) {
+      tmp.release();
+      throw;
+    }
+    tmp.release();
+    return DFA_OK;
+  });
+}
+
+int dfa_vector_add_local(double* d_b, int64_t size, int32_t m, const int32_t* h_rows,
+                         const double* h_block, void* stream) {
+  return guarded([&]() -> int {
+    if (!d_b || size < 0 || m < 0) return fail("invalid arguments", DFA_ERR_ARG);
+    if (m == 0) return DFA_OK;
+    if (!h_rows || !h_block) return fail("NULL block or index array", DFA_ERR_ARG);
+    for (int32_t i = 0; i < m; ++i)
+      if (h_rows[i] < 0 || h_rows[i] >= size) return fail("row index out of range", DFA_ERR_ARG);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dfa::DeviceArena tmp;
+    try {
+      const int32_t* d_rows = tmp.upload(h_rows, static_cast<size_t>(m), s);
+      const double* d_block = tmp.upload(h_block, static_cast<size_t>(m), s);
+      hip_check(dfa::launch_block_add(m, d_rows, d_block, d_b, s), "block_add_kernel");
+      hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    } catch (...) {
+      tmp.release();
+      throw;
+    }
+    tmp.release();
     return DFA_OK;
   });
 }

@@ -103,4 +103,7 @@ struct dfa_assembler_s {
   int32_t* d_facet_cell = nullptr;
   int8_t* d_facet_local = nullptr;
   dfa::DeviceArena facets;
+  // Dirichlet values shifted by x0 (dfa_assemble_system_x0), allocated on first use
+  double* d_bc_x0 = nullptr;
+  dfa::DeviceArena x0buf;
 };

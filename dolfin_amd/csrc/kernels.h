@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 namespace dfa {
 
@@ -86,6 +87,8 @@ hipError_t launch_p1_color_gather(int64_t num_cells, int32_t nvc, const int32_t*
                                   int32_t* out_kdofs, uint8_t* out_rank, hipStream_t s);
 hipError_t launch_p1_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t s);
 size_t p1_blocked_lds_bytes(const P1BlockTopo& t, const P1Pass& p);
+// Name and template arguments of the kernel launch_p1_blocked runs for (t, p).
+std::string p1_blocked_kernel_name(const P1BlockTopo& t, const P1Pass& p);
 hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);
 
 // ---- P1 exterior facets -----------------------------------------------------------------
@@ -132,6 +135,16 @@ struct CsrView {
 hipError_t launch_spmv(const CsrView& A, double alpha, const double* x, double beta, double* y,
                        hipStream_t s);
 hipError_t launch_diagonal(const CsrView& A, double* d, hipStream_t s);
+// Host-side block insertion (GenericMatrix::add_local): slot[i*n+j] = position of
+// (rows[i], cols[j]) in the CSR (-1 if the pattern lacks it), then values[slot] += block.
+hipError_t launch_block_slots(int32_t m, const int32_t* rows, int32_t n, const int32_t* cols,
+                              const int32_t* row_ptr, const int32_t* col, int32_t* slot,
+                              hipStream_t s);
+hipError_t launch_block_add(int64_t nk, const int32_t* slot, const double* block, double* values,
+                            hipStream_t s);
+// out[i] = mask[i] ? x0[i] - g[i] : 0 (the boundary values of an incremental solve).
+hipError_t launch_dirichlet_x0(int64_t n, const uint8_t* mask, const double* g, const double* x0,
+                               double* out, hipStream_t s);
 
 // ---- vector P2 elasticity --------------------------------------------------------------
 struct VP2Topo {
@@ -146,6 +159,7 @@ struct VP2Topo {
 
 struct VP2Pass {
   bool want_A = false, want_b = false, bc = false;
+  bool add = false;  // add to the tensors instead of overwriting them (blocked scatter)
   const double* x = nullptr;
   const double* mu = nullptr;
   const double* lmbda = nullptr;
@@ -178,6 +192,8 @@ struct VP2BlockTopo {
 };
 hipError_t launch_vp2_blocked(const VP2BlockTopo& t, const VP2Pass& p, hipStream_t s);
 size_t vp2_blocked_lds_bytes(const VP2BlockTopo& t, bool want_b);
+// Name and template arguments of the kernel launch_vp2_blocked / launch_vp2_atomic runs.
+std::string vp2_kernel_name(bool blocked, const VP2Pass& p);
 
 // ---- adjoint -------------------------------------------------------------------------
 struct AdjointP1 {

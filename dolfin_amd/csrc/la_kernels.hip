@@ -102,7 +102,78 @@ __global__ __launch_bounds__(kLaThreads) void diagonal_kernel(int64_t nrows, con
   d[r] = v;
 }
 
+// Newton-increment Dirichlet values: out = x0 - g on Dirichlet dofs, 0 elsewhere.
+__global__ __launch_bounds__(kLaThreads) void dirichlet_x0_kernel(int64_t n,
+                                                                  const uint8_t* __restrict__ mask,
+                                                                  const double* __restrict__ g,
+                                                                  const double* __restrict__ x0,
+                                                                  double* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kLaThreads + threadIdx.x;
+  if (i >= n) return;
+  out[i] = mask[i] ? x0[i] - g[i] : 0.0;
+}
+
+// Slot of column c in row r of a sorted CSR row (-1 if absent), for block entry k.
+__global__ __launch_bounds__(kLaThreads) void block_slots_kernel(int32_t m, const int32_t* __restrict__ rows,
+                                                                 int32_t n, const int32_t* __restrict__ cols,
+                                                                 const int32_t* __restrict__ rp,
+                                                                 const int32_t* __restrict__ col,
+                                                                 int32_t* __restrict__ slot) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kLaThreads + threadIdx.x;
+  if (k >= static_cast<int64_t>(m) * n) return;
+  const int32_t r = rows[k / n], c = cols[k % n];
+  int lo = rp[r], hi = rp[r + 1];
+  int found = -1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int cm = col[mid];
+    if (cm == c) {
+      found = mid;
+      break;
+    }
+    if (cm < c) lo = mid + 1;
+    else hi = mid;
+  }
+  slot[k] = found;
+}
+
+// values[slot[k]] += block[k] (atomic: a block may name one entry twice).
+__global__ __launch_bounds__(kLaThreads) void block_add_kernel(int64_t nk, const int32_t* __restrict__ slot,
+                                                               const double* __restrict__ block,
+                                                               double* __restrict__ values) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kLaThreads + threadIdx.x;
+  if (k >= nk) return;
+  unsafeAtomicAdd(values + slot[k], block[k]);
+}
+
 }  // namespace
+
+hipError_t launch_block_slots(int32_t m, const int32_t* rows, int32_t n, const int32_t* cols,
+                              const int32_t* row_ptr, const int32_t* col, int32_t* slot,
+                              hipStream_t s) {
+  const int64_t nk = static_cast<int64_t>(m) * n;
+  if (nk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(block_slots_kernel, dim3(static_cast<unsigned>((nk + kLaThreads - 1) / kLaThreads)),
+                     dim3(kLaThreads), 0, s, m, rows, n, cols, row_ptr, col, slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_add(int64_t nk, const int32_t* slot, const double* block, double* values,
+                            hipStream_t s) {
+  if (nk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(block_add_kernel, dim3(static_cast<unsigned>((nk + kLaThreads - 1) / kLaThreads)),
+                     dim3(kLaThreads), 0, s, nk, slot, block, values);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirichlet_x0(int64_t n, const uint8_t* mask, const double* g, const double* x0,
+                               double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t grid = (n + kLaThreads - 1) / kLaThreads;
+  hipLaunchKernelGGL(dirichlet_x0_kernel, dim3(static_cast<unsigned>(grid)), dim3(kLaThreads), 0, s,
+                     n, mask, g, x0, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_spmv(const CsrView& A, double alpha, const double* x, double beta, double* y,
                        hipStream_t s) {

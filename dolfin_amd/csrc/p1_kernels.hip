@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <string>
 
 #include "common.h"
 #include "kernels.h"
@@ -636,6 +638,26 @@ hipError_t launch_p1_color_gather(int64_t nc, int32_t nvc, const int32_t* perm,
   hipLaunchKernelGGL(p1_color_gather_kernel, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s,
                      nc, nvc, perm, cells, cdofs, kdofs, row_ptr, col, oc, od, ok, orank);
   return hipGetLastError();
+}
+
+std::string p1_blocked_kernel_name(const P1BlockTopo& t, const P1Pass& p) {
+  // the choices of launch_blocked_t, launch_chain_t and launch_blocked_tu
+  char buf[192];
+  if (p.gdim == 3 && p.a_form == DFA_FORM_P1_POISSON && !p.bc && chains_fit(t, p)) {
+    const int T = t.chain_threads == 512 ? 512 : 256;
+    const int RR = T == 512 ? (t.max_chains <= 512 ? 1 : 2)
+                            : (t.max_chains <= 256 ? 1 : (t.max_chains <= 512 ? 2 : 4));
+    std::snprintf(buf, sizeof(buf), "p1_chain_kernel<WANT_B=%d,T=%d,RR=%d>", p.want_b ? 1 : 0, T, RR);
+    return buf;
+  }
+  const bool strided = t.max_row_len <= kMaxRank;
+  const int T = strided ? (t.threads == 512 ? 512 : 256) : 256;
+  const int RR = strided && t.max_cells <= 4 * T ? 4 : 0;
+  std::snprintf(buf, sizeof(buf),
+                "p1_blocked_kernel<D=%d,AFORM=%d,WANT_B=%d,BC=%d,STRIDED=%d,T=%d,WIDE=%d,RR=%d>",
+                p.gdim, p.a_form, p.want_b ? 1 : 0, p.bc ? 1 : 0, strided ? 1 : 0, T,
+                strided && t.wide_nodes ? 1 : 0, RR);
+  return buf;
 }
 
 hipError_t launch_p1_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t s) {

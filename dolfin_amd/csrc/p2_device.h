@@ -509,10 +509,15 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
       for (int j = lane; j < 3 * len; j += 64) {
         const int comp = j >= len ? (j >= 2 * len ? 2 : 1) : 0;
         const int64_t base = comp == 0 ? b0 : (comp == 1 ? b1 : b2);
-        __builtin_nontemporal_store(svals[off + j], p.values + base + (j - comp * len));
+        double* dst = p.values + base + (j - comp * len);
+        const double v = p.add ? svals[off + j] + *dst : svals[off + j];
+        __builtin_nontemporal_store(v, dst);
       }
     }
-    if (WANT_B && lane < 3) p.b[srows[3 * o + lane]] = sbacc[3 * o + lane];
+    if (WANT_B && lane < 3) {
+      double* dst = p.b + srows[3 * o + lane];
+      *dst = p.add ? *dst + sbacc[3 * o + lane] : sbacc[3 * o + lane];
+    }
   }
 }
 

@@ -21,13 +21,7 @@
 // (s1, s2) = (1, deg_a) for UFC component-major numbering or (3, 1) for node-major.
 #include <hip/hip_runtime.h>
 
-#include <type_traits>
-
-#include "common.h"
-#include "kernels.h"
-
-#include <hip/hip_runtime.h>
-
+#include <string>
 #include <type_traits>
 
 #include "common.h"
@@ -180,6 +174,11 @@ hipError_t launch_vp2_atomic(const VP2Topo& t, const VP2Pass& p, hipStream_t s) 
 }
 
 
+
+std::string vp2_kernel_name(bool blocked, const VP2Pass& p) {
+  return std::string(blocked ? "vp2_blocked_kernel" : "vp2_atomic_kernel") + "<WANT_A=" +
+         (p.want_A ? "1" : "0") + ",WANT_B=" + (p.want_b ? "1" : "0") + ",BC=" + (p.bc ? "1" : "0") + ">";
+}
 
 size_t vp2_blocked_lds_bytes(const VP2BlockTopo& t, bool want_b) {
   return static_cast<size_t>(vp2_lds(t.max_verts, t.max_owned, t.max_vals, want_b).total);

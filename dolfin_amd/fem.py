@@ -75,6 +75,14 @@ class Engine:
         return {"plan_bytes": pb.value, "num_blocks": nb.value, "cell_appearances": ca.value,
                 "lds_bytes": lds.value}
 
+    def pass_kernel(self, a_kind, L_kind=-1, has_bc=False, scatter=_lib.SCATTER_AUTO):
+        """Name and template arguments of the kernel an assembly with these forms would run
+        (a_kind or L_kind -1: matrix or vector alone)."""
+        buf = ctypes.create_string_buffer(256)
+        _lib.check(_lib.load().dfa_pass_kernel(self._h, a_kind, L_kind, int(bool(has_bc)), scatter,
+                                               buf, 256))
+        return buf.value.decode()
+
     def color_ptr(self):
         """Offsets of each colour's cells in the coloured scatter (P1; built on first use)."""
         L = _lib.load()
@@ -305,46 +313,28 @@ def _assemble_facets(eng, x, a_facets, L_facets, A_values, b_values, mask=None, 
 
 # ---- boundary conditions ------------------------------------------------------------------
 def boundary_facet_nodes(V):
-    """Dofs on boundary facets (DirichletBC 'on_boundary', topological method): facets of
-    exactly one cell; P1 -> their vertices, vector P2 -> their vertices and edges."""
-    mesh = V.mesh()
-    c = mesh.cells().astype(np.int64)
-    n = c.shape[1]
-    nv = mesh.num_vertices()
-    facets = []
-    for skip in range(n):
-        f = np.sort(np.delete(c, skip, axis=1), axis=1)
-        facets.append(f)
-    F = np.concatenate(facets, axis=0)
-    key = np.zeros(F.shape[0], dtype=np.int64)
-    for k in range(F.shape[1]):
-        key = key * nv + F[:, k]
-    uniq, inv, cnt = np.unique(key, return_inverse=True, return_counts=True)
-    bf = F[cnt[inv] == 1]
-    onv = np.zeros(nv, dtype=bool)
-    onv[bf.ravel()] = True
-    if V.element == _lib.ELEMENT_P1:
-        dofs_of_vertex = np.full(nv, -1, dtype=np.int64)
-        dofs_of_vertex[mesh.cells().ravel()] = V.cell_dofs().ravel()
-        return np.unique(dofs_of_vertex[onv & (dofs_of_vertex >= 0)])
-    # vector P2: boundary edges are edges of boundary facets
-    ce, ne = mesh.cell_edges()
-    loc = np.array([[2, 3], [1, 3], [1, 2], [0, 3], [0, 2], [0, 1]])
-    a = np.minimum(c[:, loc[:, 0]], c[:, loc[:, 1]])
-    b = np.maximum(c[:, loc[:, 0]], c[:, loc[:, 1]])
-    ekey = a * nv + b
-    fe = set()
-    for i, j in ((0, 1), (0, 2), (1, 2)):
-        fe.update((bf[:, i] * nv + bf[:, j]).tolist())
-    fe = np.fromiter(fe, dtype=np.int64)
-    onedge = np.isin(ekey, fe)
+    """Dofs on boundary facets (DirichletBC 'on_boundary', topological method): the facets of
+    exactly one cell, taken from the library's exterior-facet list (cell, local facet k =
+    the facet opposite local vertex k); P1 -> their vertices' dofs, vector P2 -> the dofs of
+    their vertices and edges, every component."""
+    fc, fl = exterior_facets(V)
+    fc = fc.astype(np.int64)
     cd = V.cell_dofs()
-    mask = np.zeros(V.dim(), dtype=bool)
-    vert_on = onv[c]  # [nc, 4]
-    for comp in range(3):
-        mask[cd[:, comp * 10: comp * 10 + 4][vert_on]] = True
-        mask[cd[:, comp * 10 + 4: comp * 10 + 10][onedge]] = True
-    return np.nonzero(mask)[0]
+    nvc = V.mesh().cells().shape[1]
+    # local vertices of each local facet: on_facet[k, a] = (a != k)
+    on_facet = ~np.eye(nvc, dtype=bool)
+    if V.element == _lib.ELEMENT_P1:
+        vsel = on_facet[fl]                                      # [nf, nvc]
+        return np.unique(cd[fc][vsel])
+    # vector P2, component-major local dofs comp*10 + node; nodes 0-3 vertices, 4-9 the UFC
+    # edges (2,3),(1,3),(1,2),(0,3),(0,2),(0,1): an edge lies on facet k when k is not one of
+    # its vertices
+    loc = np.array([[2, 3], [1, 3], [1, 2], [0, 3], [0, 2], [0, 1]])
+    edge_on = np.array([[k not in e for e in loc] for k in range(4)])   # [4, 6]
+    nodes_on = np.concatenate([on_facet, edge_on], axis=1)[fl]        # [nf, 10]
+    c = cd[fc]                                                          # [nf, 30]
+    sel = np.concatenate([nodes_on] * 3, axis=1)
+    return np.unique(c[sel])
 
 
 class DirichletBC:
@@ -472,9 +462,17 @@ class SystemAssembler:
         self.a, self.L = a_cells[0], L
         self.bcs = [] if bcs is None else (list(bcs) if isinstance(bcs, (list, tuple)) else [bcs])
         self.scatter = scatter
+        # AssemblerBase flags (dolfin/fem/AssemblerBase.h:46-67), honoured as the reference's
+        # SystemAssembler does through init_global_tensor
+        self.add_values = False
+        self.finalize_tensor = True
         self.keep_diagonal = False
+        self._x0_vals = None
 
-    def assemble(self, A=None, b=None):
+    def assemble(self, A=None, b=None, x0=None):
+        """assemble(A, b) / assemble(A) / assemble(b=b), and with x0 the incremental variants
+        assemble(A, b, x0) / assemble(b=b, x0=x0) (SystemAssembler.h:63-84): every Dirichlet
+        value g becomes x0[dof] - g (SystemAssembler.cpp:252-275)."""
         if A is None and b is None:
             raise ValueError("nothing to assemble")
         lib = _lib.load()
@@ -488,13 +486,32 @@ class SystemAssembler:
         ac = self.a.coefficient_vectors()
         lc = self.L.coefficient_vectors()
         mask, vals = _merge_bcs(V, self.bcs)
-        _lib.check(lib.dfa_assemble_system(
-            eng.handle, self.a.kind, self.L.kind, _lib.ptr(x),
-            _lib.ptr(ac[0] if len(ac) > 0 else None), _lib.ptr(ac[1] if len(ac) > 1 else None),
-            _lib.ptr(lc[0] if lc else None), _lib.ptr(self.L.f3), _lib.ptr(mask), _lib.ptr(vals),
-            _lib.ptr(None if A is None else A.values), _lib.ptr(None if b is None else b.values),
-            self.scatter, 0, _lib.stream()))
-        if self.a_facets or self.L_facets:
+        flags = _lib.ADD_VALUES if self.add_values else 0
+        has_facets = bool(self.a_facets or self.L_facets)
+        x0v = None
+        if x0 is not None:
+            x0v = x0.values if isinstance(x0, Vector) else x0
+            if x0v.shape[0] != V.dim():
+                raise ValueError("x0 must have one entry per dof")
+            if mask is not None and has_facets:
+                # the facet pass needs the same shifted values: form them once
+                if self._x0_vals is None or self._x0_vals.shape[0] != V.dim():
+                    self._x0_vals = torch.empty(V.dim(), dtype=torch.float64, device="cuda")
+                _lib.check(lib.dfa_dirichlet_x0(V.dim(), _lib.ptr(mask), _lib.ptr(vals),
+                                                _lib.ptr(x0v), _lib.ptr(self._x0_vals),
+                                                _lib.stream()))
+                vals, x0v = self._x0_vals, None
+        args = (eng.handle, self.a.kind, self.L.kind, _lib.ptr(x),
+                _lib.ptr(ac[0] if len(ac) > 0 else None), _lib.ptr(ac[1] if len(ac) > 1 else None),
+                _lib.ptr(lc[0] if lc else None), _lib.ptr(self.L.f3), _lib.ptr(mask),
+                _lib.ptr(vals))
+        outs = (_lib.ptr(None if A is None else A.values), _lib.ptr(None if b is None else b.values),
+                self.scatter, flags, _lib.stream())
+        if x0v is not None:
+            _lib.check(lib.dfa_assemble_system_x0(*args, _lib.ptr(x0v), *outs))
+        else:
+            _lib.check(lib.dfa_assemble_system(*args, *outs))
+        if has_facets:
             # facet tensors belong to their cell's tensor before apply_bc
             # (SystemAssembler.cpp:421-481): Dirichlet rows keep the cell pass's values and
             # Dirichlet columns are lifted into b
@@ -503,6 +520,10 @@ class SystemAssembler:
             _assemble_facets(eng, x, self.a_facets, self.L_facets,
                              None if A is None else A.values, None if b is None else b.values,
                              mask=mask, vals=vals)
+        if self.finalize_tensor:
+            for T in (A, b):
+                if T is not None:
+                    T.apply("add")
         return A, b
 
 
@@ -519,9 +540,16 @@ def assemble(form, tensor=None, add_values=False, finalize_tensor=True, keep_dia
     return asm.assemble(tensor, form)
 
 
-def assemble_system(a, L, bcs=None, A_tensor=None, b_tensor=None, scatter=_lib.SCATTER_AUTO):
-    """python/dolfin/fem/assembling.py:322-385."""
+def assemble_system(a, L, bcs=None, x0=None, add_values=False, finalize_tensor=True,
+                    keep_diagonal=False, A_tensor=None, b_tensor=None, scatter=_lib.SCATTER_AUTO):
+    """python/dolfin/fem/assembling.py:322-385 (same argument order, without the form
+    compiler parameters and backend): flags set on the SystemAssembler, x0 selects the
+    incremental assemble(A, b, x0)."""
     A = Matrix() if A_tensor is None else A_tensor
     b = Vector() if b_tensor is None else b_tensor
-    SystemAssembler(a, L, bcs, scatter).assemble(A, b)
+    asm = SystemAssembler(a, L, bcs, scatter)
+    asm.add_values = add_values
+    asm.finalize_tensor = finalize_tensor
+    asm.keep_diagonal = keep_diagonal
+    asm.assemble(A, b, x0=x0)
     return A, b

@@ -25,7 +25,11 @@ parameters = {"reorder_dofs_serial": False}
 
 def node_reordering(cell_nodes, num_nodes):
     """old_to_new of DOLFIN's reverse Cuthill-McKee node renumbering (DofMapBuilder::
-    compute_node_reordering, DofMapBuilder.cpp:1473-1569, Boost ordering library)."""
+    compute_node_reordering, DofMapBuilder.cpp:1473-1569, Boost ordering library). Equal to
+    Boost's order when no vertex queues more than 16 new neighbours at once (P1 on the Kuhn
+    meshes of UnitCubeMesh: at most 14 neighbours); beyond that, equal-degree neighbours are
+    ordered by libstdc++'s std::sort, as a GCC build of Boost orders them, with no test to pin
+    it."""
     cn = np.ascontiguousarray(cell_nodes, dtype=np.int32)
     out = np.empty(int(num_nodes), dtype=np.int32)
     _lib.check(_lib.load().dfa_node_reordering(cn.shape[0], cn.shape[1], _lib.ptr(cn),
@@ -74,7 +78,14 @@ class FunctionSpace:
             nn = nv + ne
             nodes = np.concatenate([mesh.cells(), nv + ce], axis=1).astype(np.int64)  # [nc, 10]
             if reorder:
-                node_permutation = node_reordering(nodes, nn)
+                # DOLFIN orders these nodes with Boost's reverse Cuthill-McKee, whose order of
+                # equal-degree neighbours for the long BFS queues of the P2 node graph (a node
+                # has up to ~60 neighbours) is not reproduced bit for bit here; DOLFIN's own
+                # permutation is passed in instead (its DofMap gives it)
+                raise ValueError("reorder_dofs_serial for vector P2: pass DOLFIN's node "
+                                 "permutation as node_permutation (this build's reverse "
+                                 "Cuthill-McKee is node_reordering(), a locality ordering "
+                                 "with no claim to equal Boost's for queues over 16)")
             if node_permutation is None:
                 cd = np.concatenate([c * nn + nodes for c in range(3)], axis=1)
             else:
@@ -99,6 +110,16 @@ class FunctionSpace:
     def cell_dofs(self):
         return self._cell_dofs
 
+    def sub(self, i):
+        """Component i of a vector space (FunctionSpace::sub): its dofmap is the columns
+        i*10 .. i*10 + 9 of the component-major cell dofs (python/test/unit/fem/
+        test_dofmap.py:112-120)."""
+        if self.element != _lib.ELEMENT_VECTOR_P2:
+            raise ValueError("sub() needs a vector space")
+        if not 0 <= i < 3:
+            raise IndexError("component index out of range")
+        return _SubSpace(self, i)
+
     def coefficient_space(self):
         """P1 space whose dofmap addresses the P1 coefficients of forms on this space."""
         if self.element == _lib.ELEMENT_P1:
@@ -110,6 +131,19 @@ class FunctionSpace:
     def __repr__(self):
         kind = "VectorP2" if self.vector else "P1"
         return f"FunctionSpace({kind}, dim={self._dim})"
+
+
+class _SubSpace:
+    """A component of a vector P2 space: dofs of one component, numbered in the parent."""
+
+    def __init__(self, parent, i):
+        self._parent, self._i = parent, i
+
+    def mesh(self):
+        return self._parent.mesh()
+
+    def cell_dofs(self):
+        return self._parent.cell_dofs()[:, 10 * self._i:10 * self._i + 10]
 
 
 def VectorFunctionSpace(mesh, family="P", degree=2, node_permutation=None, reorder=None):

@@ -40,9 +40,24 @@ class Matrix:
     def nnz(self):
         return 0 if self._engine is None else self._engine.nnz
 
+    def local_range(self, dim=0):
+        """GenericTensor::local_range (GenericTensor.h:69): one process owns every row."""
+        return (0, self.size(dim))
+
     def zero(self):
         if self.values is not None:
             self.values.zero_()
+
+    def add_local(self, block, rows, cols):
+        """GenericMatrix::add_local(block, rows) from the host (GenericMatrix.h:106-112;
+        EigenMatrix::add, EigenMatrix.cpp:156-166): A[rows[i], cols[j]] += block[i, j]. Every
+        entry must be in the sparsity pattern."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        blk = np.ascontiguousarray(block, dtype=np.float64).reshape(r.shape[0], c.shape[0])
+        _lib.check(_lib.load().dfa_csr_add_local(self._engine.handle, _lib.ptr(self.values),
+                                                 r.shape[0], _lib.ptr(r), c.shape[0], _lib.ptr(c),
+                                                 _lib.ptr(blk), _lib.stream()))
 
     def apply(self, mode="add"):
         return None
@@ -104,9 +119,19 @@ class Vector:
     def size(self):
         return 0 if self.values is None else self.values.shape[0]
 
+    def local_range(self):
+        return (0, self.size())
+
     def zero(self):
         if self.values is not None:
             self.values.zero_()
+
+    def add_local(self, block, rows):
+        """GenericVector::add_local (EigenVector.cpp:160-165): b[rows[i]] += block[i]."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        blk = np.ascontiguousarray(block, dtype=np.float64).reshape(r.shape[0])
+        _lib.check(_lib.load().dfa_vector_add_local(_lib.ptr(self.values), self.size(), r.shape[0],
+                                                    _lib.ptr(r), _lib.ptr(blk), _lib.stream()))
 
     def apply(self, mode="add"):
         return None
@@ -128,8 +153,13 @@ def solve(A, x, b, method="cg", preconditioner="jacobi", rtol=1e-10, atol=0.0, m
           check_every=10):
     """Solve A x = b on the device (x holds the initial guess and receives the solution).
     method "cg" with preconditioner "jacobi" or "none" (EigenKrylovSolver's methods, dolfin/la/
-    EigenKrylovSolver.cpp:43-56); stops when |r| <= max(rtol |b|, atol). Returns the number
-    of iterations (dolfin::solve returns the same count); raises when it does not converge."""
+    EigenKrylovSolver.cpp:43-56); stops at the first iteration with |r| <= max(rtol |b|, atol).
+    Returns that iteration count (dolfin::solve returns the same count); raises when the method
+    does not converge or breaks down (p.Ap = 0 with r != 0).
+
+    The residual is tested on the device every iteration; the host reads the result every
+    check_every iterations, and the iterate is frozen from the converged iteration on, so the
+    answer and the count do not depend on check_every."""
     if method != "cg":
         raise ValueError("only the conjugate gradient method is provided")
     if preconditioner not in ("jacobi", "none", "default"):
@@ -147,25 +177,37 @@ def solve(A, x, b, method="cg", preconditioner="jacobi", rtol=1e-10, atol=0.0, m
     r = torch.empty_like(bv)
     r.copy_(bv)
     A.mult(xv, r, alpha=-1.0, beta=1.0)          # r = b - A x
+    tol = max(rtol * float(torch.linalg.vector_norm(bv)), atol)
+    tol2 = tol * tol
+    if float(torch.dot(r, r)) <= tol2:
+        return 0
     z = r * dinv if dinv is not None else r.clone()
     p = z.clone()
     rz = torch.dot(r, z)
-    tol = max(rtol * float(torch.linalg.vector_norm(bv)), atol)
     Ap = torch.empty_like(bv)
-    it = 0
-    while it < maxiter:
-        if it % check_every == 0 and float(torch.linalg.vector_norm(r)) <= tol:
-            return it
+    zero = torch.zeros((), dtype=bv.dtype, device=bv.device)
+    done = torch.full((), -1, dtype=torch.int64, device=bv.device)    # converged iteration
+    broke = torch.zeros((), dtype=torch.bool, device=bv.device)
+    for it in range(1, maxiter + 1):
+        active = done < 0
         A.mult(p, Ap)
-        alpha = rz / torch.dot(p, Ap)
+        pAp = torch.dot(p, Ap)
+        ok = active & (pAp != 0)
+        broke |= active & (pAp == 0)
+        alpha = torch.where(ok, rz / torch.where(ok, pAp, 1.0), zero)
         xv.add_(alpha * p)
         r.sub_(alpha * Ap)
         z = r * dinv if dinv is not None else r
         rz_new = torch.dot(r, z)
-        p.mul_(rz_new / rz).add_(z)
+        beta = torch.where(rz != 0, rz_new / torch.where(rz != 0, rz, 1.0), zero)
+        p.mul_(beta).add_(z)
         rz = rz_new
-        it += 1
-    if float(torch.linalg.vector_norm(r)) <= tol:
-        return it
+        rr = torch.dot(r, r)
+        done = torch.where(active & ~broke & (rr <= tol2), torch.full_like(done, it), done)
+        if it % check_every == 0 or it == maxiter:
+            dn = int(done)
+            if dn >= 0:
+                return dn
+            if bool(broke):
+                raise RuntimeError(f"conjugate gradients broke down (p.Ap = 0) at iteration <= {it}")
     raise RuntimeError(f"conjugate gradients did not converge in {maxiter} iterations")
-

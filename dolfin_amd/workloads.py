@@ -76,3 +76,56 @@ def vp2_algorithmic_bytes(num_cells, num_vertices, nnz, num_rows):
     written once, col and row_ptr read once."""
     return (40 * num_cells + 24 * num_vertices + 16 * num_vertices + 8 * nnz + 4 * nnz
             + 4 * (num_rows + 1))
+
+
+def batch_members(nmembers=256, seed=5, log=None):
+    """The batched workload: nmembers UnitCubeMesh(n) instances, n ~ U{8..48} drawn from
+    default_rng(seed), even members P1 kappa-Poisson, odd members vector-P2 elasticity, each
+    with its own seeded coefficients (p1_poisson_3d / vp2_elasticity_3d). Returns the list of
+    bilinear forms."""
+    from .fem import elasticity_form, poisson_form
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(8, 49, nmembers)
+    forms = []
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            mesh, V, kappa, f = p1_poisson_3d(int(n), seed=int(rng.integers(1 << 30)))
+            forms.append(poisson_form(V, to_function(V, kappa)))
+        else:
+            mesh, V, mu, lmbda, f3 = vp2_elasticity_3d(int(n), seed=int(rng.integers(1 << 30)))
+            Q = V.coefficient_space()
+            forms.append(elasticity_form(V, to_function(Q, mu), to_function(Q, lmbda)))
+        if log is not None and (i + 1) % 32 == 0:
+            log(f"batch: {i + 1}/{nmembers} members built")
+    return forms
+
+
+def vp2_node_coordinates(mesh):
+    """Coordinates of the vector-P2 nodes: the vertices, then the edge midpoints."""
+    x = mesh.coordinates()
+    ce, ne = mesh.cell_edges()
+    c = mesh.cells()
+    loc = np.array([[2, 3], [1, 3], [1, 2], [0, 3], [0, 2], [0, 1]])
+    ex = np.zeros((ne, 3))
+    ex[ce.ravel()] = (0.5 * (x[c[:, loc[:, 0]]] + x[c[:, loc[:, 1]]])).reshape(-1, 3)
+    return np.vstack([x, ex])
+
+
+def rigid_body_modes(V):
+    """The six rigid-body displacements (three translations, three rotations about the
+    cube's centre) as dof vectors of the vector-P2 space V (P2 interpolates them exactly),
+    for UFC (component-major) and node-permuted numberings alike."""
+    X = vp2_node_coordinates(V.mesh())
+    nn = X.shape[0]
+    ce, ne = V.mesh().cell_edges()
+    nodes = np.concatenate([V.mesh().cells(), V.mesh().num_vertices() + ce], axis=1)
+    cd = V.cell_dofs()
+    modes = [np.eye(3)[k] * np.ones((nn, 3)) for k in range(3)]
+    modes += [np.cross(axis, X - 0.5) for axis in np.eye(3)]
+    out = []
+    for m in modes:
+        r = np.zeros(V.dim())
+        for comp in range(3):
+            r[cd[:, 10 * comp:10 * comp + 10].ravel()] = m[nodes.ravel(), comp]
+        out.append(r)
+    return out

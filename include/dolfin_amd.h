@@ -122,13 +122,42 @@ int dfa_assemble_vector(dfa_assembler h, int32_t form, const double* d_x, const 
  * SystemAssembler::apply_bc (dolfin/fem/SystemAssembler.cpp:303-492,1051-1075).
  *   d_bc_mask [num_dofs] uint8 (nonzero = Dirichlet dof), d_bc_value [num_dofs] (may be NULL
  *   when there are no boundary conditions). d_values or d_b may be NULL (assemble(A) or
- *   assemble(b) only).
+ *   assemble(b) only). flags: DFA_ADD_VALUES adds to A and b instead of overwriting them
+ *   (AssemblerBase::add_values, which SystemAssembler honours through init_global_tensor,
+ *   dolfin/fem/AssemblerBase.cpp:137-285).
  */
 int dfa_assemble_system(dfa_assembler h, int32_t a_form, int32_t L_form, const double* d_x,
                         const double* d_a_c0, const double* d_a_c1, const double* d_L_c0,
                         const double* h_f3, const uint8_t* d_bc_mask, const double* d_bc_value,
                         double* d_values, double* d_b, int32_t scatter, int32_t flags,
                         void* stream);
+
+/*
+ * SystemAssembler::assemble(A, b, x0) / assemble(b, x0) (dolfin/fem/SystemAssembler.h:79,84):
+ * the same pass as dfa_assemble_system for an incremental (Newton) solve. Every Dirichlet
+ * value g becomes x0[dof] - g before the cell loop (SystemAssembler.cpp:252-275), so the
+ * solution increment du of A du = b makes x0 - du meet the boundary condition.
+ *   d_x0 [num_dofs] (device). Without Dirichlet data (d_bc_mask NULL) x0 changes nothing.
+ * The shifted values live in a buffer of the handle (allocated on the first call).
+ */
+int dfa_assemble_system_x0(dfa_assembler h, int32_t a_form, int32_t L_form, const double* d_x,
+                           const double* d_a_c0, const double* d_a_c1, const double* d_L_c0,
+                           const double* h_f3, const uint8_t* d_bc_mask, const double* d_bc_value,
+                           const double* d_x0, double* d_values, double* d_b, int32_t scatter,
+                           int32_t flags, void* stream);
+
+/* The boundary-value modification of SystemAssembler.cpp:252-275 on its own, for callers
+   that pass the modified values to several passes (cell and exterior-facet):
+   d_out[i] = d_bc_mask[i] ? d_x0[i] - d_bc_value[i] : 0, i < n (device arrays). */
+int dfa_dirichlet_x0(int64_t n, const uint8_t* d_bc_mask, const double* d_bc_value,
+                     const double* d_x0, double* d_out, void* stream);
+
+/* Introspection for tests and benchmarks (no counterpart in the reference): the name and
+   template arguments of the kernel dfa_assemble_system (a_form, L_form >= 0), or
+   dfa_assemble_matrix (L_form -1) / dfa_assemble_vector (a_form -1), would launch for this
+   handle, with or without Dirichlet data, written to name[len]. Launches nothing. */
+int dfa_pass_kernel(dfa_assembler h, int32_t a_form, int32_t L_form, int32_t has_bc,
+                    int32_t scatter, char* name, int32_t len);
 
 /*
  * Adjoint (reverse-mode) assembly of J = u^T K(c, x) u - F(x)^T u without forming K.
@@ -230,6 +259,22 @@ int dfa_assemble_facets(dfa_assembler h, int32_t a_form, int32_t L_form, const d
  */
 int dfa_csr_spmv(dfa_assembler h, const double* d_values, double alpha, const double* d_x,
                  double beta, double* d_y, void* stream);
+
+/*
+ * GenericMatrix::add_local(block, rows) from the host (dolfin/la/GenericMatrix.h:106-112,
+ * EigenMatrix::add, dolfin/la/EigenMatrix.cpp:156-166): d_values[(h_rows[i], h_cols[j])] +=
+ * h_block[i*n + j] for the dense m x n block. Every entry must be in the handle's sparsity
+ * pattern (the pattern is fixed once built; nothing is inserted): otherwise the call fails and
+ * changes nothing. Synchronous (returns after the addition is done on `stream`).
+ */
+int dfa_csr_add_local(dfa_assembler h, double* d_values, int32_t m, const int32_t* h_rows,
+                      int32_t n, const int32_t* h_cols, const double* h_block, void* stream);
+
+/* GenericVector::add_local(block, m, rows) from the host (EigenVector::add_local,
+   dolfin/la/EigenVector.cpp:160-165): d_b[h_rows[i]] += h_block[i], 0 <= h_rows[i] < size.
+   Synchronous. */
+int dfa_vector_add_local(double* d_b, int64_t size, int32_t m, const int32_t* h_rows,
+                         const double* h_block, void* stream);
 
 /* The diagonal of that matrix (0 where the pattern has no diagonal entry): the Jacobi
    preconditioner of EigenKrylovSolver (dolfin/la/EigenKrylovSolver.cpp:52-56). */

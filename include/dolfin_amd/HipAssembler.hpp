@@ -6,11 +6,17 @@
 // to the GPU:
 //   HipAssembler         add_values / finalize_tensor / keep_diagonal flags
 //                        (dolfin/fem/AssemblerBase.h:46-67), assemble(tensor, form)
-//   HipSystemAssembler   (a, L, Dirichlet values) -> assemble(A, b) / (A) / (b)
-//   DeviceCSRMatrix      the GenericMatrix subset the assembler uses: empty, size, nnz, zero,
-//                        apply, norm("frobenius"), data(), mult, diagonal
-//                        (dolfin/la/EigenMatrix.cpp:77-105, 180-186, 230-255, 341-355, 417-420)
-//   DeviceVector         GenericVector subset: size, zero, apply, norm("l2"), get_local
+//   HipSystemAssembler   (a, L, Dirichlet values) -> assemble(A, b) / (A) / (b) and the
+//                        incremental assemble(A, b, x0) / (b, x0) (SystemAssembler.h:63-84),
+//                        with the same three flags
+//   TensorLayout         the sparsity a tensor is initialised from (GenericTensor::init,
+//                        dolfin/la/GenericTensor.h:58; dolfin/la/TensorLayout.h)
+//   DeviceCSRMatrix      the GenericMatrix subset the assembler uses: init(TensorLayout), empty,
+//                        size, local_range, nnz, zero, add_local, apply, norm("frobenius"),
+//                        data(), mult, diagonal (dolfin/la/GenericMatrix.h:56-255,
+//                        EigenMatrix.cpp:77-105, 156-186, 230-255, 341-355, 384-420)
+//   DeviceVector         GenericVector subset: size, local_range, zero, add_local, apply,
+//                        norm("l2"), get_local
 // Errors from the library are rethrown as std::runtime_error with the library's message,
 // like dolfin_error (dolfin/log/Logger.cpp:142-185).
 #pragma once
@@ -22,6 +28,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../dolfin_amd.h"
@@ -118,9 +125,22 @@ struct Form {
   double f3[3] = {0, 0, 0};       // constant body force
 };
 
+// The layout a matrix is initialised from: DOLFIN's TensorLayout holds the index maps and the
+// sparsity pattern (dolfin/la/TensorLayout.h, built by AssemblerBase::init_global_tensor,
+// dolfin/fem/AssemblerBase.cpp:137-285); here the Space holds the device CSR structure built
+// once from the dofmap, and the layout names it.
+struct TensorLayout {
+  std::shared_ptr<const Space> space;
+  explicit TensorLayout(std::shared_ptr<const Space> s) : space(std::move(s)) {}
+  int64_t size(int) const { return space->num_rows(); }
+};
+
 class DeviceCSRMatrix {
  public:
   bool empty() const { return space_ == nullptr; }
+  // GenericTensor::init(const TensorLayout&) (dolfin/la/GenericTensor.h:58): device values
+  // for the layout's pattern, zeroed (EigenMatrix::init, EigenMatrix.cpp:384-410).
+  void init(const TensorLayout& layout) { init(layout.space); }
   void init(std::shared_ptr<const Space> space) {
     space_ = std::move(space);
     values_.resize(static_cast<size_t>(space_->nnz()));
@@ -128,9 +148,19 @@ class DeviceCSRMatrix {
   }
   const Space* space() const { return space_.get(); }
   int64_t size(int) const { return space_ ? space_->num_rows() : 0; }
+  // GenericTensor::local_range: one process owns every row (serial DOLFIN semantics).
+  std::pair<int64_t, int64_t> local_range(int dim) const { return {0, size(dim)}; }
   int64_t nnz() const { return space_ ? space_->nnz() : 0; }
   void zero(hipStream_t s = nullptr) {
     if (values_.size()) hip(hipMemsetAsync(values_.get(), 0, values_.size() * sizeof(double), s), "hipMemsetAsync");
+  }
+  // GenericMatrix::add_local(block, rows) (GenericMatrix.h:106-112) from host data: the dense
+  // block[i * cols.size() + j] is added at (rows[i], cols[j]); every entry must be in the
+  // pattern. Synchronous. Assembly itself never calls this: it scatters on the device.
+  void add_local(const double* block, const std::vector<int32_t>& rows,
+                 const std::vector<int32_t>& cols, hipStream_t s = nullptr) {
+    check(dfa_csr_add_local(space_->handle(), values_.get(), static_cast<int32_t>(rows.size()),
+                            rows.data(), static_cast<int32_t>(cols.size()), cols.data(), block, s));
   }
   void apply(const std::string&) {}
   double* values() { return values_.get(); }
@@ -171,11 +201,18 @@ class DeviceVector {
     zero();
   }
   int64_t size() const { return static_cast<int64_t>(values_.size()); }
+  std::pair<int64_t, int64_t> local_range() const { return {0, size()}; }
   void zero(hipStream_t s = nullptr) {
     if (values_.size()) hip(hipMemsetAsync(values_.get(), 0, values_.size() * sizeof(double), s), "hipMemsetAsync");
   }
+  // GenericVector::add_local(block, m, rows) (EigenVector.cpp:160-165) from host data.
+  void add_local(const double* block, const std::vector<int32_t>& rows, hipStream_t s = nullptr) {
+    check(dfa_vector_add_local(values_.get(), size(), static_cast<int32_t>(rows.size()), rows.data(),
+                               block, s));
+  }
   void apply(const std::string&) {}
   double* values() { return values_.get(); }
+  const double* values() const { return values_.get(); }
   std::vector<double> get_local() const { return values_.download(); }
   double norm(const std::string& type) const {
     if (type != "l2") throw std::runtime_error("only the l2 norm is provided");
@@ -276,8 +313,10 @@ class HipAssembler : public AssemblerFlags {
 };
 
 // SystemAssembler with one set of Dirichlet values (bc_mask/bc_value on the device, one
-// entry per dof), applied per cell as in SystemAssembler::apply_bc.
-class HipSystemAssembler {
+// entry per dof), applied per cell as in SystemAssembler::apply_bc. The AssemblerBase flags
+// add_values / finalize_tensor / keep_diagonal apply as in the reference (keep_diagonal is
+// fixed when the Space is created).
+class HipSystemAssembler : public AssemblerFlags {
  public:
   HipSystemAssembler(Form a, Form L, const uint8_t* d_bc_mask = nullptr,
                      const double* d_bc_value = nullptr, int32_t scatter = DFA_SCATTER_AUTO,
@@ -289,23 +328,42 @@ class HipSystemAssembler {
       throw std::runtime_error("Dirichlet mask and values go together");
   }
 
-  void assemble(DeviceCSRMatrix& A, DeviceVector& b) { run(&A, &b); }
-  void assemble(DeviceCSRMatrix& A) { run(&A, nullptr); }
-  void assemble(DeviceVector& b) { run(nullptr, &b); }
+  void assemble(DeviceCSRMatrix& A, DeviceVector& b) { run(&A, &b, nullptr); }
+  void assemble(DeviceCSRMatrix& A) { run(&A, nullptr, nullptr); }
+  void assemble(DeviceVector& b) { run(nullptr, &b, nullptr); }
+  // Incremental variants (SystemAssembler.h:79,84): Dirichlet values become x0[dof] - g
+  // (SystemAssembler.cpp:252-275).
+  void assemble(DeviceCSRMatrix& A, DeviceVector& b, const DeviceVector& x0) { run(&A, &b, &x0); }
+  void assemble(DeviceVector& b, const DeviceVector& x0) { run(nullptr, &b, &x0); }
 
  private:
-  void run(DeviceCSRMatrix* A, DeviceVector* b) {
+  void run(DeviceCSRMatrix* A, DeviceVector* b, const DeviceVector* x0) {
     if (A && (A->empty() || A->space() != a_.space.get())) A->init(a_.space);
     if (b && (b->empty() || b->size() != a_.space->num_rows())) b->init(a_.space->num_rows());
+    if (x0 && x0->size() != a_.space->num_rows())
+      throw std::runtime_error("x0 must have one entry per dof");
+    const int32_t flags = add_values ? DFA_ADD_VALUES : 0;
+    const double* bcv = value_;
+    if (x0 && mask_) {
+      // the cell pass and the facet pass use the same shifted values
+      shifted_.resize(static_cast<size_t>(a_.space->num_rows()));
+      check(dfa_dirichlet_x0(a_.space->num_rows(), mask_, value_, x0->values(), shifted_.get(),
+                             stream_));
+      bcv = shifted_.get();
+    }
     check(dfa_assemble_system(a_.space->handle(), a_.kind, L_.kind, a_.x, a_.c0, a_.c1, L_.c0,
-                              L_.f3, mask_, value_, A ? A->values() : nullptr,
-                              b ? b->values() : nullptr, scatter_, 0, stream_));
+                              L_.f3, mask_, bcv, A ? A->values() : nullptr,
+                              b ? b->values() : nullptr, scatter_, flags, stream_));
     // exterior-facet tensors join their cell's tensor before apply_bc
     // (dolfin/fem/SystemAssembler.cpp:421-481)
     if (!ds_.empty())
       check(dfa_assemble_facets(a_.space->handle(), ds_.a_form(), ds_.L_form(), a_.x, ds_.alpha,
-                                ds_.L_data(), ds_.facet_mask, mask_, value_,
+                                ds_.L_data(), ds_.facet_mask, mask_, bcv,
                                 A ? A->values() : nullptr, b ? b->values() : nullptr, stream_));
+    if (finalize_tensor) {
+      if (A) A->apply("add");
+      if (b) b->apply("add");
+    }
   }
   Form a_, L_;
   const uint8_t* mask_;
@@ -313,6 +371,7 @@ class HipSystemAssembler {
   int32_t scatter_;
   hipStream_t stream_;
   FacetTerms ds_;
+  DeviceArray<double> shifted_;
 };
 
 }  // namespace dolfin_amd

@@ -231,9 +231,14 @@ def assemble_vector(kind, x, cells, cell_dofs, ndofs, coeff_dofs=None, c0=None, 
 
 def assemble_system(a_kind, L_kind, x, cells, cell_dofs, row_ptr, col, bc_mask, bc_value,
                     coeff_dofs=None, a_c0=None, a_c1=None, L_c0=None, f3=None,
-                    want_A=True, want_b=True, threads=0):
+                    want_A=True, want_b=True, threads=0, x0=None):
     """threads=0: the serial loop (ascending cells); threads>0: the same loop on that many host
-    threads with atomic scatter (for CPU timing; equal up to summation order)."""
+    threads with atomic scatter (for CPU timing; equal up to summation order).
+    x0: SystemAssembler::assemble(A, b, x0), dolfin/fem/SystemAssembler.cpp:252-275 -- before
+    the cell loop every boundary value g of a Dirichlet dof becomes x0[dof] - g."""
+    if x0 is not None:
+        m = np.asarray(bc_mask) != 0
+        bc_value = np.where(m, np.asarray(x0, dtype=np.float64) - np.asarray(bc_value, dtype=np.float64), 0.0)
     x = _c(x, np.float64)
     cells = _c(cells, np.int32)
     cd = _c(cell_dofs, np.int32)
@@ -313,8 +318,12 @@ def assemble_facets(a_kind, L_kind, x, cells, cell_dofs, facets, row_ptr=None, c
 
 def assemble_system_facets(a_kind, L_kind, fa_kind, fL_kind, x, cells, cell_dofs, facets,
                            row_ptr, col, bc_mask, bc_value, coeff_dofs=None, a_c0=None,
-                           L_c0=None, alpha=1.0, g=None, facet_mask=None):
-    """SystemAssembler with the facet tensors merged into each cell tensor before apply_bc."""
+                           L_c0=None, alpha=1.0, g=None, facet_mask=None, x0=None):
+    """SystemAssembler with the facet tensors merged into each cell tensor before apply_bc;
+    x0 as in assemble_system (SystemAssembler.cpp:252-275)."""
+    if x0 is not None:
+        m = np.asarray(bc_mask) != 0
+        bc_value = np.where(m, np.asarray(x0, dtype=np.float64) - np.asarray(bc_value, dtype=np.float64), 0.0)
     x = _c(x, np.float64)
     cells = _c(cells, np.int32)
     cd = _c(cell_dofs, np.int32)
@@ -410,10 +419,18 @@ def _bfs_levels(adj, s):
     return ecc, last_min
 
 
+MAX_QUEUE = 16
+
+
 def _stable_by_degree(adj, seq):
-    # Boost sorts each vertex's newly queued neighbours with std::sort by degree; for the
-    # short lists of these small test graphs (at most 16) libstdc++ uses insertion sort,
-    # which keeps equal degrees in queue order, as this stable sort does.
+    # Boost sorts each vertex's newly queued neighbours with std::sort by degree. For lists of
+    # at most 16 libstdc++'s std::sort is an insertion sort, which keeps equal degrees in queue
+    # order, as this stable sort does. Longer lists go through introsort, whose order of equal
+    # degrees this oracle does not restate: it refuses them rather than guess (parity of the
+    # reverse Cuthill-McKee numbering is unpinned there).
+    if len(seq) > MAX_QUEUE:
+        raise ValueError(f"a vertex queues {len(seq)} > {MAX_QUEUE} new neighbours at once: "
+                         "Boost's order of equal degrees is not restated for such graphs")
     return sorted(seq, key=lambda v: len(adj[v]))
 
 

@@ -43,6 +43,8 @@ static void box_mesh(int n, std::vector<double>& x, std::vector<int32_t>& cells)
 
 static bool close(double a, double b) { return std::fabs(a - b) < 1e-10; }
 
+static std::vector<double> h0_of(int64_t n) { return std::vector<double>(static_cast<size_t>(n), 0.0); }
+
 int main() {
   bool ok = true;
   {
@@ -77,6 +79,71 @@ int main() {
     ok &= close(nA, 96.847818767384) && close(nb, 96.564760289080);
     sys.assemble(A);  // re-assembly of A only
     ok &= close(A.norm("frobenius"), 96.847818767384);
+    // add_values (AssemblerBase.h:53): a second pass adds instead of overwriting
+    sys.add_values = true;
+    sys.assemble(A, b);
+    sys.add_values = false;
+    std::printf("add_values: |A| = %.12f |b| = %.12f\n", A.norm("frobenius"), b.norm("l2"));
+    ok &= close(A.norm("frobenius"), 2 * 96.847818767384) && close(b.norm("l2"), 2 * 96.564760289080);
+    // incremental assembly (SystemAssembler.h:79,84): x0 equal to the boundary values gives
+    // the system of the homogeneous condition (x0 - g = 0 on the boundary)
+    DeviceVector x0, bx, bh;
+    x0.init(nv);
+    x0.add_local(hg.data(), [&] {
+      std::vector<int32_t> r(nv);
+      for (int64_t v = 0; v < nv; ++v) r[v] = static_cast<int32_t>(v);
+      return r;
+    }());
+    DeviceCSRMatrix Ax, Ah;
+    sys.assemble(Ax, bx, x0);
+    DeviceArray<double> g0;
+    g0.upload(h0_of(nv).data(), nv);
+    HipSystemAssembler hom(a, L, mask.get(), g0.get());
+    hom.assemble(Ah, bh);
+    const std::vector<double> vbx = bx.get_local(), vbh = bh.get_local();
+    double db = 0.0, bcmax = 0.0;
+    for (int64_t v = 0; v < nv; ++v) {
+      db = std::fmax(db, std::fabs(vbx[v] - vbh[v]));
+      if (hm[v]) bcmax = std::fmax(bcmax, std::fabs(vbx[v]));
+    }
+    std::printf("x0 = g: max|b(x0) - b(g=0)| = %.3e, max|b| on the boundary = %.3e, |A| = %.12f\n",
+                db, bcmax, Ax.norm("frobenius"));
+    ok &= db < 1e-13 && bcmax == 0.0 && close(Ax.norm("frobenius"), 96.847818767384);
+    DeviceVector bx2;
+    sys.assemble(bx2, x0);  // the vector-only variant
+    const std::vector<double> vbx2 = bx2.get_local();
+    double db2 = 0.0;
+    for (int64_t v = 0; v < nv; ++v) db2 = std::fmax(db2, std::fabs(vbx2[v] - vbx[v]));
+    ok &= db2 < 1e-13;
+    // GenericTensor::init(TensorLayout), local_range, add_local (host blocks)
+    DeviceCSRMatrix M;
+    M.init(TensorLayout(V));
+    const auto lr = M.local_range(0);
+    ok &= lr.first == 0 && lr.second == nv && M.nnz() == V->nnz();
+    const double blk[4] = {1.0, 2.0, 3.0, 4.0};
+    M.add_local(blk, {0, 1}, {0, 1});
+    M.add_local(blk, {0}, {0});
+    std::vector<int32_t> rp, cc;
+    std::vector<double> mv;
+    M.data(rp, cc, mv);
+    double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0, rest = 0.0;
+    for (int32_t r = 0; r < 2; ++r)
+      for (int32_t k = rp[r]; k < rp[r + 1]; ++k) {
+        if (r == 0 && cc[k] == 0) s00 = mv[k];
+        else if (r == 0 && cc[k] == 1) s01 = mv[k];
+        else if (r == 1 && cc[k] == 0) s10 = mv[k];
+        else if (r == 1 && cc[k] == 1) s11 = mv[k];
+        else rest += std::fabs(mv[k]);
+      }
+    bool refused = false;
+    try {
+      M.add_local(blk, {0}, {static_cast<int32_t>(nv - 1)});  // (0, last vertex): not a pair of one cell
+    } catch (const std::runtime_error&) {
+      refused = true;
+    }
+    std::printf("add_local: %.1f %.1f %.1f %.1f, entry outside the pattern refused: %s\n", s00, s01,
+                s10, s11, refused ? "yes" : "no");
+    ok &= s00 == 2.0 && s01 == 2.0 && s10 == 3.0 && s11 == 4.0 && rest == 0.0 && refused;
     // 0*v*dx + 1*v*ds: the entries sum to the boundary measure 6.0
     // (python/test/unit/fem/test_assembler.py:140-153)
     DeviceArray<double> zero;

@@ -122,3 +122,32 @@ def test_cg_poisson_nodal_known_answer():
     dfa.solve(A, x, b, "cg", "jacobi", rtol=1e-13)
     assert nv == V.dim()
     assert np.max(np.abs(x.get_local() - exact)) < 1e-10
+
+
+def test_cg_diagonal_system_stops_exactly():
+    # every dof Dirichlet: A is the identity and b the boundary values. Jacobi-CG reaches r = 0
+    # at iteration 1; later iterations must not divide 0 by 0 (p.Ap = 0), and the count is the
+    # converged iteration, whatever the host check interval
+    mesh, V, kappa, f = workloads.p1_poisson_3d(4, seed=2)
+    g = np.random.default_rng(3).uniform(-1, 1, V.dim())
+    bc = dfa.DirichletBC(V, g, dofs=np.arange(V.dim()))
+    A, b = dfa.assemble_system(dfa.poisson_form(V, workloads.to_function(V, kappa)),
+                               dfa.load_form(V, workloads.to_function(V, f)), bc)
+    for every in (1, 10):
+        x = dfa.Vector(V.dim())
+        it = dfa.solve(A, x, b, "cg", "jacobi", rtol=1e-12, check_every=every)
+        assert it == 1
+        xv = x.get_local()
+        assert np.all(np.isfinite(xv)) and np.max(np.abs(xv - g)) < 1e-15
+    # the count does not depend on the check interval on a real system either
+    bc2 = dfa.DirichletBC(V, 0.5)
+    A2, b2 = dfa.assemble_system(dfa.poisson_form(V, workloads.to_function(V, kappa)),
+                                 dfa.load_form(V, workloads.to_function(V, f)), bc2)
+    its = set()
+    sols = []
+    for every in (1, 7, 10):
+        x = dfa.Vector(V.dim())
+        its.add(dfa.solve(A2, x, b2, "cg", "jacobi", rtol=1e-10, check_every=every))
+        sols.append(x.get_local())
+    assert len(its) == 1
+    assert all(np.array_equal(sols[0], s) for s in sols[1:])

@@ -310,3 +310,39 @@ def test_shape_derivative_identity_p1(oracle):
         gV = V[c].T @ G  # grad V, gV[i, j] = d V_i / d x_j
         total += vol * (-2 * gu @ gV @ gu + gu @ gu * np.trace(gV))
     assert abs(total - np.sum(dx * V)) < 1e-12 * max(1, abs(total))
+
+
+def _dense(rp, col, vals):
+    n = rp.shape[0] - 1
+    A = np.zeros((n, n))
+    A[np.repeat(np.arange(n), np.diff(rp)), col] = vals
+    return A
+
+
+def test_x0_newton_increment(oracle):
+    # SystemAssembler::assemble(A, b, x0) (SystemAssembler.cpp:252-275). The boundary values
+    # become x0 - g, so the increment du of A du = b makes x0 - du meet the condition g (a
+    # sign error would give 2 x0 - g); x0 leaves A unchanged; x0 = g gives the system of the
+    # homogeneous condition.
+    O = oracle
+    x, cells = O.box_mesh(3, 3, 3)
+    n = x.shape[0]
+    rp, col = O.sparsity(cells, n)
+    bc = O.boundary_vertices(x).astype(np.uint8)
+    rng = np.random.default_rng(11)
+    k, f = rng.uniform(0.5, 2, n), rng.uniform(-1, 1, n)
+    g, x0 = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    A, b = O.assemble_system(O.P1_POISSON, O.P1_LOAD, x, cells, cells, rp, col, bc, g,
+                             a_c0=k, L_c0=f)
+    Ax, bx = O.assemble_system(O.P1_POISSON, O.P1_LOAD, x, cells, cells, rp, col, bc, g,
+                               a_c0=k, L_c0=f, x0=x0)
+    np.testing.assert_array_equal(A, Ax)
+    du = np.linalg.solve(_dense(rp, col, Ax), bx)
+    m = bc != 0
+    np.testing.assert_allclose((x0 - du)[m], g[m], rtol=0, atol=1e-12)
+    _, bg = O.assemble_system(O.P1_POISSON, O.P1_LOAD, x, cells, cells, rp, col, bc, g,
+                              a_c0=k, L_c0=f, x0=np.where(m, g, 7.0))
+    _, b0 = O.assemble_system(O.P1_POISSON, O.P1_LOAD, x, cells, cells, rp, col, bc,
+                              np.zeros(n), a_c0=k, L_c0=f)
+    np.testing.assert_array_equal(bg, b0)
+    assert np.all(bg[m] == 0.0)

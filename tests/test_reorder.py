@@ -100,11 +100,15 @@ def test_reordered_spaces():
     V = dfa.FunctionSpace(mesh, "P", 1, reorder=True)
     perm = dfa.node_reordering(mesh.cells(), mesh.num_vertices())
     np.testing.assert_array_equal(V.cell_dofs(), perm[mesh.cells()])
-    W = dfa.VectorFunctionSpace(mesh, "P", 2, reorder=True)
     ce, ne = mesh.cell_edges()
     nodes = np.concatenate([mesh.cells(), mesh.num_vertices() + ce], axis=1)
-    pn = dfa.node_reordering(nodes, mesh.num_vertices() + ne)[nodes]
+    perm2 = dfa.node_reordering(nodes, mesh.num_vertices() + ne)
+    W = dfa.VectorFunctionSpace(mesh, "P", 2, node_permutation=perm2)
+    pn = perm2[nodes]
     np.testing.assert_array_equal(W.cell_dofs(), np.concatenate([3 * pn + c for c in range(3)], axis=1))
+    # DOLFIN's vector-P2 reordering is not claimed: it has to be passed in
+    with pytest.raises(ValueError, match="node_permutation"):
+        dfa.VectorFunctionSpace(mesh, "P", 2, reorder=True)
     assert dfa.parameters["reorder_dofs_serial"] is False
     dfa.parameters["reorder_dofs_serial"] = True
     try:
@@ -124,3 +128,48 @@ def test_reordering_edge_cases():
     col = np.array([0], np.int32)  # self loop
     out = np.empty(2, np.int32)
     assert _lib.load().dfa_graph_reordering(2, _p(ptr), _p(col), _p(out)) != 0
+
+
+def test_oracle_refuses_long_queues(oracle):
+    # a star of 18 leaves, started at a leaf (pseudo-peripheral): the centre then queues the
+    # 17 other leaves, of equal degree, at once, where Boost's std::sort leaves their order to
+    # libstdc++'s introsort, which the oracle does not restate
+    adj = _adj(19, [(0, k) for k in range(1, 19)])
+    with pytest.raises(ValueError, match="16"):
+        oracle.reverse_cuthill_mckee(adj)
+    adj17 = _adj(18, [(0, k) for k in range(1, 18)])  # 16 queued at once: still restated
+    assert sorted(oracle.reverse_cuthill_mckee(adj17).tolist()) == list(range(18))
+
+
+def test_kuhn_meshes_stay_in_the_pinned_regime(oracle):
+    # the P1 node graph of UnitCubeMesh's Kuhn tetrahedra: at most 14 neighbours per vertex, so
+    # no BFS step queues more than 16 and the library's order is the oracle's (Boost's)
+    mesh = dfa.UnitCubeMesh(4, 4, 4)
+    adj = oracle.node_graph(mesh.cells(), mesh.num_vertices())
+    assert max(len(a) for a in adj) == 14
+    np.testing.assert_array_equal(dfa.node_reordering(mesh.cells(), mesh.num_vertices()),
+                                  oracle.reverse_cuthill_mckee(adj))
+
+
+def test_vector_sub_dofmaps():
+    # python/test/unit/fem/test_dofmap.py:112-120: the vector cell dofs are the component
+    # sub-dofmaps concatenated, the components share no dof; and each sub-dofmap gives every
+    # node one dof (the same in every cell), distinct nodes distinct dofs
+    mesh = dfa.UnitCubeMesh(2, 3, 2)
+    ce, ne = mesh.cell_edges()
+    nn = mesh.num_vertices() + ne
+    nodes = np.concatenate([mesh.cells(), mesh.num_vertices() + ce], axis=1)
+    perm = np.random.default_rng(4).permutation(nn)
+    for W in (dfa.VectorFunctionSpace(mesh, "P", 2),
+              dfa.VectorFunctionSpace(mesh, "P", 2, node_permutation=perm)):
+        subs = [W.sub(i).cell_dofs() for i in range(3)]
+        np.testing.assert_array_equal(np.concatenate(subs, axis=1), W.cell_dofs())
+        for i in range(3):
+            for j in range(i + 1, 3):
+                assert np.intersect1d(subs[i], subs[j]).size == 0
+            dof_of = np.full(nn, -1, np.int64)
+            dof_of[nodes.ravel()] = subs[i].ravel()
+            np.testing.assert_array_equal(dof_of[nodes], subs[i])      # one dof per node
+            assert np.unique(dof_of).size == nn                         # distinct per node
+        assert np.array_equal(np.unique(W.cell_dofs()),
+                              np.arange(3 * nn))
