@@ -236,7 +236,9 @@ def test_x0_c_abi_entry_points(oracle):
                 _lib.SCATTER_AUTO, 0, _lib.stream()))
         outs.append((A, b))
     torch.cuda.synchronize()
-    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    # same values; the LDS accumulation order of the blocked kernel may differ between passes
+    for u, v in zip(outs[0], outs[1]):
+        assert (torch.linalg.norm(u - v) / torch.linalg.norm(v)).item() < 1e-14
     # NULL x0 is refused by the one-call variant
     assert lib.dfa_assemble_system_x0(eng.handle, _lib.FORM_P1_POISSON, _lib.FORM_P1_LOAD,
                                       _lib.ptr(xd), _lib.ptr(K), None, _lib.ptr(F), None, None,
