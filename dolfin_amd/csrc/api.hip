@@ -241,6 +241,38 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
               t.chain_threads = default_chain_threads();
               t.chain_ptr = h->plan.upload(C.chain_ptr, s);
               t.chain_rec = h->plan.upload(C.chain_rec, s);
+              // fixed-stride slots (host arrays live until the synchronisation below)
+              const int32_t SN = P.max_nodes, SR = std::max<int32_t>(P.max_rows, 1);
+              const int64_t nb = P.num_blocks;
+              std::vector<int4> hdr(static_cast<size_t>(nb));
+              std::vector<int32_t> sv(static_cast<size_t>(nb * SN)), sk;
+              std::vector<int2> sr(static_cast<size_t>(nb * SR), make_int2(0, 0));
+              std::vector<int32_t> sd(static_cast<size_t>(nb * SR), 0);
+              if (!P.kdof_is_vertex) sk.resize(static_cast<size_t>(nb * SN));
+              for (int64_t b = 0; b < nb; ++b) {
+                const int32_t n0 = P.node_ptr[b], nn = P.node_ptr[b + 1] - n0;
+                const int32_t r0 = P.row_ptr[b], nr = P.row_ptr[b + 1] - r0;
+                hdr[b] = make_int4(C.chain_ptr[b], C.chain_ptr[b + 1] - C.chain_ptr[b], nn, nr);
+                for (int32_t i = 0; i < SN; ++i) {
+                  const int32_t j = n0 + (i < nn ? i : 0);
+                  sv[b * SN + i] = P.node_vertex[j];
+                  if (!sk.empty()) sk[b * SN + i] = P.node_kdof[j];
+                }
+                for (int32_t i = 0; i < nr; ++i) {
+                  const int32_t d = P.row_dof[r0 + i];
+                  const int32_t c0 = csr.row_ptr[d], len = csr.row_ptr[d + 1] - c0;
+                  sr[b * SR + i] = make_int2(c0, len | (static_cast<int32_t>(P.row_diag[r0 + i]) << 8));
+                  sd[b * SR + i] = d;
+                }
+              }
+              t.slot_nodes = SN;
+              t.slot_rows = SR;
+              t.blk_hdr = h->plan.upload(hdr, s);
+              t.slot_vertex = h->plan.upload(sv, s);
+              if (!sk.empty()) t.slot_kdof = h->plan.upload(sk, s);
+              t.slot_row = h->plan.upload(sr, s);
+              t.slot_rdof = h->plan.upload(sd, s);
+              hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
             }
             t.row_ptr_b = h->plan.upload(P.row_ptr, s);
             t.row_dof = h->plan.upload(P.row_dof, s);

@@ -164,7 +164,10 @@ int dfa_batch_create(dfa_batch* out, int32_t n, const dfa_assembler* hs, const i
                            ": P1 members must be tetrahedral meshes with a blocked plan",
                        DFA_ERR_ARG);
         it.p1 = h->btopo;
-        it.kind = (it.p1.chain_ptr && it.p1.chain_rec && it.p1.max_chains <= kBatchP1Threads) ? 3 : 1;
+        it.kind = (it.p1.chain_ptr && it.p1.chain_rec && it.p1.blk_hdr &&
+                   it.p1.max_chains <= kBatchP1Threads && it.p1.max_nodes <= kMaxChainBlockNodes)
+                      ? 3
+                      : 1;
         it.p1p.gdim = 3;
         it.p1p.a_form = DFA_FORM_P1_POISSON;
         it.p1p.want_A = true;

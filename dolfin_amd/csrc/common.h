@@ -88,6 +88,9 @@ struct P1BlockPlan {
 // (p, r_k), (q, r_k), (r_k, r_{k+1}) for k = 0..5, then (p, r_m), (q, r_m), (p, q) at
 // bytes 18, 19, 20; [11] unused. Chains are sorted by length (longest first) per block.
 constexpr int kChainLen = 6;
+// Most nodes (owned rows and ghosts) a block of the edge-chain kernels may hold: two node
+// slots per thread of a 512-thread workgroup.
+constexpr int kMaxChainBlockNodes = 1024;
 constexpr int kChainWords = 12;
 struct P1ChainPlan {
   int32_t max_chains = 0;

@@ -56,6 +56,15 @@ struct P1BlockTopo {
   int32_t chain_threads = 256;
   const int32_t* chain_ptr = nullptr;
   const uint32_t* chain_rec = nullptr;
+  // Fixed-stride per-block slots of the edge-chain kernels, padded to the largest block, so
+  // that a block's node ids and owned-row words are addressed from the block index alone
+  // (no dependent load of the block's offsets before them):
+  int32_t slot_nodes = 0, slot_rows = 0;
+  const int4* blk_hdr = nullptr;         // [nb] first chain, chains, nodes, owned rows
+  const int32_t* slot_vertex = nullptr;  // [nb x slot_nodes] (padding repeats the block's node 0)
+  const int32_t* slot_kdof = nullptr;    // [nb x slot_nodes]; NULL when kdof_is_vertex
+  const int2* slot_row = nullptr;        // [nb x slot_rows] CSR start, length | diagonal << 8
+  const int32_t* slot_rdof = nullptr;    // [nb x slot_rows] dof of each owned row
   bool kdof_is_vertex = false;
   const int32_t* node_ptr = nullptr;
   const int32_t* node_vertex = nullptr;

@@ -520,52 +520,77 @@ __device__ __forceinline__ void p1_block_body(const P1BlockTopo& t, const P1Pass
 // are complete and adds every finished entry to each owned row at the recorded ranks. The
 // diagonal is minus the row sum at write-out (the element rows sum to zero).
 // ---------------------------------------------------------------------------------------
-// The matrix walk of one chain (see above). K(p, q), K(p, r_k), K(q, r_k) are summed in
-// registers while r_k slides; every finished entry is added once per owned row of the pair.
-__device__ __forceinline__ void chain_walk_matrix(const uint32_t (&d)[kChainWords],
-                                                  const double* snode, double* svals, uint32_t nr) {
+
+// 1/d (d > 0, finite) from the hardware estimate and one Newton step: the estimate's error
+// e becomes e^2, below 1e-14 relative, far inside the 1e-12 parity bar.
+__device__ __forceinline__ double recip1(double d) {
+  const double r = __builtin_amdgcn_rcp(d);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
+// Byte address in LDS of rank `nib` of the owned row whose byte base is `row_base`.
+__device__ __forceinline__ double* row_slot(double* svals, uint32_t row_base, uint32_t nib) {
+  return reinterpret_cast<double*>(reinterpret_cast<char*>(svals) + row_base + (nib << 3));
+}
+
+// The walk of one chain; whether a node is an owned row is tested once per node (p and q once
+// per chain, r_k once per cell).
+template <bool WANT_B>
+__device__ __forceinline__ void chain_walk(const uint32_t (&d)[kChainWords], const double* snode,
+                                           double* svals, double* sbv, uint32_t nr) {
+  constexpr uint32_t kRowBytes = kRowStride * 8;
   const uint32_t lp = d[0] & 0xffffu, lq = d[0] >> 16;
   const int m = static_cast<int>(d[4] >> 16);
   auto rid = [&](int k) { return (d[1 + k / 2] >> (16 * (k & 1))) & 0xffffu; };
   auto rank_byte = [&](int j) { return (d[5 + j / 4] >> (8 * (j % 4))) & 0xffu; };
+  const bool op = lp < nr, oq = lq < nr;
   // the finished entry (u, v) = (v, u) with value w, into every owned row of the pair
-  auto emit = [&](uint32_t u, uint32_t v, uint32_t byte, double w) {
-    if (u < nr) atomic_add_lds(svals + u * kRowStride + (byte & 15u), w);
-    if (v < nr) atomic_add_lds(svals + v * kRowStride + (byte >> 4), w);
+  auto emit = [&](bool ou, uint32_t u, bool ov, uint32_t v, uint32_t byte, double w) {
+    if (ou) atomic_add_lds(row_slot(svals, u * kRowBytes, byte & 15u), w);
+    if (ov) atomic_add_lds(row_slot(svals, v * kRowBytes, byte >> 4), w);
   };
-  // node record: x, y | z, kappa | f
+  auto emit_b = [&](bool ou, uint32_t u, double w) {
+    if constexpr (WANT_B) {
+      if (ou) atomic_add_lds(sbv + u, w);
+    }
+  };
+  // node record: x, y | z, kappa | z, f
   auto xy = [&](uint32_t l) { return *reinterpret_cast<const double2*>(snode + l * 6); };
   auto zk = [&](uint32_t l) { return *reinterpret_cast<const double2*>(snode + l * 6 + 2); };
+  auto fv = [&](uint32_t l) { return snode[l * 6 + 5]; };
 
-  // fixed over the chain: p, e1 = q - p, kappa_p + kappa_q
   const double2 pxy = xy(lp), pzk = zk(lp), qxy = xy(lq), qzk = zk(lq);
   const double px = pxy.x, py = pxy.y, pz = pzk.x;
   const double e1x = qxy.x - px, e1y = qxy.y - py, e1z = qzk.x - pz;
   const double kpq = pzk.y + qzk.y;
-  // sliding: e2 = r_k - p and kappa of r_k
+  const double fpq = WANT_B ? fv(lp) + fv(lq) : 0.0;
   uint32_t cur = rid(0);
-  double e2x, e2y, e2z, kr;
+  bool oc = cur < nr;
+  double e2x, e2y, e2z, kr, fr;
   {
     const double2 rxy = xy(cur), rzk = zk(cur);
     e2x = rxy.x - px;
     e2y = rxy.y - py;
     e2z = rzk.x - pz;
     kr = rzk.y;
+    fr = WANT_B ? fv(cur) : 0.0;
   }
-  double apq = 0.0, cp = 0.0, cq = 0.0;
+  double apq = 0.0, cp = 0.0, cq = 0.0, swf = 0.0, sw = 0.0, cb = 0.0;
 #pragma unroll
   for (int k = 0; k < kChainLen; ++k) {
     if (k < m) {
       const uint32_t ln = rid(k + 1);
+      const bool on = ln < nr;
       const double2 nxy = xy(ln), nzk = zk(ln);
+      const double fn = WANT_B ? fv(ln) : 0.0;
       const double e3x = nxy.x - px, e3y = nxy.y - py, e3z = nzk.x - pz;
       // cofactor gradients of q, r_k, r_{k+1}: g1 = e2 x e3, g2 = e3 x e1, g3 = e1 x e2
       const double g1x = e2y * e3z - e2z * e3y, g1y = e2z * e3x - e2x * e3z, g1z = e2x * e3y - e2y * e3x;
       const double g2x = e3y * e1z - e3z * e1y, g2y = e3z * e1x - e3x * e1z, g2z = e3x * e1y - e3y * e1x;
       const double g3x = e1y * e2z - e1z * e2y, g3y = e1z * e2x - e1x * e2z, g3z = e1x * e2y - e1y * e2x;
-      const double det = e1x * g1x + e1y * g1y + e1z * g1z;
+      const double adet = fabs(e1x * g1x + e1y * g1y + e1z * g1z);
       // kappa-Poisson: K_ab = mean(kappa) / (6 |det|) g_a . g_b, g_p = -(g1 + g2 + g3)
-      const double w = (kpq + kr + nzk.y) * (1.0 / 24.0) * recip(fabs(det));
+      const double w = (kpq + kr + nzk.y) * (1.0 / 24.0) * recip1(adet);
       const double g11 = fma(g1x, g1x, fma(g1y, g1y, g1z * g1z));
       const double g22 = fma(g2x, g2x, fma(g2y, g2y, g2z * g2z));
       const double g33 = fma(g3x, g3x, fma(g3y, g3y, g3z * g3z));
@@ -576,220 +601,222 @@ __device__ __forceinline__ void chain_walk_matrix(const uint32_t (&d)[kChainWord
       cp -= w * (g12 + g22 + g23);
       cq += w * g12;
       // r_k is in no later cell of the chain: its entries with p and q are complete
-      emit(lp, cur, rank_byte(3 * k), cp);
-      emit(lq, cur, rank_byte(3 * k + 1), cq);
-      emit(cur, ln, rank_byte(3 * k + 2), w * g23);
+      emit(op, lp, oc, cur, rank_byte(3 * k), cp);
+      emit(oq, lq, oc, cur, rank_byte(3 * k + 1), cq);
+      emit(oc, cur, on, ln, rank_byte(3 * k + 2), w * g23);
       cp = -w * (g13 + g23 + g33);
       cq = w * g13;
-      e2x = e3x;
-      e2y = e3y;
-      e2z = e3z;
-      kr = nzk.y;
-      cur = ln;
-    }
-  }
-  emit(lp, cur, rank_byte(18), cp);
-  emit(lq, cur, rank_byte(19), cq);
-  emit(lp, lq, rank_byte(20), apq);
-}
-
-// Matrix and load vector in one walk: the same node reads serve both (x, y | z, kappa | z, f
-// records: three 128-bit reads per node instead of four over two walks), and |det| is
-// computed once per cell.
-__device__ __forceinline__ void chain_walk_fused(const uint32_t (&d)[kChainWords],
-                                                 const double* snode, double* svals, double* sbv,
-                                                 uint32_t nr) {
-  const uint32_t lp = d[0] & 0xffffu, lq = d[0] >> 16;
-  const int m = static_cast<int>(d[4] >> 16);
-  auto rid = [&](int k) { return (d[1 + k / 2] >> (16 * (k & 1))) & 0xffffu; };
-  auto rank_byte = [&](int j) { return (d[5 + j / 4] >> (8 * (j % 4))) & 0xffu; };
-  auto emit = [&](uint32_t u, uint32_t v, uint32_t byte, double w) {
-    if (u < nr) atomic_add_lds(svals + u * kRowStride + (byte & 15u), w);
-    if (v < nr) atomic_add_lds(svals + v * kRowStride + (byte >> 4), w);
-  };
-  auto emit_b = [&](uint32_t u, double w) {
-    if (u < nr) atomic_add_lds(sbv + u, w);
-  };
-  auto xy = [&](uint32_t l) { return *reinterpret_cast<const double2*>(snode + l * 6); };
-  auto zk = [&](uint32_t l) { return *reinterpret_cast<const double2*>(snode + l * 6 + 2); };
-  auto fv = [&](uint32_t l) { return snode[l * 6 + 5]; };
-
-  const double2 pxy = xy(lp), pzk = zk(lp), qxy = xy(lq), qzk = zk(lq);
-  const double px = pxy.x, py = pxy.y, pz = pzk.x;
-  const double e1x = qxy.x - px, e1y = qxy.y - py, e1z = qzk.x - pz;
-  const double kpq = pzk.y + qzk.y;
-  const double fpq = fv(lp) + fv(lq);
-  uint32_t cur = rid(0);
-  double e2x, e2y, e2z, kr, fr;
-  {
-    const double2 rxy = xy(cur), rzk = zk(cur);
-    e2x = rxy.x - px;
-    e2y = rxy.y - py;
-    e2z = rzk.x - pz;
-    kr = rzk.y;
-    fr = fv(cur);
-  }
-  double apq = 0.0, cp = 0.0, cq = 0.0, swf = 0.0, sw = 0.0, cb = 0.0;
-#pragma unroll
-  for (int k = 0; k < kChainLen; ++k) {
-    if (k < m) {
-      const uint32_t ln = rid(k + 1);
-      const double2 nxy = xy(ln), nzk = zk(ln);
-      const double fn = fv(ln);
-      const double e3x = nxy.x - px, e3y = nxy.y - py, e3z = nzk.x - pz;
-      const double g1x = e2y * e3z - e2z * e3y, g1y = e2z * e3x - e2x * e3z, g1z = e2x * e3y - e2y * e3x;
-      const double g2x = e3y * e1z - e3z * e1y, g2y = e3z * e1x - e3x * e1z, g2z = e3x * e1y - e3y * e1x;
-      const double g3x = e1y * e2z - e1z * e2y, g3y = e1z * e2x - e1x * e2z, g3z = e1x * e2y - e1y * e2x;
-      const double adet = fabs(e1x * g1x + e1y * g1y + e1z * g1z);
-      const double w = (kpq + kr + nzk.y) * (1.0 / 24.0) * recip(adet);
-      const double g11 = fma(g1x, g1x, fma(g1y, g1y, g1z * g1z));
-      const double g22 = fma(g2x, g2x, fma(g2y, g2y, g2z * g2z));
-      const double g33 = fma(g3x, g3x, fma(g3y, g3y, g3z * g3z));
-      const double g12 = fma(g1x, g2x, fma(g1y, g2y, g1z * g2z));
-      const double g13 = fma(g1x, g3x, fma(g1y, g3y, g1z * g3z));
-      const double g23 = fma(g2x, g3x, fma(g2y, g3y, g2z * g3z));
-      apq -= w * (g11 + g12 + g13);
-      cp -= w * (g12 + g22 + g23);
-      cq += w * g12;
-      emit(lp, cur, rank_byte(3 * k), cp);
-      emit(lq, cur, rank_byte(3 * k + 1), cq);
-      emit(cur, ln, rank_byte(3 * k + 2), w * g23);
-      cp = -w * (g13 + g23 + g33);
-      cq = w * g13;
-      // load vector: b_a += |det| / 120 (f_a + sum of the cell's f)
-      const double wb = adet * (1.0 / 120.0);
-      const double F = fpq + fr + fn;
-      swf = fma(wb, F, swf);
-      sw += wb;
-      emit_b(cur, fma(wb, F + fr, cb));
-      cb = wb * (F + fn);
+      if constexpr (WANT_B) {
+        // load vector: b_a += |det| / 120 (f_a + sum of the cell's f)
+        const double wb = adet * (1.0 / 120.0);
+        const double F = fpq + fr + fn;
+        swf = fma(wb, F, swf);
+        sw += wb;
+        emit_b(oc, cur, fma(wb, F + fr, cb));
+        cb = wb * (F + fn);
+      }
       e2x = e3x;
       e2y = e3y;
       e2z = e3z;
       kr = nzk.y;
       fr = fn;
       cur = ln;
+      oc = on;
     }
   }
-  emit(lp, cur, rank_byte(18), cp);
-  emit(lq, cur, rank_byte(19), cq);
-  emit(lp, lq, rank_byte(20), apq);
-  emit_b(cur, cb);
-  emit_b(lp, fma(fv(lp), sw, swf));
-  emit_b(lq, fma(fv(lq), sw, swf));
+  emit(op, lp, oc, cur, rank_byte(18), cp);
+  emit(oq, lq, oc, cur, rank_byte(19), cq);
+  emit(op, lp, oq, lq, rank_byte(20), apq);
+  if constexpr (WANT_B) {
+    emit_b(oc, cur, cb);
+    emit_b(op, lp, fma(fv(lp), sw, swf));
+    emit_b(oq, lq, fma(fv(lq), sw, swf));
+  }
 }
 
-template <bool WANT_B, int T, int RR>
-__device__ __forceinline__ void p1_chain_body(const P1BlockTopo& t, const P1Pass& p,
-                                              const LdsLayout& L, unsigned char* lds, int blk) {
-  double* snode = reinterpret_cast<double*>(lds + L.node);
-  double* svals = reinterpret_cast<double*>(lds + L.vals);
-  double* sbv = reinterpret_cast<double*>(lds + L.bv);
-  int32_t* srdof = reinterpret_cast<int32_t*>(lds + L.rdof);
-  int32_t* srbase = reinterpret_cast<int32_t*>(lds + L.rbase);
-  const int tid = threadIdx.x;
-  // chain records first, so that their latency overlaps the node gather
-  const int c0 = t.chain_ptr[blk], nch = t.chain_ptr[blk + 1] - c0;
-  const uint4* __restrict__ crec = reinterpret_cast<const uint4*>(t.chain_rec) + 3 * static_cast<int64_t>(c0);
+// DPP move with bound control (every lane of the row is active here, so no old value is
+// needed): VALU only, no LDS-pipe permute.
+template <int CTRL>
+__device__ __forceinline__ double dpp_bc(double v) {
+  const int2 a = *reinterpret_cast<int2*>(&v);
+  int2 r;
+  r.x = __builtin_amdgcn_mov_dpp(a.x, CTRL, 0xF, 0xF, true);
+  r.y = __builtin_amdgcn_mov_dpp(a.y, CTRL, 0xF, 0xF, true);
+  return *reinterpret_cast<double*>(&r);
+}
+
+// The same over 16-lane rows: the quad sums, then rotations of the row by 4 and 8 lanes.
+__device__ __forceinline__ double row16_sum_bc(double v) {
+  v += dpp_bc<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_bc<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_bc<0x124>(v);  // row_ror 4
+  v += dpp_bc<0x128>(v);  // row_ror 8
+  return v;
+}
+
+// A block header read through the constant address space: the plan is read-only for the whole
+// launch, so the (uniform) load is a scalar load into SGPRs.
+__device__ __forceinline__ int4 block_header(const int4* hdr, int blk) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(4))) int4 ConstInt4;
+  return ((ConstInt4*)hdr)[blk];
+#else
+  return hdr[blk];
+#endif
+}
+
+// Per-block data of the edge-chain kernels, loaded in two rounds: node ids, owned-row words
+// and chain records (addressed from the block index and its header), then the nodes'
+// coordinates and coefficients; held in registers until stored to LDS.
+template <int T, int RR>
+struct ChainLoad {
+  static constexpr int NSL = (kMaxChainBlockNodes + T - 1) / T;  // node slots per thread
+  int32_t v[NSL], kd[NSL];
+  int2 rw;
+  int32_t dof;
   uint32_t d[RR][kChainWords];
+  double2 xy[NSL], zk[NSL];
+  double f[NSL];
+};
+
+// round 1
+template <bool WANT_B, int T, int RR>
+__device__ __forceinline__ void chain_load_ids(const P1BlockTopo& t, int blk, const int4& hd,
+                                               ChainLoad<T, RR>& c) {
+  constexpr int NSL = ChainLoad<T, RR>::NSL;
+  const int tid = threadIdx.x;
+  const int SN = t.slot_nodes, SR = t.slot_rows;
+  const int64_t sn0 = static_cast<int64_t>(blk) * SN, sr0 = static_cast<int64_t>(blk) * SR;
+#pragma unroll
+  for (int h = 0; h < NSL; ++h) {
+    const int i = tid + h * T;
+    c.v[h] = 0;
+    c.kd[h] = 0;
+    if (i < SN) {
+      c.v[h] = t.slot_vertex[sn0 + i];
+      c.kd[h] = t.slot_kdof ? t.slot_kdof[sn0 + i] : c.v[h];
+    }
+  }
+  c.rw = make_int2(0, 0);
+  c.dof = 0;
+  if (tid < SR) {
+    c.rw = t.slot_row[sr0 + tid];
+    if constexpr (WANT_B) c.dof = t.slot_rdof[sr0 + tid];
+  }
+  const uint4* __restrict__ crec = reinterpret_cast<const uint4*>(t.chain_rec) + 3 * static_cast<int64_t>(hd.x);
 #pragma unroll
   for (int j = 0; j < RR; ++j) {
     const int i = tid + j * T;
 #pragma unroll
     for (int h = 0; h < 3; ++h) {
-      const uint4 w = i < nch ? crec[3 * i + h] : make_uint4(0, 0, 0, 0);
-      d[j][4 * h] = w.x;
-      d[j][4 * h + 1] = w.y;
-      d[j][4 * h + 2] = w.z;
-      d[j][4 * h + 3] = w.w;
+      const uint4 w = i < hd.y ? crec[3 * i + h] : make_uint4(0, 0, 0, 0);
+      c.d[j][4 * h] = w.x;
+      c.d[j][4 * h + 1] = w.y;
+      c.d[j][4 * h + 2] = w.z;
+      c.d[j][4 * h + 3] = w.w;
     }
   }
-  // The first node and the first row of each thread are fetched in two rounds of independent
-  // loads (ids, then coordinates / coefficients and CSR row starts) so that the node and row
-  // chains of dependent loads overlap; later nodes and rows, if any, follow in loops.
-  const int n0 = t.node_ptr[blk], nn = t.node_ptr[blk + 1] - n0;
-  const int r0 = t.row_ptr_b[blk], nr = t.row_ptr_b[blk + 1] - r0;
-  const bool has_node = tid < nn, has_row = tid < nr;
-  int32_t v = 0, kd = 0, dof = 0, dg = 0;
-  if (has_node) {
-    v = t.node_vertex[n0 + tid];
-    kd = t.kdof_is_vertex ? v : t.node_kdof[n0 + tid];
+}
+
+// round 2 (padding slots repeat node 0: harmless reads)
+template <bool WANT_B, int T, int RR>
+__device__ __forceinline__ void chain_gather(const P1BlockTopo& t, const P1Pass& p, ChainLoad<T, RR>& c) {
+  constexpr int NSL = ChainLoad<T, RR>::NSL;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int h = 0; h < NSL; ++h) {
+    c.xy[h] = make_double2(0.0, 0.0);
+    c.zk[h] = make_double2(0.0, 0.0);
+    c.f[h] = 0.0;
+    if (tid + h * T < t.slot_nodes) {
+      const int64_t vx = 3 * static_cast<int64_t>(c.v[h]);
+      c.xy[h] = make_double2(p.x[vx], p.x[vx + 1]);
+      c.zk[h] = make_double2(p.x[vx + 2], p.kappa[c.kd[h]]);
+      c.f[h] = WANT_B ? p.f[c.kd[h]] : 0.0;
+    }
   }
-  if (has_row) {
-    dof = t.row_dof[r0 + tid];
-    dg = t.row_diag[r0 + tid];
+}
+
+// LDS images of one block: node records x, y | z, kappa | z, f, owned-row words (CSR start,
+// length | diagonal << 8), row dofs, zeroed value rows and load-vector entries.
+template <bool WANT_B, int T, int RR>
+__device__ __forceinline__ void chain_store(const P1BlockTopo& t, const LdsLayout& L, unsigned char* lds,
+                                            int nr, const ChainLoad<T, RR>& c) {
+  constexpr int NSL = ChainLoad<T, RR>::NSL;
+  double* snode = reinterpret_cast<double*>(lds + L.node);
+  double* svals = reinterpret_cast<double*>(lds + L.vals);
+  double* sbv = reinterpret_cast<double*>(lds + L.bv);
+  int32_t* srdof = reinterpret_cast<int32_t*>(lds + L.rdof);
+  int2* srow = reinterpret_cast<int2*>(lds + L.rbase);
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int h = 0; h < NSL; ++h) {
+    const int i = tid + h * T;
+    if (i < t.slot_nodes) {
+      double2* rec = reinterpret_cast<double2*>(snode + i * 6);
+      rec[0] = c.xy[h];
+      rec[1] = c.zk[h];
+      rec[2] = make_double2(c.zk[h].x, c.f[h]);
+    }
   }
-  double nx = 0.0, ny = 0.0, nz = 0.0, nk = 0.0, nf = 0.0;
-  if (has_node) {
-    nx = p.x[3 * static_cast<int64_t>(v)];
-    ny = p.x[3 * static_cast<int64_t>(v) + 1];
-    nz = p.x[3 * static_cast<int64_t>(v) + 2];
-    nk = p.kappa[kd];
-    if constexpr (WANT_B) nf = p.f[kd];
-  }
-  int32_t rb0 = 0, rb1 = 0;
-  if (has_row) {
-    rb0 = t.csr_row_ptr[dof];
-    rb1 = t.csr_row_ptr[dof + 1];
-  }
-  // one 8-byte word per owned row: CSR start, row length | diagonal position << 8
-  int2* srow = reinterpret_cast<int2*>(srbase);
-  if (has_node) {
-    double2* rec = reinterpret_cast<double2*>(snode + tid * 6);
-    rec[0] = make_double2(nx, ny);
-    rec[1] = make_double2(nz, nk);
-    rec[2] = make_double2(nz, nf);
-  }
-  if (has_row) {
-    srdof[tid] = dof;
-    srow[tid] = make_int2(rb0, (rb1 - rb0) | (dg << 8));
-    if constexpr (WANT_B) sbv[tid] = 0.0;
-  }
-  for (int i = tid + T; i < nn; i += T) {
-    const int32_t vi = t.node_vertex[n0 + i];
-    const int32_t ki = t.kdof_is_vertex ? vi : t.node_kdof[n0 + i];
-    const double zi = p.x[3 * static_cast<int64_t>(vi) + 2];
-    double2* rec = reinterpret_cast<double2*>(snode + i * 6);
-    rec[0] = make_double2(p.x[3 * static_cast<int64_t>(vi)], p.x[3 * static_cast<int64_t>(vi) + 1]);
-    rec[1] = make_double2(zi, p.kappa[ki]);
-    rec[2] = make_double2(zi, WANT_B ? p.f[ki] : 0.0);
-  }
-  for (int i = tid + T; i < nr; i += T) {
-    const int32_t di = t.row_dof[r0 + i];
-    srdof[i] = di;
-    const int32_t b0 = t.csr_row_ptr[di];
-    srow[i] = make_int2(b0, (t.csr_row_ptr[di + 1] - b0) | (static_cast<int>(t.row_diag[r0 + i]) << 8));
-    if constexpr (WANT_B) sbv[i] = 0.0;
+  if (tid < t.slot_rows) {
+    srow[tid] = c.rw;
+    if constexpr (WANT_B) {
+      srdof[tid] = c.dof;
+      sbv[tid] = 0.0;
+    }
   }
   for (int i = tid; i < (nr * kRowStride + 1) / 2; i += T)
     reinterpret_cast<double2*>(svals)[i] = make_double2(0.0, 0.0);
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < RR; ++j)
-    if (tid + j * T < nch) {
-      if constexpr (WANT_B) chain_walk_fused(d[j], snode, svals, sbv, static_cast<uint32_t>(nr));
-      else chain_walk_matrix(d[j], snode, svals, static_cast<uint32_t>(nr));
-    }
-  __syncthreads();
+}
 
-  // write-out: 16 lanes per row, diagonal = minus the row sum
+// Write-out of one block from LDS: 16 lanes per row (a row holds at most 16 entries; the walk
+// never writes a slot at or past the row length, nor the diagonal, so both read as zero); the
+// diagonal is minus the row sum.
+template <bool WANT_B, int T>
+__device__ __forceinline__ void chain_write(const P1Pass& p, const LdsLayout& L, unsigned char* lds, int nr) {
+  const double* svals = reinterpret_cast<const double*>(lds + L.vals);
+  const double* sbv = reinterpret_cast<const double*>(lds + L.bv);
+  const int32_t* srdof = reinterpret_cast<const int32_t*>(lds + L.rdof);
+  const int2* srow = reinterpret_cast<const int2*>(lds + L.rbase);
+  const int tid = threadIdx.x;
   const int grp = tid >> 4, ln = tid & 15;
   for (int i = grp; i < nr; i += T / 16) {
-    const int2 rw = srow[i];
-    const int base = rw.x, len = rw.y & 0xff;
-    double v = ln < len ? svals[i * kRowStride + ln] : 0.0;
-    const double rs = row16_sum(v);
-    if (ln == (rw.y >> 8)) v = -rs;
+    const int2 w = srow[i];
+    const int len = w.y & 0xff, dg = w.y >> 8;
+    double v = svals[i * kRowStride + ln];
+    const double rs = row16_sum_bc(v);
+    if (ln == dg) v = -rs;
     if (ln < len) {
-      if (p.add) v += p.values[base + ln];
-      p.values[base + ln] = v;
+      double* out = p.values + static_cast<int64_t>(w.x) + ln;
+      *out = p.add ? *out + v : v;
     }
   }
   if constexpr (WANT_B)
     for (int i = tid; i < nr; i += T) {
-      const int32_t dof = srdof[i];
-      p.b[dof] = p.add ? p.b[dof] + sbv[i] : sbv[i];
+      const int32_t di = srdof[i];
+      p.b[di] = p.add ? p.b[di] + sbv[i] : sbv[i];
     }
+}
+
+// One row block by edge chains (the batched kernel's unit).
+template <bool WANT_B, int T, int RR>
+__device__ __forceinline__ void p1_chain_body(const P1BlockTopo& t, const P1Pass& p,
+                                              const LdsLayout& L, unsigned char* lds, int blk) {
+  const int4 hd = block_header(t.blk_hdr, blk);  // first chain, chains, nodes, owned rows
+  ChainLoad<T, RR> c;
+  chain_load_ids<WANT_B, T, RR>(t, blk, hd, c);
+  chain_gather<WANT_B, T, RR>(t, p, c);
+  chain_store<WANT_B, T, RR>(t, L, lds, hd.w, c);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < RR; ++j)
+    if (static_cast<int>(threadIdx.x) + j * T < hd.y)
+      chain_walk<WANT_B>(c.d[j], reinterpret_cast<const double*>(lds + L.node),
+                         reinterpret_cast<double*>(lds + L.vals), reinterpret_cast<double*>(lds + L.bv),
+                         static_cast<uint32_t>(hd.w));
+  __syncthreads();
+  chain_write<WANT_B, T>(p, L, lds, hd.w);
 }
 
 }  // namespace p1dev

@@ -511,8 +511,9 @@ hipError_t launch_blocked_tu(const P1BlockTopo& t, const P1Pass& p, hipStream_t 
 // The edge-chain kernel applies to tetrahedral kappa-Poisson (+ load) without Dirichlet rows
 // when the plan has chains and every row fits the strided LDS layout.
 bool chains_fit(const P1BlockTopo& t, const P1Pass& p) {
-  return t.chain_ptr && t.chain_rec && p.gdim == 3 && p.a_form == DFA_FORM_P1_POISSON && !p.bc &&
-         t.max_row_len <= kMaxRank && t.wide_nodes && t.max_chains <= 1024;
+  return t.chain_ptr && t.chain_rec && t.blk_hdr && p.gdim == 3 && p.a_form == DFA_FORM_P1_POISSON && !p.bc &&
+         t.max_row_len <= kMaxRank && t.wide_nodes && t.max_chains <= 1024 &&
+         t.max_nodes <= kMaxChainBlockNodes;
 }
 
 template <bool WANT_B, int T, int RR>
