@@ -39,7 +39,7 @@ EXPORTS = ["dfa_last_error", "dfa_version", "dfa_assembler_create", "dfa_assembl
            "dfa_exterior_facets", "dfa_assemble_facets", "dfa_csr_spmv", "dfa_csr_diagonal",
            "dfa_node_reordering", "dfa_graph_reordering", "dfa_mesh_coloring", "dfa_color_info",
            "dfa_device_info", "dfa_assemble_system_x0", "dfa_dirichlet_x0", "dfa_pass_kernel",
-           "dfa_csr_add_local", "dfa_vector_add_local"]
+           "dfa_csr_add_local", "dfa_vector_add_local", "dfa_assemble_system_x0_noalloc"]
 
 _lib = None
 
@@ -69,6 +69,7 @@ def load():
                                       _i32, _i32, _vp]
     L.dfa_assemble_system_x0.argtypes = [_vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _i32, _i32, _vp]
+    L.dfa_assemble_system_x0_noalloc.argtypes = L.dfa_assemble_system_x0.argtypes
     L.dfa_dirichlet_x0.argtypes = [_i64, _vp, _vp, _vp, _vp, _vp]
     L.dfa_csr_add_local.argtypes = [_vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp]
     L.dfa_vector_add_local.argtypes = [_vp, _i64, _i32, _vp, _vp, _vp]

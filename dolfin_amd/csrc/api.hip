@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -664,6 +665,75 @@ int dfa_assemble_system_x0(dfa_assembler h, int32_t a_form, int32_t L_form, cons
       hip_check(dfa::launch_dirichlet_x0(h->ndofs, d_bc_mask, d_bc_value, d_x0, h->d_bc_x0, s),
                 "dirichlet_x0_kernel");
       bcv = h->d_bc_x0;
+    }
+    return run_pass(h, a_form, L_form, d_x, d_a_c0, d_a_c1, d_L_c0, h_f3, d_bc_mask, bcv,
+                    d_values, d_b, scatter, flags, s);
+  });
+}
+
+}  // extern "C"
+
+// Device array of the allocation-free entry point: the shifted Dirichlet values of
+// dfa_assemble_system_x0_noalloc. Each GPU holds its own copy until the process ends. In host
+// code the name is only a stand-in for the symbol; its device address on each device comes
+// from hipGetSymbolAddress (kept per device below) and is the only form passed to kernels.
+namespace {
+static __device__ __align__(256) double scratch[DFA_NOALLOC_MAX_DOFS];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (resolved once per device).
+double* scratch_on_current_device() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int dfa_assemble_system_x0_noalloc(dfa_assembler h, int32_t a_form, int32_t L_form,
+                                   const double* d_x, const double* d_a_c0, const double* d_a_c1,
+                                   const double* d_L_c0, const double* h_f3,
+                                   const uint8_t* d_bc_mask, const double* d_bc_value,
+                                   const double* d_x0, double* d_values, double* d_b,
+                                   int32_t scatter, int32_t flags, void* stream) {
+  return guarded([&]() -> int {
+    if (!h) return fail("NULL handle", DFA_ERR_ARG);
+    if (!d_x0) return fail("x0 is NULL (use dfa_assemble_system)", DFA_ERR_ARG);
+    if (!d_values && !d_b) return fail("nothing to assemble (A and b are NULL)", DFA_ERR_ARG);
+    if (a_form < 0 || L_form < 0) return fail("assemble_system needs a bilinear and a linear form", DFA_ERR_ARG);
+    if (L_form == DFA_FORM_VP2_BODY_FORCE && !h_f3) return fail("f3 is NULL", DFA_ERR_ARG);
+    if ((d_bc_mask == nullptr) != (d_bc_value == nullptr))
+      return fail("Dirichlet mask and values go together", DFA_ERR_ARG);
+    if (h->ndofs > DFA_NOALLOC_MAX_DOFS)
+      return fail("dfa_assemble_system_x0_noalloc: " + std::to_string(h->ndofs) +
+                      " dofs exceed its fixed device array of " +
+                      std::to_string(DFA_NOALLOC_MAX_DOFS) + " (use dfa_assemble_system_x0)",
+                  DFA_ERR_LIMIT);
+    // the coloured scatter builds its colour plan (device memory) on first use
+    if (h->element == DFA_ELEMENT_P1 && scatter == DFA_SCATTER_COLORED && !h->has_colors)
+      return fail("dfa_assemble_system_x0_noalloc: the coloured scatter allocates its colour plan "
+                  "on first use; run one assembly with it through an allocating entry point first",
+                  DFA_ERR_STATE);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const double* bcv = d_bc_value;
+    if (d_bc_mask) {
+      // every entry is written (x0 - g on Dirichlet dofs, 0 elsewhere): nothing of an earlier
+      // call's contents is read
+      double* ptr = scratch_on_current_device();
+      hip_check(dfa::launch_dirichlet_x0(h->ndofs, d_bc_mask, d_bc_value, d_x0, ptr, s),
+                "dirichlet_x0_kernel");
+      bcv = ptr;
     }
     return run_pass(h, a_form, L_form, d_x, d_a_c0, d_a_c1, d_L_c0, h_f3, d_bc_mask, bcv,
                     d_values, d_b, scatter, flags, s);

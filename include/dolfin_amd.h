@@ -146,6 +146,26 @@ int dfa_assemble_system_x0(dfa_assembler h, int32_t a_form, int32_t L_form, cons
                            const double* d_x0, double* d_values, double* d_b, int32_t scatter,
                            int32_t flags, void* stream);
 
+/*
+ * Allocation-free variant of dfa_assemble_system_x0 (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under HIP graph capture (a Newton loop
+ * captured once and replayed), tight solver loops, harnesses that forbid hipMalloc. Same
+ * arguments and results; the shifted Dirichlet values live in a fixed device array of the
+ * library (DFA_NOALLOC_MAX_DOFS doubles = 32 MiB, held by each GPU the library runs on until
+ * the process ends) instead of a buffer allocated on the handle's first x0 call. Handles with
+ * more dofs are refused (DFA_ERR_LIMIT); so is the coloured scatter before its colour plan
+ * exists (DFA_ERR_STATE: it is built, in device memory, on first use). The call performs no
+ * device allocation. All calls on one GPU share the array: calls that overlap on one GPU
+ * (different streams or threads) race and must be serialised by the caller.
+ */
+#define DFA_NOALLOC_MAX_DOFS 4194304
+int dfa_assemble_system_x0_noalloc(dfa_assembler h, int32_t a_form, int32_t L_form,
+                                   const double* d_x, const double* d_a_c0, const double* d_a_c1,
+                                   const double* d_L_c0, const double* h_f3,
+                                   const uint8_t* d_bc_mask, const double* d_bc_value,
+                                   const double* d_x0, double* d_values, double* d_b,
+                                   int32_t scatter, int32_t flags, void* stream);
+
 /* The boundary-value modification of SystemAssembler.cpp:252-275 on its own, for callers
    that pass the modified values to several passes (cell and exterior-facet):
    d_out[i] = d_bc_mask[i] ? d_x0[i] - d_bc_value[i] : 0, i < n (device arrays). */
