@@ -25,6 +25,7 @@ HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and matrix)
 METRIC = "cells assembled/s + % HBM roofline, 1 GPU (P1 128^3, P2 elast.); adjoint/forward time"
 HERE = os.path.dirname(os.path.abspath(__file__))
+SECTIONS = "adjoint,spmv,strategies,rcm,vp2,batch"
 
 
 def parse():
@@ -37,6 +38,8 @@ def parse():
     ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "blocked", "colored"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--sections", default=SECTIONS,
+                    help="comma-separated secondary workloads to run (default: all of " + SECTIONS + ")")
     ap.add_argument("--batch", type=int, default=256,
                     help="members of the batched launch (0 = skip): n ~ U{8..48}, half P1, half P2")
     return ap.parse_args()
@@ -64,6 +67,8 @@ def timed_steps(fn, steps, warmup):
 
 
 def load_traffic(name):
+    """HBM bytes per launch of a kernel and workload, as measured by the rocprofv3 counter
+    runs recorded in profiles/traffic.json (not measured in this run), or None."""
     path = os.path.join(HERE, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -71,6 +76,14 @@ def load_traffic(name):
         return json.load(open(path)).get(name)
     except Exception:
         return None
+
+
+def traffic_source(name):
+    try:
+        src = json.load(open(os.path.join(HERE, "profiles", "traffic.json")))["_sources"][name]
+    except Exception:
+        return None
+    return f"profiles/traffic.json[{name}]: rocprofv3 --pmc counters, {src}"
 
 
 def bench_batch(nmembers, log, seed=5):
@@ -191,6 +204,7 @@ def main():
                    "plan": info, "device": device_info(dev)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_source(f"p1_{used_scatter}_{args.n}"),
                      "algorithmic_bytes": alg_bytes, "kernel_ms": ev_ms},
         "first_assembly_s": t_first,
     }
@@ -198,6 +212,8 @@ def main():
     # ---- CPU baseline: the oracle's SystemAssembler cell loop on the host ----------------------
     if not args.no_cpu_baseline:
         import oracle as O
+        log("cpu baseline: building the oracle with -O3 -march=native for this host")
+        cflags = O.use_native_build()
         rp, col = A.structure()
         rp, col = rp.cpu().numpy(), col.cpu().numpy()
         x, cells, cd = mesh.coordinates(), mesh.cells(), V.cell_dofs()
@@ -223,14 +239,26 @@ def main():
             "value": nc / sn, "unit": "cells/s", "cores": nthreads, "kind": "port",
             "sample": f"oracle SystemAssembler cell loop (Eigen coeffRef semantics), the full "
                       f"{args.n}^3 workload (A+b values, structure prebuilt), {nthreads} host "
-                      f"threads over contiguous cell ranges with atomic scatter, median of 5"}
+                      f"threads over contiguous cell ranges with atomic scatter, median of 5, "
+                      f"g++ {cflags}"}
         result["cpu_baseline_1thread"] = {
             "value": nc / s1, "unit": "cells/s", "cores": 1, "kind": "port",
-            "sample": f"the same loop serial (DOLFIN's per-process semantics), median of 5",
+            "sample": f"the same loop serial (DOLFIN's per-process semantics), median of 5, "
+                      f"g++ {cflags}",
             "first_assembly_s": cpu_struct + t1[0], "structure_s": cpu_struct}
-        result["cpu_baseline_host"] = {"nproc": os.cpu_count()}
+        result["cpu_baseline_host"] = {"nproc": os.cpu_count(), "cpu_model": O.cpu_model(),
+                                       "compiler_flags": cflags}
 
     # ---- secondary: adjoint on P1 and vector P2 elasticity -----------------------------------
+    sections = set(args.sections.split(","))
+    unknown = sections - set(SECTIONS.split(","))
+    if unknown:
+        print(f"unknown --sections {sorted(unknown)}; choose from {SECTIONS}", file=sys.stderr)
+        sys.exit(2)
+
+    def want(name):
+        return name in sections
+
     if not args.no_secondary:
         sec = {}
         kk = torch.from_numpy(kappa).cuda()
@@ -238,78 +266,82 @@ def main():
         uu = torch.from_numpy(u).cuda()
         xx = mesh.coordinates_device()
 
-        def adj():
-            adjoint._run(V, dfa._lib.FORM_P1_POISSON, xx, kk, None, ff, None, uu)
-        w_adj, ev_adj = timed_steps(adj, max(10, args.steps // 2), 3)
-        sec["p1_adjoint"] = {"ms": float(np.mean(ev_adj)),
-                             "ratio_to_forward": float(np.mean(ev_adj)) / ev_ms,
-                             "workload": f"dJ/dkappa, dJ/dx of u^T K u - F^T u, P1 {args.n}^3"}
-        log("adjoint timed")
-        xs = torch.rand(V.dim(), dtype=torch.float64, device="cuda")
-        ys = torch.empty_like(xs)
-        w_mv, ev_mv = timed_steps(lambda: A.mult(xs, ys), max(20, args.steps), 5)
-        mv_ms = float(np.mean(ev_mv))
-        mv_bytes = 12 * eng.nnz + 4 * (V.dim() + 1) + 16 * V.dim()
-        sec["p1_spmv"] = {"ms": mv_ms, "hbm_achieved_GBs": mv_bytes / (mv_ms * 1e-3) / 1e9,
-                          "hbm_frac": mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                          "algorithmic_bytes": mv_bytes,
-                          "traffic": load_traffic(f"p1_spmv_{args.n}"),
-                          "workload": f"y = A x on the assembled P1 {args.n}^3 matrix"}
-        log("spmv timed")
-        # the other two scatter strategies on the same A+b workload (the choice between
-        # owner-computes, graph-coloured read-add-write and global atomics)
-        strat = {}
-        for name, sc in (("atomic", dfa.SCATTER_ATOMIC), ("colored", dfa.SCATTER_COLORED)):
-            if sc == scatter:
-                continue
-            asm_s = dfa.SystemAssembler(a, L, scatter=sc)
+        if want("adjoint"):
+            def adj():
+                adjoint._run(V, dfa._lib.FORM_P1_POISSON, xx, kk, None, ff, None, uu)
+            w_adj, ev_adj = timed_steps(adj, max(10, args.steps // 2), 3)
+            sec["p1_adjoint"] = {"ms": float(np.mean(ev_adj)),
+                                 "ratio_to_forward": float(np.mean(ev_adj)) / ev_ms,
+                                 "workload": f"dJ/dkappa, dJ/dx of u^T K u - F^T u, P1 {args.n}^3"}
+            log("adjoint timed")
+        if want("spmv"):
+            xs = torch.rand(V.dim(), dtype=torch.float64, device="cuda")
+            ys = torch.empty_like(xs)
+            w_mv, ev_mv = timed_steps(lambda: A.mult(xs, ys), max(20, args.steps), 5)
+            mv_ms = float(np.mean(ev_mv))
+            mv_bytes = 12 * eng.nnz + 4 * (V.dim() + 1) + 16 * V.dim()
+            sec["p1_spmv"] = {"ms": mv_ms, "hbm_achieved_GBs": mv_bytes / (mv_ms * 1e-3) / 1e9,
+                              "hbm_frac": mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                              "algorithmic_bytes": mv_bytes,
+                              "traffic": load_traffic(f"p1_spmv_{args.n}"),
+                              "workload": f"y = A x on the assembled P1 {args.n}^3 matrix"}
+            log("spmv timed")
+        if want("strategies"):
+            # the other two scatter strategies on the same A+b workload (the choice between
+            # owner-computes, graph-coloured read-add-write and global atomics)
+            strat = {}
+            for name, sc in (("atomic", dfa.SCATTER_ATOMIC), ("colored", dfa.SCATTER_COLORED)):
+                if sc == scatter:
+                    continue
+                asm_s = dfa.SystemAssembler(a, L, scatter=sc)
+                t = time.perf_counter()
+                asm_s.assemble(A, b)
+                torch.cuda.synchronize()
+                t_s = time.perf_counter() - t
+                w_s, ev_s = timed_steps(lambda: asm_s.assemble(A, b), 10, 2)
+                strat[name] = {"ms": float(np.mean(ev_s)), "first_call_s": t_s,
+                               "hbm_frac": alg_bytes / (float(np.mean(ev_s)) * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                if sc == dfa.SCATTER_COLORED:
+                    strat[name]["colors"] = len(eng.color_ptr()) - 1
+                    strat[name]["traffic"] = load_traffic(f"p1_colored_{args.n}")
+                else:
+                    strat[name]["traffic"] = load_traffic(f"p1_atomic_{args.n}")
+                log(f"{name} scatter timed")
+            asm.assemble(A, b)
+            from dolfin_amd import autotune
+            pick, ttimes = autotune.tune_scatter([a], reps=3)
+            names = {dfa.SCATTER_BLOCKED: "blocked", dfa.SCATTER_COLORED: "colored",
+                     dfa.SCATTER_ATOMIC: "atomic"}
+            sec["p1_scatter_strategies"] = dict(
+                strat, workload=f"P1 {args.n}^3 A+b, same inputs as the headline, other scatters",
+                autotuned={"choice": names[pick],
+                           "matrix_ms": {names[k]: v for k, v in ttimes.items()}})
+        if want("rcm"):
+            # the same workload with DOLFIN's default numbering (reverse Cuthill-McKee nodes)
             t = time.perf_counter()
-            asm_s.assemble(A, b)
+            Vr = dfa.FunctionSpace(mesh, "P", 1, reorder=True)
+            t_rcm = time.perf_counter() - t
+            new_of = np.empty(V.dim(), np.int64)  # new dof of each vertex
+            new_of[mesh.cells().ravel()] = Vr.cell_dofs().ravel()
+            kr = np.empty_like(kappa)
+            fr = np.empty_like(f)
+            kr[new_of] = kappa
+            fr[new_of] = f
+            asm_r = dfa.SystemAssembler(dfa.poisson_form(Vr, workloads.to_function(Vr, kr)),
+                                        dfa.load_form(Vr, workloads.to_function(Vr, fr)), scatter=scatter)
+            Ar, br = dfa.Matrix(), dfa.Vector()
+            t = time.perf_counter()
+            asm_r.assemble(Ar, br)
             torch.cuda.synchronize()
-            t_s = time.perf_counter() - t
-            w_s, ev_s = timed_steps(lambda: asm_s.assemble(A, b), 10, 2)
-            strat[name] = {"ms": float(np.mean(ev_s)), "first_call_s": t_s,
-                           "hbm_frac": alg_bytes / (float(np.mean(ev_s)) * 1e-3) / 1e9 / HBM_PEAK_GBS}
-            if sc == dfa.SCATTER_COLORED:
-                strat[name]["colors"] = len(eng.color_ptr()) - 1
-                strat[name]["traffic"] = load_traffic(f"p1_colored_{args.n}")
-            else:
-                strat[name]["traffic"] = load_traffic(f"p1_atomic_{args.n}")
-            log(f"{name} scatter timed")
-        asm.assemble(A, b)
-        from dolfin_amd import autotune
-        pick, ttimes = autotune.tune_scatter([a], reps=3)
-        names = {dfa.SCATTER_BLOCKED: "blocked", dfa.SCATTER_COLORED: "colored",
-                 dfa.SCATTER_ATOMIC: "atomic"}
-        sec["p1_scatter_strategies"] = dict(
-            strat, workload=f"P1 {args.n}^3 A+b, same inputs as the headline, other scatters",
-            autotuned={"choice": names[pick],
-                       "matrix_ms": {names[k]: v for k, v in ttimes.items()}})
-        # the same workload with DOLFIN's default numbering (reverse Cuthill-McKee nodes)
-        t = time.perf_counter()
-        Vr = dfa.FunctionSpace(mesh, "P", 1, reorder=True)
-        t_rcm = time.perf_counter() - t
-        new_of = np.empty(V.dim(), np.int64)  # new dof of each vertex
-        new_of[mesh.cells().ravel()] = Vr.cell_dofs().ravel()
-        kr = np.empty_like(kappa)
-        fr = np.empty_like(f)
-        kr[new_of] = kappa
-        fr[new_of] = f
-        asm_r = dfa.SystemAssembler(dfa.poisson_form(Vr, workloads.to_function(Vr, kr)),
-                                    dfa.load_form(Vr, workloads.to_function(Vr, fr)), scatter=scatter)
-        Ar, br = dfa.Matrix(), dfa.Vector()
-        t = time.perf_counter()
-        asm_r.assemble(Ar, br)
-        torch.cuda.synchronize()
-        t_rfirst = time.perf_counter() - t
-        w_r, ev_r = timed_steps(lambda: asm_r.assemble(Ar, br), max(20, args.steps), 5)
-        sec["p1_rcm"] = {"ms": float(np.mean(ev_r)), "first_assembly_s": t_rfirst,
-                         "reordering_s": t_rcm,
-                         "workload": f"P1 {args.n}^3 A+b with reverse Cuthill-McKee dof numbering "
-                                     f"(reorder_dofs_serial)"}
-        del Ar, br, asm_r
-        log("rcm-numbered assembly timed")
-        if args.p2_n > 0:
+            t_rfirst = time.perf_counter() - t
+            w_r, ev_r = timed_steps(lambda: asm_r.assemble(Ar, br), max(20, args.steps), 5)
+            sec["p1_rcm"] = {"ms": float(np.mean(ev_r)), "first_assembly_s": t_rfirst,
+                             "reordering_s": t_rcm,
+                             "workload": f"P1 {args.n}^3 A+b with reverse Cuthill-McKee dof numbering "
+                                         f"(reorder_dofs_serial)"}
+            del Ar, br, asm_r
+            log("rcm-numbered assembly timed")
+        if args.p2_n > 0 and want("vp2"):
             log(f"vector P2 {args.p2_n}^3 setup")
             del A, b
             torch.cuda.empty_cache()
@@ -340,6 +372,7 @@ def main():
                 "hbm_frac": alg2 / (ms2 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "fp64_TFLOPs_at_39900_per_cell": nc2 * 39900 / (ms2 * 1e-3) / 1e12,
                 "algorithmic_bytes": alg2, "traffic": load_traffic(f"vp2_{sc2}_{args.p2_n}"),
+                "traffic_source": traffic_source(f"vp2_{sc2}_{args.p2_n}"),
                 "first_assembly_s": t2_first}
             x2 = m2.coordinates_device()
             mu_t, la_t, u_t = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (mu, lmbda, u2))
@@ -363,7 +396,7 @@ def main():
                                "algorithmic_bytes": mv2_bytes,
                                "traffic": load_traffic(f"vp2_spmv_{args.p2_n}"),
                                "workload": f"y = A x on the assembled vector-P2 {args.p2_n}^3 matrix"}
-        if args.batch > 0:
+        if args.batch > 0 and want("batch"):
             sec["batch"] = bench_batch(args.batch, log)
         result["secondary"] = sec
     print(json.dumps(result), flush=True)

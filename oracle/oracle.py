@@ -32,6 +32,29 @@ def lib():
     return _lib
 
 
+def use_native_build():
+    """Rebuild the oracle with -O3 -march=native for the host this runs on and route every
+    later call of this module through that build (the CPU-baseline timing uses it). Returns
+    the compiler flags used."""
+    global _lib
+    subprocess.check_call(["make", "-s", "-C", _HERE, "native"])
+    _lib = ctypes.CDLL(os.path.join(_HERE, "_build", "liboracle_native.so"))
+    _declare(_lib)
+    return "-O3 -march=native"
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or "" where that file is absent."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
