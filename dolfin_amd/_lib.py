@@ -38,7 +38,7 @@ EXPORTS = ["dfa_last_error", "dfa_version", "dfa_assembler_create", "dfa_assembl
            "dfa_batch_create", "dfa_batch_launch", "dfa_batch_info", "dfa_batch_destroy",
            "dfa_exterior_facets", "dfa_assemble_facets", "dfa_csr_spmv", "dfa_csr_diagonal",
            "dfa_node_reordering", "dfa_graph_reordering", "dfa_mesh_coloring", "dfa_color_info",
-           "dfa_device_info", "dfa_assemble_system_x0", "dfa_dirichlet_x0", "dfa_pass_kernel",
+           "dfa_device_info", "dfa_assemble_system_x0", "dfa_dirichlet_x0", "dfa_pass_kernel", "dfa_plan_builder",
            "dfa_csr_add_local", "dfa_vector_add_local", "dfa_assemble_system_x0_noalloc"]
 
 _lib = None
@@ -74,6 +74,7 @@ def load():
     L.dfa_csr_add_local.argtypes = [_vp, _vp, _i32, _vp, _i32, _vp, _vp, _vp]
     L.dfa_vector_add_local.argtypes = [_vp, _i64, _i32, _vp, _vp, _vp]
     L.dfa_pass_kernel.argtypes = [_vp, _i32, _i32, _i32, _i32, ctypes.c_char_p, _i32]
+    L.dfa_plan_builder.argtypes = [_vp, ctypes.c_void_p]
     L.dfa_adjoint.argtypes = [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.dfa_plan_info.argtypes = [_vp, _i32, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i64), ctypes.POINTER(_i32)]

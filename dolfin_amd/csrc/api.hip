@@ -124,6 +124,197 @@ struct PhaseTimer {
     t = n;
   }
 };
+
+bool env_flag(const char* name) {
+  const char* e = std::getenv(name);
+  return e && e[0] == '1';
+}
+
+// The host build of the P1 block plan, its edge chains and the chain kernels' fixed-stride
+// slots (the arrays the device build produces as well).
+struct HostP1Plan {
+  dfa::P1BlockPlan P;
+  dfa::P1ChainPlan C;
+  bool chains = false;
+  int32_t SN = 0, SR = 0;
+  std::vector<int4> hdr;
+  std::vector<int32_t> sv, sk, sd;
+  std::vector<int2> sr;
+};
+
+bool build_host_p1_plan(int32_t gdim, int64_t nv, int64_t nc, const int32_t* cells, const int32_t* cd,
+                        const int32_t* kd, const double* x, const dfa::HostCSR& csr, bool want_chains,
+                        HostP1Plan& hp, std::string& err) {
+  try {
+    auto& P = hp.P;
+    dfa::build_p1_block_plan(gdim, nv, nc, cells, cd, kd, x, csr, default_block_rows(gdim), P);
+    if (!want_chains) return true;
+    dfa::build_p1_chain_plan(P, hp.C);
+    hp.chains = true;
+    const auto& C = hp.C;
+    hp.SN = P.max_nodes;
+    hp.SR = std::max<int32_t>(P.max_rows, 1);
+    const int32_t SN = hp.SN, SR = hp.SR;
+    const int64_t nb = P.num_blocks;
+    hp.hdr.assign(static_cast<size_t>(nb), make_int4(0, 0, 0, 0));
+    hp.sv.assign(static_cast<size_t>(nb * SN), 0);
+    hp.sr.assign(static_cast<size_t>(nb * SR), make_int2(0, 0));
+    hp.sd.assign(static_cast<size_t>(nb * SR), 0);
+    if (!P.kdof_is_vertex) hp.sk.assign(static_cast<size_t>(nb * SN), 0);
+    for (int64_t b = 0; b < nb; ++b) {
+      const int32_t n0 = P.node_ptr[b], nn = P.node_ptr[b + 1] - n0;
+      const int32_t r0 = P.row_ptr[b], nr = P.row_ptr[b + 1] - r0;
+      hp.hdr[b] = make_int4(C.chain_ptr[b], C.chain_ptr[b + 1] - C.chain_ptr[b], nn, nr);
+      for (int32_t i = 0; i < SN; ++i) {  // padding repeats the block's node 0
+        const int32_t j = n0 + (i < nn ? i : 0);
+        hp.sv[b * SN + i] = P.node_vertex[j];
+        if (!hp.sk.empty()) hp.sk[b * SN + i] = P.node_kdof[j];
+      }
+      for (int32_t i = 0; i < nr; ++i) {
+        const int32_t d = P.row_dof[r0 + i];
+        const int32_t c0 = csr.row_ptr[d], len = csr.row_ptr[d + 1] - c0;
+        hp.sr[b * SR + i] = make_int2(c0, len | (static_cast<int32_t>(P.row_diag[r0 + i]) << 8));
+        hp.sd[b * SR + i] = d;
+      }
+    }
+    return true;
+  } catch (const std::exception& e) {
+    err = e.what();
+    return false;
+  }
+}
+
+void upload_host_p1_plan(dfa_assembler_s* h, const HostP1Plan& hp, hipStream_t s) {
+  const auto& P = hp.P;
+  auto& t = h->btopo;
+  t.num_blocks = P.num_blocks;
+  t.max_nodes = P.max_nodes;
+  t.max_rows = P.max_rows;
+  t.max_nnz = P.max_nnz;
+  t.max_row_len = P.max_row_len;
+  t.threads = default_block_threads();
+  if (const char* e = std::getenv("DFA_NODE_RECORD")) t.wide_nodes = std::atoi(e) != 32;
+  t.max_cells = P.max_cells;
+  t.kdof_is_vertex = P.kdof_is_vertex;
+  t.node_ptr = h->plan.upload(P.node_ptr, s);
+  t.node_vertex = h->plan.upload(P.node_vertex, s);
+  t.node_kdof = h->plan.upload(P.node_kdof, s);
+  t.node_dof = h->plan.upload(P.node_dof, s);
+  t.cell_ptr = h->plan.upload(P.cell_ptr, s);
+  std::vector<uint64_t> rec(2 * P.cell_ids.size());
+  for (size_t k = 0; k < P.cell_ids.size(); ++k) {
+    rec[2 * k] = P.cell_ids[k];
+    rec[2 * k + 1] = P.cell_rank[k];
+  }
+  t.cell_rec = h->plan.upload(rec, s);
+  if (!P.cell_rank_wide.empty()) t.cell_rank_wide = h->plan.upload(P.cell_rank_wide, s);
+  if (hp.chains) {
+    t.max_chains = hp.C.max_chains;
+    t.chain_threads = default_chain_threads();
+    t.chain_ptr = h->plan.upload(hp.C.chain_ptr, s);
+    t.chain_rec = h->plan.upload(hp.C.chain_rec, s);
+    t.slot_nodes = hp.SN;
+    t.slot_rows = hp.SR;
+    t.blk_hdr = h->plan.upload(hp.hdr, s);
+    t.slot_vertex = h->plan.upload(hp.sv, s);
+    if (!hp.sk.empty()) t.slot_kdof = h->plan.upload(hp.sk, s);
+    t.slot_row = h->plan.upload(hp.sr, s);
+    t.slot_rdof = h->plan.upload(hp.sd, s);
+  }
+  t.row_ptr_b = h->plan.upload(P.row_ptr, s);
+  t.row_dof = h->plan.upload(P.row_dof, s);
+  t.row_info = h->plan.upload(P.row_info, s);
+  t.row_diag = h->plan.upload(P.row_diag, s);
+  t.blk_nnz = h->plan.upload(P.blk_nnz, s);
+  t.run_ptr = h->plan.upload(P.run_ptr, s);
+  t.runs = h->plan.upload(P.runs, s);
+  h->kdof_injective = P.kdof_injective;
+  t.csr_row_ptr = h->d_row_ptr;
+  h->appearances = static_cast<int64_t>(P.cell_ids.size());
+  h->has_plan = true;
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");  // host vectors die after this
+}
+
+// Element-by-element comparison of the device-built plan with the host build ("" = equal).
+std::string compare_p1_plans(dfa_assembler_s* h, const HostP1Plan& hp, bool host_ok,
+                             const std::string& herr, bool dev_failed, const std::string& derr,
+                             hipStream_t s) {
+  if (!host_ok || dev_failed) {
+    if (host_ok != !dev_failed)
+      return std::string("device plan ") + (dev_failed ? "failed (" + derr + ")" : "built") +
+             ", host plan " + (host_ok ? "built" : "failed (" + herr + ")");
+    return herr == derr ? "" : "different errors: device '" + derr + "', host '" + herr + "'";
+  }
+  const auto& P = hp.P;
+  const auto& t = h->btopo;
+  auto same = [&](const char* name, const auto* d, const auto& hv) -> std::string {
+    using T = std::remove_cv_t<std::remove_pointer_t<decltype(d)>>;
+    const size_t n = hv.size();
+    if (n && !d) return std::string(name) + ": missing on the device";
+    std::vector<T> dv(n);
+    if (n) hip_check(hipMemcpy(dv.data(), d, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (n && std::memcmp(dv.data(), hv.data(), n * sizeof(T)) != 0) {
+      size_t i = 0;
+      while (std::memcmp(&dv[i], &hv[i], sizeof(T)) == 0) ++i;
+      return std::string(name) + ": first difference at element " + std::to_string(i) + " of " +
+             std::to_string(n);
+    }
+    return "";
+  };
+  (void)s;
+  std::string r;
+  auto chk = [&](std::string x) {
+    if (r.empty() && !x.empty()) r = x;
+  };
+  auto scalar = [&](const char* name, int64_t a, int64_t b) {
+    if (a != b) chk(std::string(name) + ": device " + std::to_string(a) + ", host " + std::to_string(b));
+  };
+  scalar("num_blocks", t.num_blocks, P.num_blocks);
+  scalar("max_nodes", t.max_nodes, P.max_nodes);
+  scalar("max_rows", t.max_rows, P.max_rows);
+  scalar("max_nnz", t.max_nnz, P.max_nnz);
+  scalar("max_cells", t.max_cells, P.max_cells);
+  scalar("max_row_len", t.max_row_len, P.max_row_len);
+  scalar("kdof_is_vertex", t.kdof_is_vertex, P.kdof_is_vertex);
+  scalar("kdof_injective", h->kdof_injective, P.kdof_injective);
+  scalar("appearances", h->appearances, static_cast<int64_t>(P.cell_ids.size()));
+  if (!r.empty()) return r;
+  std::vector<uint64_t> rec(2 * P.cell_ids.size());
+  for (size_t k = 0; k < P.cell_ids.size(); ++k) {
+    rec[2 * k] = P.cell_ids[k];
+    rec[2 * k + 1] = P.cell_rank[k];
+  }
+  chk(same("node_ptr", t.node_ptr, P.node_ptr));
+  chk(same("node_vertex", t.node_vertex, P.node_vertex));
+  chk(same("node_kdof", t.node_kdof, P.node_kdof));
+  chk(same("node_dof", t.node_dof, P.node_dof));
+  chk(same("cell_ptr", t.cell_ptr, P.cell_ptr));
+  chk(same("cell_rec", t.cell_rec, rec));
+  chk(same("cell_rank_wide", t.cell_rank_wide, P.cell_rank_wide));
+  chk(same("row_ptr", t.row_ptr_b, P.row_ptr));
+  chk(same("row_dof", t.row_dof, P.row_dof));
+  chk(same("row_info", t.row_info, P.row_info));
+  chk(same("row_diag", t.row_diag, P.row_diag));
+  chk(same("blk_nnz", t.blk_nnz, P.blk_nnz));
+  chk(same("run_ptr", t.run_ptr, P.run_ptr));
+  chk(same("runs", t.runs, P.runs));
+  if (hp.chains) {
+    scalar("max_chains", t.max_chains, hp.C.max_chains);
+    scalar("slot_nodes", t.slot_nodes, hp.SN);
+    scalar("slot_rows", t.slot_rows, hp.SR);
+    if (!r.empty()) return r;
+    chk(same("chain_ptr", t.chain_ptr, hp.C.chain_ptr));
+    chk(same("chain_rec", t.chain_rec, hp.C.chain_rec));
+    chk(same("blk_hdr", t.blk_hdr, hp.hdr));
+    chk(same("slot_vertex", t.slot_vertex, hp.sv));
+    chk(same("slot_kdof", t.slot_kdof, hp.sk));
+    chk(same("slot_row", t.slot_row, hp.sr));
+    chk(same("slot_rdof", t.slot_rdof, hp.sd));
+  } else if (t.chain_ptr) {
+    chk("chains: built on the device only");
+  }
+  return r;
+}
 }  // namespace
 
 int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t nc,
@@ -176,22 +367,38 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
       h->d_kdofs = h->base.upload(h_coeff_dofs, static_cast<size_t>(nc * nvc), s);
       timer.mark("check + upload mesh");
       // CSR structure: built on the device (sort of the cell pairs) unless device memory is
-      // short or DFA_HOST_CSR=1, then on the host; both give the same arrays
+      // short or DFA_HOST_CSR=1, then on the host; both give the same arrays. The column
+      // indices come back to the host only when a host plan builder needs them.
+      const bool verify_plan = env_flag("DFA_PLAN_VERIFY");
+      const bool device_plan = element == DFA_ELEMENT_P1 && gdim == 3 && h_x &&
+                               !env_flag("DFA_HOST_PLAN");
       dfa::HostCSR csr;
-      const char* host_csr = std::getenv("DFA_HOST_CSR");
       int32_t* drp = nullptr;
       int32_t* dcol = nullptr;
-      if (!(host_csr && host_csr[0] == '1') &&
-          dfa::build_csr_device(nc, nd, h->d_cdofs, ndofs, keep_diagonal != 0, csr, &drp, &dcol, s)) {
+      bool col_on_host = true;
+      if (!env_flag("DFA_HOST_CSR") &&
+          dfa::build_csr_device(nc, nd, h->d_cdofs, ndofs, keep_diagonal != 0, csr, &drp, &dcol, s,
+                                !device_plan || verify_plan)) {
+        h->nnz = csr.row_ptr.back();
         h->d_row_ptr = h->base.adopt(drp, static_cast<int64_t>(csr.row_ptr.size() * sizeof(int32_t)));
-        h->d_col = h->base.adopt(dcol, static_cast<int64_t>(csr.col.size() * sizeof(int32_t)));
+        h->d_col = h->base.adopt(dcol, h->nnz * static_cast<int64_t>(sizeof(int32_t)));
+        col_on_host = !device_plan || verify_plan;
       } else {
         dfa::build_csr(nc, nd, h_cell_dofs, ndofs, keep_diagonal != 0, csr);
+        h->nnz = static_cast<int64_t>(csr.col.size());
         h->d_row_ptr = h->base.upload(csr.row_ptr, s);
         h->d_col = h->base.upload(csr.col, s);
       }
-      h->nnz = static_cast<int64_t>(csr.col.size());
       timer.mark("CSR structure");
+      auto ensure_host_col = [&]() {
+        if (col_on_host) return;
+        csr.col.resize(static_cast<size_t>(h->nnz));
+        if (h->nnz)
+          hip_check(hipMemcpyAsync(csr.col.data(), h->d_col, h->nnz * sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        col_on_host = true;
+      };
       {
         const std::vector<int32_t> sb = dfa::build_spmv_blocks(csr.row_ptr, dfa::kSpmvBudget,
                                                                dfa::kSpmvMaxRows);
@@ -205,93 +412,61 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
       }
       if (element == DFA_ELEMENT_P1) {
         if (h_x) {
-          try {
-            dfa::P1BlockPlan P;
-            dfa::build_p1_block_plan(gdim, nv, nc, h_cells, h_cell_dofs, h_coeff_dofs, h_x, csr,
-                                     default_block_rows(gdim), P);
-            timer.mark("P1 block plan");
-            auto& t = h->btopo;
-            t.num_blocks = P.num_blocks;
-            t.max_nodes = P.max_nodes;
-            t.max_rows = P.max_rows;
-            t.max_nnz = P.max_nnz;
-            t.max_row_len = P.max_row_len;
-            t.threads = default_block_threads();
-            if (const char* e = std::getenv("DFA_NODE_RECORD")) t.wide_nodes = std::atoi(e) != 32;
-            t.max_cells = P.max_cells;
-            t.kdof_is_vertex = P.kdof_is_vertex;
-            t.node_ptr = h->plan.upload(P.node_ptr, s);
-            t.node_vertex = h->plan.upload(P.node_vertex, s);
-            t.node_kdof = h->plan.upload(P.node_kdof, s);
-            t.node_dof = h->plan.upload(P.node_dof, s);
-            t.cell_ptr = h->plan.upload(P.cell_ptr, s);
-            // host arrays stay alive until the stream synchronisation below
-            std::vector<uint64_t> rec(2 * P.cell_ids.size());
-            for (size_t k = 0; k < P.cell_ids.size(); ++k) {
-              rec[2 * k] = P.cell_ids[k];
-              rec[2 * k + 1] = P.cell_rank[k];
+          const char* ce = std::getenv("DFA_P1_CHAINS");
+          const bool want_chains = gdim == 3 && !(ce && std::atoi(ce) == 0);
+          int rc = dfa::kDevicePlanUnsupported;
+          std::string derr;
+          if (device_plan) {
+            dfa::DevicePlanInput in;
+            in.gdim = gdim;
+            in.nv = nv;
+            in.nc = nc;
+            in.ndofs = ndofs;
+            in.nkdofs = nkdofs;
+            in.d_cells = h->d_cells;
+            in.d_cdofs = h->d_cdofs;
+            in.d_kdofs = h->d_kdofs;
+            in.h_x = h_x;
+            in.d_row_ptr = h->d_row_ptr;
+            in.d_col = h->d_col;
+            for (size_t r = 0; r + 1 < csr.row_ptr.size(); ++r)
+              in.max_row_len = std::max(in.max_row_len, csr.row_ptr[r + 1] - csr.row_ptr[r]);
+            dfa::DevicePlanStats st;
+            dfa::P1BlockTopo t;
+            rc = dfa::build_p1_plan_device(in, default_block_rows(gdim), want_chains,
+                                           default_block_threads(), default_chain_threads(), h->plan,
+                                           t, st, derr, s);
+            if (rc == dfa::kDevicePlanOk) {
+              if (const char* e = std::getenv("DFA_NODE_RECORD")) t.wide_nodes = std::atoi(e) != 32;
+              t.csr_row_ptr = h->d_row_ptr;
+              h->btopo = t;
+              h->appearances = st.appearances;
+              h->kdof_injective = st.kdof_injective;
+              h->has_plan = true;
+              h->plan_on_device = true;
+              timer.mark("P1 block + chain plan (device)");
+            } else if (rc == dfa::kDevicePlanError) {
+              h->plan.release();
+              h->plan_error = derr;
+            } else {
+              h->plan.release();
             }
-            t.cell_rec = h->plan.upload(rec, s);
-            if (!P.cell_rank_wide.empty()) t.cell_rank_wide = h->plan.upload(P.cell_rank_wide, s);
-            dfa::P1ChainPlan C;
-            const char* ce = std::getenv("DFA_P1_CHAINS");
-            if (gdim == 3 && !(ce && std::atoi(ce) == 0)) {
-              dfa::build_p1_chain_plan(P, C);
-              timer.mark("P1 edge-chain plan");
-              t.max_chains = C.max_chains;
-              t.chain_threads = default_chain_threads();
-              t.chain_ptr = h->plan.upload(C.chain_ptr, s);
-              t.chain_rec = h->plan.upload(C.chain_rec, s);
-              // fixed-stride slots (host arrays live until the synchronisation below)
-              const int32_t SN = P.max_nodes, SR = std::max<int32_t>(P.max_rows, 1);
-              const int64_t nb = P.num_blocks;
-              std::vector<int4> hdr(static_cast<size_t>(nb));
-              std::vector<int32_t> sv(static_cast<size_t>(nb * SN)), sk;
-              std::vector<int2> sr(static_cast<size_t>(nb * SR), make_int2(0, 0));
-              std::vector<int32_t> sd(static_cast<size_t>(nb * SR), 0);
-              if (!P.kdof_is_vertex) sk.resize(static_cast<size_t>(nb * SN));
-              for (int64_t b = 0; b < nb; ++b) {
-                const int32_t n0 = P.node_ptr[b], nn = P.node_ptr[b + 1] - n0;
-                const int32_t r0 = P.row_ptr[b], nr = P.row_ptr[b + 1] - r0;
-                hdr[b] = make_int4(C.chain_ptr[b], C.chain_ptr[b + 1] - C.chain_ptr[b], nn, nr);
-                for (int32_t i = 0; i < SN; ++i) {
-                  const int32_t j = n0 + (i < nn ? i : 0);
-                  sv[b * SN + i] = P.node_vertex[j];
-                  if (!sk.empty()) sk[b * SN + i] = P.node_kdof[j];
-                }
-                for (int32_t i = 0; i < nr; ++i) {
-                  const int32_t d = P.row_dof[r0 + i];
-                  const int32_t c0 = csr.row_ptr[d], len = csr.row_ptr[d + 1] - c0;
-                  sr[b * SR + i] = make_int2(c0, len | (static_cast<int32_t>(P.row_diag[r0 + i]) << 8));
-                  sd[b * SR + i] = d;
-                }
-              }
-              t.slot_nodes = SN;
-              t.slot_rows = SR;
-              t.blk_hdr = h->plan.upload(hdr, s);
-              t.slot_vertex = h->plan.upload(sv, s);
-              if (!sk.empty()) t.slot_kdof = h->plan.upload(sk, s);
-              t.slot_row = h->plan.upload(sr, s);
-              t.slot_rdof = h->plan.upload(sd, s);
-              hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+          }
+          if (rc == dfa::kDevicePlanUnsupported || verify_plan) {
+            ensure_host_col();
+            HostP1Plan hp;
+            std::string herr;
+            const bool ok = build_host_p1_plan(gdim, nv, nc, h_cells, h_cell_dofs, h_coeff_dofs, h_x, csr,
+                                               want_chains, hp, herr);
+            timer.mark("P1 block + chain plan (host)");
+            if (rc == dfa::kDevicePlanUnsupported) {
+              if (ok) upload_host_p1_plan(h, hp, s);
+              else h->plan_error = herr;
+            } else {
+              h->plan_verify_error = compare_p1_plans(h, hp, ok, herr, rc == dfa::kDevicePlanError, derr, s);
+              if (!h->plan_verify_error.empty())
+                std::fprintf(stderr, "[dfa] plan verification: %s\n", h->plan_verify_error.c_str());
             }
-            t.row_ptr_b = h->plan.upload(P.row_ptr, s);
-            t.row_dof = h->plan.upload(P.row_dof, s);
-            t.row_info = h->plan.upload(P.row_info, s);
-            t.row_diag = h->plan.upload(P.row_diag, s);
-            t.blk_nnz = h->plan.upload(P.blk_nnz, s);
-            t.run_ptr = h->plan.upload(P.run_ptr, s);
-            t.runs = h->plan.upload(P.runs, s);
-            h->kdof_injective = P.kdof_injective;
-            t.csr_row_ptr = h->d_row_ptr;
-            h->appearances = static_cast<int64_t>(P.cell_ids.size());
-            h->has_plan = true;
-            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-          } catch (const dfa::HipError&) {
-            throw;
-          } catch (const std::exception& e) {
-            h->plan.release();
-            h->plan_error = e.what();
           }
         } else {
           h->plan_error = "no coordinates given at creation";
@@ -737,6 +912,16 @@ int dfa_assemble_system_x0_noalloc(dfa_assembler h, int32_t a_form, int32_t L_fo
     }
     return run_pass(h, a_form, L_form, d_x, d_a_c0, d_a_c1, d_L_c0, h_f3, d_bc_mask, bcv,
                     d_values, d_b, scatter, flags, s);
+  });
+}
+
+int dfa_plan_builder(dfa_assembler h, int32_t* on_device) {
+  return guarded([&]() -> int {
+    if (!h || !on_device) return fail("NULL argument", DFA_ERR_ARG);
+    *on_device = h->has_plan && h->plan_on_device ? 1 : 0;
+    if (!h->plan_verify_error.empty())
+      return fail("device plan differs from the host plan: " + h->plan_verify_error, DFA_ERR_STATE);
+    return DFA_OK;
   });
 }
 

@@ -58,7 +58,34 @@ struct DeviceArena {
 // is short.
 bool build_csr_device(int64_t num_cells, int32_t nd, const int32_t* d_cell_dofs,
                       int64_t num_rows, bool keep_diagonal, HostCSR& out, int32_t** d_row_ptr,
-                      int32_t** d_col, hipStream_t stream);
+                      int32_t** d_col, hipStream_t stream, bool download_col = true);
+
+// Device build of the P1 block plan and edge chains (plan_device.hip): the same arrays as
+// build_p1_block_plan + build_p1_chain_plan + the slot arrays of the edge-chain kernels,
+// allocated in `plan` and described by `t`. Tetrahedra only.
+struct DevicePlanInput {
+  int32_t gdim = 3;
+  int64_t nv = 0, nc = 0, ndofs = 0, nkdofs = 0;
+  const int32_t* d_cells = nullptr;
+  const int32_t* d_cdofs = nullptr;
+  const int32_t* d_kdofs = nullptr;
+  const double* h_x = nullptr;       // host coordinates (uploaded for the partition)
+  const int32_t* d_row_ptr = nullptr;
+  const int32_t* d_col = nullptr;
+  int32_t max_row_len = 0;
+};
+struct DevicePlanStats {
+  int64_t appearances = 0;
+  bool kdof_injective = false;
+  bool chains_built = false;
+  int32_t lds_bytes = 0;
+};
+constexpr int kDevicePlanOk = 0;
+constexpr int kDevicePlanError = 1;        // the input has no blocked plan (message in err)
+constexpr int kDevicePlanUnsupported = 2;  // outside the device build's limits: build on the host
+int build_p1_plan_device(const DevicePlanInput& in, int32_t target_rows, bool want_chains,
+                         int32_t block_threads, int32_t chain_threads, DeviceArena& plan,
+                         P1BlockTopo& t, DevicePlanStats& stats, std::string& err, hipStream_t s);
 
 }  // namespace dfa
 
@@ -76,7 +103,9 @@ struct dfa_assembler_s {
   dfa::DeviceArena base;
   // blocked P1 plan
   bool has_plan = false;
+  bool plan_on_device = false;   // the P1 block plan was built by the device planner
   std::string plan_error;
+  std::string plan_verify_error; // DFA_PLAN_VERIFY=1: how the device plan differs from the host's
   dfa::P1BlockTopo btopo;
   dfa::DeviceArena plan;
   int64_t appearances = 0;

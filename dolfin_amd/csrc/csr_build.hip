@@ -77,7 +77,7 @@ struct DeviceBuffer {
 
 bool build_csr_device(int64_t nc, int32_t nd, const int32_t* d_cell_dofs, int64_t ndofs,
                       bool keep_diagonal, HostCSR& out, int32_t** d_row_ptr, int32_t** d_col,
-                      hipStream_t s) {
+                      hipStream_t s, bool download_col) {
   if (ndofs <= 0 || ndofs > INT32_MAX) return false;
   int cb = 1;
   while ((int64_t{1} << cb) < ndofs) ++cb;
@@ -138,10 +138,10 @@ bool build_csr_device(int64_t nc, int32_t nd, const int32_t* d_cell_dofs, int64_
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) {
     out.row_ptr.resize(static_cast<size_t>(ndofs) + 1);
-    out.col.resize(static_cast<size_t>(nnz));
+    out.col.resize(download_col ? static_cast<size_t>(nnz) : 0);
     e = hipMemcpyAsync(out.row_ptr.data(), rp, out.row_ptr.size() * sizeof(int32_t),
                        hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess && nnz > 0)
+    if (e == hipSuccess && nnz > 0 && download_col)
       e = hipMemcpyAsync(out.col.data(), col, out.col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
   }
