@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--n", type=int, default=128, help="P1 mesh size (cubes per side)")
     ap.add_argument("--p2-n", type=int, default=64, help="vector P2 mesh size (0 = skip)")
     ap.add_argument("--scatter", default="auto", choices=["auto", "atomic", "blocked", "colored"])
+    ap.add_argument("--settle", type=float, default=0.3,
+                    help="seconds of untimed steps after the warmup steps, before timing (clock ramp)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--sections", default=SECTIONS,
@@ -45,15 +47,25 @@ def parse():
     return ap.parse_args()
 
 
-def timed_steps(fn, steps, warmup):
-    """Run warmup steps, then `steps` timed steps back to back between two HIP events on the
-    launch stream (no event between steps, so none of their overhead is counted). Returns
-    (wall seconds, GPU ms per step repeated per step); the host queues launches faster than
-    the kernels run, so the GPU time per step is the kernels' time plus launch gaps."""
+def timed_steps(fn, steps, warmup, settle_s=0.0):
+    """Run warmup steps, then untimed steps for settle_s more seconds (the GPU raises its
+    clocks over the first tens of milliseconds of sustained work: at 128^3 the first 50 A+b
+    passes average 0.227 ms, passes after 40 ms of work 0.197 ms), then `steps` timed steps
+    back to back between two HIP events on the launch stream (no event between steps, so
+    none of their overhead is counted). Returns (wall seconds, GPU ms per step repeated per
+    step, settle steps run); the host queues launches faster than the kernels run, so the GPU
+    time per step is the kernels' time plus launch gaps."""
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    settle = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < settle_s:
+        for _ in range(10):
+            fn()
+        settle += 10
+        torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
@@ -63,7 +75,7 @@ def timed_steps(fn, steps, warmup):
     b.record(s)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return wall, [a.elapsed_time(b) / steps] * steps
+    return wall, [a.elapsed_time(b) / steps] * steps, settle
 
 
 def load_traffic(name):
@@ -86,23 +98,43 @@ def traffic_source(name):
     return f"profiles/traffic.json[{name}]: rocprofv3 --pmc counters, {src}"
 
 
-def bench_batch(nmembers, log, seed=5):
+def bench_batch(nmembers, log, seed=5, settle_s=0.0):
     """Batched launch: nmembers UnitCubeMesh(n) instances, n ~ U{8..48}, half P1 kappa-Poisson
     and half vector-P2 elasticity with random coefficients, assembled into separate CSR
-    matrices by one graph-captured launch of two kernels (vector-P2, P1); the launch mode (one
-    workgroup per row block, or persistent workgroups per CU) is autotuned."""
+    matrices by one graph-captured launch of two kernels (vector-P2, P1). Autotuned: the cells
+    per workgroup of each kernel (P1 rows per block, vector-P2 LDS values per block) on a
+    sample of the members, then the launch mode (one workgroup per row block, or persistent
+    workgroups per CU) on the whole batch. Setup is reported by phase."""
     import torch
     import dolfin_amd as dfa
     from dolfin_amd import autotune, workloads
     from dolfin_amd.batch import BatchAssembler
     t = time.perf_counter()
+    specs = workloads.batch_specs(nmembers, seed)
     forms = workloads.batch_members(nmembers, seed, log=log)
+    t_members = time.perf_counter() - t
+    t = time.perf_counter()
+    sample = workloads.batch_sample(specs)
+    (p1_rows, p2_vals), plan_times = autotune.tune_batch_plans(sample, workloads.batch_form)
+    t_tune_plans = time.perf_counter() - t
+    log(f"batch: plan sizes tuned ({t_tune_plans:.1f} s): P1 {p1_rows} rows, P2 {p2_vals} values")
+    t = time.perf_counter()
+    autotune.set_plan_defaults(p1_rows, 0, p2_vals)
+    try:
+        batch = BatchAssembler(forms, grid_per_cu=0)   # builds every member's structure and plan
+    finally:
+        autotune.set_plan_defaults(0, 0, 0)
+    torch.cuda.synchronize()
+    t_plans = time.perf_counter() - t
+    log(f"batch: structures and plans built ({t_plans:.1f} s)")
+    t = time.perf_counter()
+    best, gtimes = autotune.tune_batch_grid(
+        lambda g: batch if g == 0 else BatchAssembler(forms, grid_per_cu=g), grids=(1, 2, 0), reps=3)
+    if best != 0:
+        batch = BatchAssembler(forms, grid_per_cu=best)
+    t_tune_grid = time.perf_counter() - t
     alg = 0
     cells = sum(a.V.mesh().num_cells() for a in forms)
-    best, gtimes = autotune.tune_batch_grid(lambda g: BatchAssembler(forms, grid_per_cu=g),
-                                            grids=(1, 2, 0), reps=3)
-    batch = BatchAssembler(forms, grid_per_cu=best)
-    t_setup = time.perf_counter() - t
     for a, A in zip(forms, batch.tensors):
         e = dfa.engine_for(a.V)
         m = a.V.mesh()
@@ -110,7 +142,7 @@ def bench_batch(nmembers, log, seed=5):
             alg += workloads.p1_algorithmic_bytes(m.num_cells(), 4, m.num_vertices(), 3, e.nnz, a.V.dim())
         else:
             alg += workloads.vp2_algorithmic_bytes(m.num_cells(), m.num_vertices(), e.nnz, a.V.dim())
-    w, ev = timed_steps(batch.assemble, 5, 2)
+    w, ev, _ = timed_steps(batch.assemble, 5, 2, settle_s=settle_s)
     ms = float(np.mean(ev))
     return {"workload": f"{nmembers} UnitCubeMesh instances n~U{{8..48}} (seed {seed}), half P1 "
                         f"kappa-Poisson, half vector-P2 elasticity, {cells} cells in total, one "
@@ -118,7 +150,13 @@ def bench_batch(nmembers, log, seed=5):
             "ms": ms, "cells_per_s": cells / (ms * 1e-3),
             "hbm_achieved_GBs": alg / (ms * 1e-3) / 1e9, "hbm_frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "algorithmic_bytes": alg, "info": batch.info, "grid_per_cu": best,
-            "grid_sweep_ms": {str(k): v for k, v in gtimes.items()}, "setup_s": t_setup}
+            "grid_sweep_ms": {str(k): v for k, v in gtimes.items()},
+            "plan_sizes": {"p1_rows_per_block": p1_rows, "vp2_values_per_block": p2_vals,
+                           "sample_members": len(sample),
+                           "sweep_ms": {k: {str(c): v for c, v in d.items()}
+                                        for k, d in plan_times.items()}},
+            "setup_s": {"members": t_members, "tune_plan_sizes": t_tune_plans,
+                        "structures_and_plans": t_plans, "tune_launch_mode": t_tune_grid}}
 
 
 def device_info(dev):
@@ -172,7 +210,9 @@ def main():
     log(f"first assembly {t_first:.1f} s; timing {args.steps} steps")
     eng = dfa.engine_for(V)
     info = eng.plan_info(scatter)
-    wall, ev = timed_steps(lambda: asm.assemble(A, b), args.steps, args.warmup)
+    _, ev_cold, _ = timed_steps(lambda: asm.assemble(A, b), args.steps, args.warmup)
+    wall, ev, n_settle = timed_steps(lambda: asm.assemble(A, b), args.steps, args.warmup,
+                                     settle_s=args.settle)
     nc = mesh.num_cells()
     ms = 1e3 * wall / args.steps
     ev_ms = float(np.mean(ev))
@@ -207,6 +247,10 @@ def main():
                      "traffic_source": traffic_source(f"p1_{used_scatter}_{args.n}"),
                      "algorithmic_bytes": alg_bytes, "kernel_ms": ev_ms},
         "first_assembly_s": t_first,
+        "settle": {"seconds": args.settle, "steps": n_settle,
+                   "cold_kernel_ms": float(np.mean(ev_cold)),
+                   "note": "untimed steps after the warmup steps while the GPU raises its clocks; "
+                           "cold_kernel_ms is the same K steps timed right after the warmup steps"},
     }
 
     # ---- CPU baseline: the oracle's SystemAssembler cell loop on the host ----------------------
@@ -269,7 +313,7 @@ def main():
         if want("adjoint"):
             def adj():
                 adjoint._run(V, dfa._lib.FORM_P1_POISSON, xx, kk, None, ff, None, uu)
-            w_adj, ev_adj = timed_steps(adj, max(10, args.steps // 2), 3)
+            w_adj, ev_adj, _ = timed_steps(adj, max(10, args.steps // 2), 3, settle_s=args.settle)
             sec["p1_adjoint"] = {"ms": float(np.mean(ev_adj)),
                                  "ratio_to_forward": float(np.mean(ev_adj)) / ev_ms,
                                  "workload": f"dJ/dkappa, dJ/dx of u^T K u - F^T u, P1 {args.n}^3"}
@@ -277,7 +321,7 @@ def main():
         if want("spmv"):
             xs = torch.rand(V.dim(), dtype=torch.float64, device="cuda")
             ys = torch.empty_like(xs)
-            w_mv, ev_mv = timed_steps(lambda: A.mult(xs, ys), max(20, args.steps), 5)
+            w_mv, ev_mv, _ = timed_steps(lambda: A.mult(xs, ys), max(20, args.steps), 5, settle_s=args.settle)
             mv_ms = float(np.mean(ev_mv))
             mv_bytes = 12 * eng.nnz + 4 * (V.dim() + 1) + 16 * V.dim()
             sec["p1_spmv"] = {"ms": mv_ms, "hbm_achieved_GBs": mv_bytes / (mv_ms * 1e-3) / 1e9,
@@ -298,7 +342,7 @@ def main():
                 asm_s.assemble(A, b)
                 torch.cuda.synchronize()
                 t_s = time.perf_counter() - t
-                w_s, ev_s = timed_steps(lambda: asm_s.assemble(A, b), 10, 2)
+                w_s, ev_s, _ = timed_steps(lambda: asm_s.assemble(A, b), 10, 2, settle_s=args.settle)
                 strat[name] = {"ms": float(np.mean(ev_s)), "first_call_s": t_s,
                                "hbm_frac": alg_bytes / (float(np.mean(ev_s)) * 1e-3) / 1e9 / HBM_PEAK_GBS}
                 if sc == dfa.SCATTER_COLORED:
@@ -334,7 +378,7 @@ def main():
             asm_r.assemble(Ar, br)
             torch.cuda.synchronize()
             t_rfirst = time.perf_counter() - t
-            w_r, ev_r = timed_steps(lambda: asm_r.assemble(Ar, br), max(20, args.steps), 5)
+            w_r, ev_r, _ = timed_steps(lambda: asm_r.assemble(Ar, br), max(20, args.steps), 5, settle_s=args.settle)
             sec["p1_rcm"] = {"ms": float(np.mean(ev_r)), "first_assembly_s": t_rfirst,
                              "reordering_s": t_rcm,
                              "workload": f"P1 {args.n}^3 A+b with reverse Cuthill-McKee dof numbering "
@@ -359,7 +403,7 @@ def main():
             e2 = dfa.engine_for(V2)
             info2 = e2.plan_info()
             sc2 = "blocked" if info2["num_blocks"] > 0 else "atomic"
-            w2, ev2 = timed_steps(lambda: asm2.assemble(A2, b2), max(5, args.steps // 5), 2)
+            w2, ev2, _ = timed_steps(lambda: asm2.assemble(A2, b2), max(5, args.steps // 5), 2, settle_s=args.settle)
             nc2 = m2.num_cells()
             ms2 = float(np.mean(ev2))
             alg2 = workloads.vp2_algorithmic_bytes(nc2, m2.num_vertices(), e2.nnz, V2.dim())
@@ -380,14 +424,14 @@ def main():
 
             def adj2():
                 adjoint._run(V2, dfa._lib.FORM_VP2_ELASTICITY, x2, mu_t, la_t, None, f3a, u_t)
-            w_a2, ev_a2 = timed_steps(adj2, max(5, args.steps // 5), 2)
+            w_a2, ev_a2, _ = timed_steps(adj2, max(5, args.steps // 5), 2, settle_s=args.settle)
             sec["vp2_adjoint"] = {"ms": float(np.mean(ev_a2)),
                                   "ratio_to_forward": float(np.mean(ev_a2)) / ms2,
                                   "workload": f"dJ/dmu, dJ/dlmbda, dJ/dx of u^T K u - F^T u, "
                                               f"vector P2 {args.p2_n}^3"}
             xs2 = torch.rand(V2.dim(), dtype=torch.float64, device="cuda")
             ys2 = torch.empty_like(xs2)
-            w_mv2, ev_mv2 = timed_steps(lambda: A2.mult(xs2, ys2), max(10, args.steps // 5), 2)
+            w_mv2, ev_mv2, _ = timed_steps(lambda: A2.mult(xs2, ys2), max(10, args.steps // 5), 2, settle_s=args.settle)
             mv2_ms = float(np.mean(ev_mv2))
             mv2_bytes = 12 * e2.nnz + 4 * (V2.dim() + 1) + 16 * V2.dim()
             sec["vp2_spmv"] = {"ms": mv2_ms,
@@ -397,7 +441,7 @@ def main():
                                "traffic": load_traffic(f"vp2_spmv_{args.p2_n}"),
                                "workload": f"y = A x on the assembled vector-P2 {args.p2_n}^3 matrix"}
         if args.batch > 0 and want("batch"):
-            sec["batch"] = bench_batch(args.batch, log)
+            sec["batch"] = bench_batch(args.batch, log, settle_s=args.settle)
         result["secondary"] = sec
     print(json.dumps(result), flush=True)
 

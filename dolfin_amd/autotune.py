@@ -93,13 +93,40 @@ def tune_scatter(forms, scatters=None, reps=5):
 
 def tune_batch_grid(make_batch, grids=(1, 2, 4, 0), reps=5):
     """Time the batched launch for several persistent-workgroups-per-CU counts (0: one
-    workgroup per row block)."""
+    workgroup per row block). make_batch(g) should reuse the members' spaces, whose plans are
+    then already built."""
     times = {}
     for g in grids:
         b = make_batch(g)
         times[g] = _time(b.assemble, reps)
         del b
     best = min(times, key=times.get)
+    return best, times
+
+
+def tune_batch_plans(specs, make_form, p1_rows=(128, 160, 192), p2_vals=(4096, 4608, 6144),
+                     reps=3):
+    """Cells per workgroup of the batched launch: the row-block size of the P1 members and the
+    LDS values per block of the vector-P2 members, timed through the batched launch itself on
+    sample members (specs, built by make_form on fresh spaces so that every candidate builds
+    its own plans). The two element kinds run as separate kernels of the launch, so each is
+    tuned on its own members. Returns ((p1_rows, p2_vals), {"p1_rows": {rows: ms},
+    "p2_vals": {vals: ms}})."""
+    from .batch import BatchAssembler
+    times = {"p1_rows": {}, "p2_vals": {}}
+    for kind, key, cands in ((0, "p1_rows", p1_rows), (1, "p2_vals", p2_vals)):
+        members = [sp for sp in specs if sp[0] == kind]
+        if not members:
+            continue
+        for c in cands:
+            set_plan_defaults(c if kind == 0 else 0, 0, c if kind == 1 else 0)
+            try:
+                b = BatchAssembler([make_form(sp) for sp in members])
+            finally:
+                set_plan_defaults(0, 0, 0)
+            times[key][c] = _time(b.assemble, reps)
+            del b
+    best = tuple(min(times[k], key=times[k].get) if times[k] else 0 for k in ("p1_rows", "p2_vals"))
     return best, times
 
 

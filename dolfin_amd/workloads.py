@@ -78,26 +78,49 @@ def vp2_algorithmic_bytes(num_cells, num_vertices, nnz, num_rows):
             + 4 * (num_rows + 1))
 
 
-def batch_members(nmembers=256, seed=5, log=None):
-    """The batched workload: nmembers UnitCubeMesh(n) instances, n ~ U{8..48} drawn from
-    default_rng(seed), even members P1 kappa-Poisson, odd members vector-P2 elasticity, each
-    with its own seeded coefficients (p1_poisson_3d / vp2_elasticity_3d). Returns the list of
-    bilinear forms."""
-    from .fem import elasticity_form, poisson_form
+def batch_specs(nmembers=256, seed=5):
+    """The members of the batched workload as (kind, n, member seed): UnitCubeMesh(n)
+    instances, n ~ U{8..48} drawn from default_rng(seed), even members (kind 0) P1
+    kappa-Poisson, odd members (kind 1) vector-P2 elasticity, each with its own seed."""
     rng = np.random.default_rng(seed)
     sizes = rng.integers(8, 49, nmembers)
+    return [(i % 2, int(n), int(rng.integers(1 << 30))) for i, n in enumerate(sizes)]
+
+
+def batch_form(spec):
+    """The bilinear form of one batch member (kind, n, seed): p1_poisson_3d or
+    vp2_elasticity_3d with its seeded coefficients, on a fresh function space."""
+    from .fem import elasticity_form, poisson_form
+    kind, n, s = spec
+    if kind == 0:
+        mesh, V, kappa, f = p1_poisson_3d(n, seed=s)
+        return poisson_form(V, to_function(V, kappa))
+    mesh, V, mu, lmbda, f3 = vp2_elasticity_3d(n, seed=s)
+    Q = V.coefficient_space()
+    return elasticity_form(V, to_function(Q, mu), to_function(Q, lmbda))
+
+
+def batch_members(nmembers=256, seed=5, log=None):
+    """The batched workload's bilinear forms (batch_specs, batch_form)."""
     forms = []
-    for i, n in enumerate(sizes):
-        if i % 2 == 0:
-            mesh, V, kappa, f = p1_poisson_3d(int(n), seed=int(rng.integers(1 << 30)))
-            forms.append(poisson_form(V, to_function(V, kappa)))
-        else:
-            mesh, V, mu, lmbda, f3 = vp2_elasticity_3d(int(n), seed=int(rng.integers(1 << 30)))
-            Q = V.coefficient_space()
-            forms.append(elasticity_form(V, to_function(Q, mu), to_function(Q, lmbda)))
+    for i, spec in enumerate(batch_specs(nmembers, seed)):
+        forms.append(batch_form(spec))
         if log is not None and (i + 1) % 32 == 0:
             log(f"batch: {i + 1}/{nmembers} members built")
     return forms
+
+
+def batch_sample(specs, per_kind=6):
+    """A tuning sample of a batch: per element kind, per_kind members spread evenly over the
+    kind's sizes (smallest to largest)."""
+    out = []
+    for kind in (0, 1):
+        ks = sorted((s for s in specs if s[0] == kind), key=lambda s: s[1])
+        if not ks:
+            continue
+        idx = np.unique(np.linspace(0, len(ks) - 1, min(per_kind, len(ks))).round().astype(int))
+        out += [ks[i] for i in idx]
+    return out
 
 
 def vp2_node_coordinates(mesh):
