@@ -376,13 +376,16 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
       int32_t* drp = nullptr;
       int32_t* dcol = nullptr;
       bool col_on_host = true;
+      // vector P2 needs the columns on the host only when its CSR was not expanded from the
+      // node graph (the ranks are then read from the node CSR)
+      const bool want_col = element == DFA_ELEMENT_P1 ? (!device_plan || verify_plan) : false;
       if (!env_flag("DFA_HOST_CSR") &&
           dfa::build_csr_device(nc, nd, h->d_cdofs, ndofs, keep_diagonal != 0, csr, &drp, &dcol, s,
-                                !device_plan || verify_plan)) {
+                                want_col)) {
         h->nnz = csr.row_ptr.back();
         h->d_row_ptr = h->base.adopt(drp, static_cast<int64_t>(csr.row_ptr.size() * sizeof(int32_t)));
         h->d_col = h->base.adopt(dcol, h->nnz * static_cast<int64_t>(sizeof(int32_t)));
-        col_on_host = !device_plan || verify_plan;
+        col_on_host = want_col;
       } else {
         dfa::build_csr(nc, nd, h_cell_dofs, ndofs, keep_diagonal != 0, csr);
         h->nnz = static_cast<int64_t>(csr.col.size());
@@ -473,6 +476,7 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
         }
       } else {
         std::vector<uint8_t> rank;
+        if (!csr.node_layout) ensure_host_col();
         h->vp2_component_major = dfa::build_vp2_ranks(nc, h_cell_dofs, ndofs, csr, rank);
         timer.mark("vector-P2 ranks");
         h->d_vp2_rank = h->base.upload(rank, s);

@@ -39,6 +39,12 @@ struct UninitAlloc : std::allocator<T> {
 struct HostCSR {
   std::vector<int32_t> row_ptr;
   std::vector<int32_t, UninitAlloc<int32_t>> col;
+  // vector P2 built from its node graph: the node-level CSR (node n = dof of component 0 for
+  // the component-major layout, that dof / 3 for the node-major one) and the layout
+  // (0: not built that way, 1: component-major, 2: node-major)
+  int node_layout = 0;
+  std::vector<int32_t> node_row_ptr;
+  std::vector<int32_t, UninitAlloc<int32_t>> node_col;
 };
 
 // Sorted unique column set per row from cell_dofs x cell_dofs (+ optional diagonal):

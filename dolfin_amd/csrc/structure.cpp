@@ -884,6 +884,33 @@ namespace dfa {
 // component stride = deg) or node-major (block-interleaved numbering, s1 = 3, stride 1).
 bool build_vp2_ranks(int64_t nc, const int32_t* cd, int64_t ndofs, const HostCSR& csr,
                      std::vector<uint8_t>& rank) {
+  if (csr.node_layout) {
+    // the CSR was expanded from the node graph, whose layout was checked on every cell: the
+    // rank of node b in node a's row is its position in the node CSR row
+    const bool node_major = csr.node_layout == 2;
+    rank.assign(static_cast<size_t>(nc) * 100, 0);
+    std::atomic<bool> failed{false};
+    parallel_for(nc, [&](int64_t c0, int64_t c1, int) {
+      for (int64_t c = c0; c < c1; ++c) {
+        int32_t nd[10];
+        for (int a = 0; a < 10; ++a) nd[a] = node_major ? cd[c * 30 + a] / 3 : cd[c * 30 + a];
+        for (int a = 0; a < 10; ++a) {
+          const int32_t* b0 = csr.node_col.data() + csr.node_row_ptr[nd[a]];
+          const int32_t* e0 = csr.node_col.data() + csr.node_row_ptr[nd[a] + 1];
+          for (int b = 0; b < 10; ++b) {
+            const int32_t* q = std::lower_bound(b0, e0, nd[b]);
+            if (q == e0 || *q != nd[b] || q - b0 > 255) {
+              failed = true;
+              return;
+            }
+            rank[c * 100 + a * 10 + b] = static_cast<uint8_t>(q - b0);
+          }
+        }
+      }
+    });
+    if (failed) throw std::runtime_error("vector P2 node graph: column not found or row longer than 256 nodes");
+    return !node_major;
+  }
   // Node id of a dof: dof of component 0 at the same cell-local node.
   std::vector<int32_t> node0(ndofs, -1), comp(ndofs, -1);
   for (int64_t c = 0; c < nc; ++c)

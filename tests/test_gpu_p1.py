@@ -227,6 +227,31 @@ def test_csr_device_build_equals_host_vp2(monkeypatch):
     assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("layout", ["component_major", "node_major"])
+def test_csr_vp2_node_blocked_build(oracle, layout, monkeypatch):
+    # The vector-P2 device build sorts the node graph and expands every node entry to its
+    # 3 x 3 block when the dofs are node-blocked (component-major UFC numbering, or node-major
+    # after a node permutation). It must equal the oracle's sparsity and the general device
+    # build (forced by DFA_CSR_GENERAL, and taken by keep_diagonal, which adds nothing here:
+    # every vector-P2 dof row holds its diagonal).
+    rng = np.random.default_rng(7)
+    mesh = dfa.UnitCubeMesh(3, 2, 4)
+    perm = None if layout == "component_major" else rng.permutation(
+        mesh.num_vertices() + mesh.cell_edges()[1]).astype(np.int32)
+    V = dfa.VectorFunctionSpace(mesh, "P", 2, node_permutation=perm)
+    fast = dfa.fem.Engine(V)
+    diag = dfa.fem.Engine(V, keep_diagonal=True)
+    monkeypatch.setenv("DFA_CSR_GENERAL", "1")
+    general = dfa.fem.Engine(V)
+    a, b, c = fast.structure(), general.structure(), diag.structure()
+    torch.cuda.synchronize()
+    orp, ocol = oracle.sparsity(V.cell_dofs(), V.dim())
+    np.testing.assert_array_equal(a[0].cpu().numpy(), orp)
+    np.testing.assert_array_equal(a[1].cpu().numpy(), ocol)
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+    assert torch.equal(a[0], c[0]) and torch.equal(a[1], c[1])
+
+
 def test_blocked_equals_atomic_128_properties():
     """Full-size P1 128^3 (12.58M cells): size-independent properties -- blocked and atomic
     scatters agree, Poisson rows sum to zero (K 1 = 0), K is symmetric, sum(b) = int f."""
