@@ -87,6 +87,55 @@ int default_block_rows(int gdim) {
   return gdim == 3 ? 160 : 512;
 }
 
+// Reverse Cuthill-McKee on the device unless DFA_HOST_RCM=1 or no GPU is present.
+bool rcm_on_device() {
+  if (const char* e = std::getenv("DFA_HOST_RCM"))
+    if (std::atoi(e) != 0) return false;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return true;
+}
+
+// dfa_node_reordering on the device: cells -> device CSR (sort of the cell pairs) -> node graph
+// -> reverse Cuthill-McKee. False when the device build does not apply (the host's then runs).
+bool rcm_nodes_device(int64_t nc, int32_t npc, const int32_t* cells, int64_t n, int32_t* old_to_new) {
+  hipStream_t s = nullptr;
+  dfa::DeviceArena a;
+  int32_t *rp = nullptr, *col = nullptr, *ap = nullptr, *adj = nullptr;
+  auto cleanup = [&]() {
+    for (int32_t* p : {rp, col, ap, adj})
+      if (p) (void)hipFree(p);
+    a.release();
+  };
+  try {
+    const int32_t* dcells = a.upload(cells, static_cast<size_t>(nc) * npc, s);
+    dfa::HostCSR csr;
+    if (!dfa::build_csr_device(nc, npc, dcells, n, false, csr, &rp, &col, s, false)) {
+      cleanup();
+      return false;
+    }
+    int32_t maxd = 0;
+    dfa::node_graph_device(n, rp, col, &ap, &adj, maxd, s);
+    bool ok = false;
+    int32_t m = 0;
+    hip_check(hipMemcpy(&m, ap + n, sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (m == 0) {  // no edges: the identity, as the host build gives
+      for (int64_t v = 0; v < n; ++v) old_to_new[v] = static_cast<int32_t>(v);
+      ok = true;
+    } else {
+      ok = dfa::rcm_ordering_device(n, ap, adj, maxd, old_to_new, s);
+    }
+    cleanup();
+    return ok;
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1031,6 +1080,11 @@ int dfa_node_reordering(int64_t num_cells, int32_t nodes_per_cell, const int32_t
         (num_nodes > 0 && !old_to_new))
       return fail("invalid arguments", DFA_ERR_ARG);
     if (num_nodes > INT32_MAX) return fail("sizes exceed int32", DFA_ERR_ARG);
+    if (num_nodes == 0) return DFA_OK;
+    if (rcm_on_device() && num_cells > 0 && rcm_nodes_device(num_cells, nodes_per_cell, cell_nodes, num_nodes,
+                                                              old_to_new))
+      return DFA_OK;
+    if (env_flag("DFA_RCM_REQUIRE_DEVICE")) return fail("the device reordering did not apply", DFA_ERR_STATE);
     // node graph: the sparsity of cell_nodes x cell_nodes without the diagonal
     dfa::HostCSR g;
     dfa::build_csr(num_cells, nodes_per_cell, cell_nodes, num_nodes, false, g);
@@ -1062,6 +1116,24 @@ int dfa_graph_reordering(int64_t num_nodes, const int32_t* graph_ptr, const int3
           return fail("adjacency lists must be sorted, in range and without self loops", DFA_ERR_ARG);
       }
     }
+    if (rcm_on_device() && graph_ptr[num_nodes] > 0) {
+      int32_t maxd = 0;
+      for (int64_t v = 0; v < num_nodes; ++v) maxd = std::max(maxd, graph_ptr[v + 1] - graph_ptr[v]);
+      hipStream_t s = nullptr;
+      dfa::DeviceArena a;
+      try {
+        const int32_t* dap = a.upload(graph_ptr, static_cast<size_t>(num_nodes) + 1, s);
+        const int32_t* dadj = a.upload(graph_adj, static_cast<size_t>(graph_ptr[num_nodes]), s);
+        const bool ok = dfa::rcm_ordering_device(num_nodes, dap, dadj, maxd, old_to_new, s);
+        a.release();
+        if (ok) return DFA_OK;
+      } catch (...) {
+        a.release();
+        throw;
+      }
+    }
+    if (env_flag("DFA_RCM_REQUIRE_DEVICE") && graph_ptr[num_nodes] > 0)
+      return fail("the device reordering did not apply", DFA_ERR_STATE);
     dfa::rcm_ordering(num_nodes, graph_ptr, graph_adj, old_to_new);
     return DFA_OK;
   });

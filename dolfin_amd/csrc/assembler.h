@@ -56,6 +56,14 @@ struct DeviceArena {
 // the arrays stay on the device (returned through d_row_ptr / d_col; the caller frees them)
 // and are also copied into `out`. Returns false, with nothing allocated, when device memory
 // is short.
+// Reverse Cuthill-McKee numbering of a device graph (rcm_device.hip); false when the device
+// build does not apply and the host's rcm_ordering is to be used.
+bool rcm_ordering_device(int64_t n, const int32_t* d_ap, const int32_t* d_adj, int32_t max_degree,
+                         int32_t* old_to_new, hipStream_t s);
+// The node graph (adjacency without self loops) of a device CSR structure.
+void node_graph_device(int64_t n, const int32_t* rp, const int32_t* col, int32_t** ap, int32_t** adj,
+                       int32_t& max_degree, hipStream_t s);
+
 bool build_csr_device(int64_t num_cells, int32_t nd, const int32_t* d_cell_dofs,
                       int64_t num_rows, bool keep_diagonal, HostCSR& out, int32_t** d_row_ptr,
                       int32_t** d_col, hipStream_t stream, bool download_col = true);
