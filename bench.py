@@ -119,11 +119,8 @@ def bench_batch(nmembers, log, seed=5, settle_s=0.0):
     t_tune_plans = time.perf_counter() - t
     log(f"batch: plan sizes tuned ({t_tune_plans:.1f} s): P1 {p1_rows} rows, P2 {p2_vals} values")
     t = time.perf_counter()
-    autotune.set_plan_defaults(p1_rows, 0, p2_vals)
-    try:
-        batch = BatchAssembler(forms, grid_per_cu=0)   # builds every member's structure and plan
-    finally:
-        autotune.set_plan_defaults(0, 0, 0)
+    # builds every member's structure and plan (8 members at a time), then the launch
+    batch = BatchAssembler(forms, grid_per_cu=0, plan_defaults=(p1_rows, 0, p2_vals))
     torch.cuda.synchronize()
     t_plans = time.perf_counter() - t
     log(f"batch: structures and plans built ({t_plans:.1f} s)")

@@ -119,11 +119,8 @@ def tune_batch_plans(specs, make_form, p1_rows=(128, 160, 192), p2_vals=(4096, 4
         if not members:
             continue
         for c in cands:
-            set_plan_defaults(c if kind == 0 else 0, 0, c if kind == 1 else 0)
-            try:
-                b = BatchAssembler([make_form(sp) for sp in members])
-            finally:
-                set_plan_defaults(0, 0, 0)
+            b = BatchAssembler([make_form(sp) for sp in members],
+                               plan_defaults=(c if kind == 0 else 0, 0, c if kind == 1 else 0))
             times[key][c] = _time(b.assemble, reps)
             del b
     best = tuple(min(times[k], key=times[k].get) if times[k] else 0 for k in ("p1_rows", "p2_vals"))

@@ -5,9 +5,11 @@ elasticity) with its own function space, coefficients and CSR tensor. The librar
 unit tables over all members' row blocks and captures two kernels (vector-P2 members, then
 P1 members, each at its own occupancy) in one hipGraph; assemble() replays it. grid_per_cu=0
 launches one workgroup per row block, grid_per_cu>0 that many persistent workgroups per CU.
+Members without a structure and plan yet get them first, several at a time (prepare_engines).
 Coefficient vectors may be updated in place between calls.
 """
 import ctypes
+from concurrent.futures import ThreadPoolExecutor
 
 import torch
 
@@ -16,12 +18,38 @@ from .fem import engine_for
 from .la import Matrix
 
 
+def prepare_engines(forms, plan_defaults=(0, 0, 0), workers=8):
+    """Builds every member's structure and plan (engine_for), `workers` members at a time: the
+    host parts of the builds (vector-P2 rank tables and block plans) run concurrently, the
+    device parts queue on the default stream. plan_defaults (P1 rows per block, P1 threads,
+    vector-P2 values per block; 0 = library default) apply to the builds, in each worker."""
+    L = _lib.load()
+
+    def one(a):
+        _lib.check(L.dfa_set_plan_defaults(*(int(v) for v in plan_defaults)))
+        try:
+            engine_for(a.V)
+        finally:
+            L.dfa_set_plan_defaults(0, 0, 0)
+
+    pending = [a for a in forms if not a.V._engines.get(False)]
+    if workers <= 1 or len(pending) <= 1:
+        for a in pending:
+            one(a)
+        return
+    torch.cuda.synchronize()
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(one, pending))
+    torch.cuda.synchronize()
+
+
 class BatchAssembler:
-    def __init__(self, forms, grid_per_cu=0):
+    def __init__(self, forms, grid_per_cu=0, plan_defaults=(0, 0, 0), workers=8):
         if not forms:
             raise ValueError("empty batch")
         L = _lib.load()
         self.forms = list(forms)
+        prepare_engines(self.forms, plan_defaults, workers)
         self.tensors = []
         engines, kinds, xs, c0s, c1s, vals = [], [], [], [], [], []
         self._keep = []
