@@ -35,6 +35,14 @@
  *   orc_adjoint_*       closed-form derivative of J = u^T K u - F^T u; pinned by central
  *                       finite differences of orc_assemble_* (tests) -- the FD check is
  *                       independent of the derivative formula.
+ *   orc_vp2_dofmap      the vector-P2 cell dofs as component sub-dofmaps concatenated (pinned
+ *                       by python/test/unit/fem/test_dofmap.py:112-120); the global order of
+ *                       the nodes (vertices, then edges by key) is an input convention, parity
+ *                       unpinned: FFC/UFC's own node order is in neither the reference nor
+ *                       this image, and the engine takes whatever dofmap DOLFIN passes.
+ * The pure-Python reverse Cuthill-McKee of oracle.py is pinned by the orderings Boost's
+ * documentation prints for its example graph; it refuses graphs where a vertex queues more
+ * than 16 new neighbours at once (Boost's order of equal degrees there: parity unpinned).
  */
 #ifndef DOLFIN_AMD_ORACLE_H
 #define DOLFIN_AMD_ORACLE_H
