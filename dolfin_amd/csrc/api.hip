@@ -1081,6 +1081,8 @@ int dfa_node_reordering(int64_t num_cells, int32_t nodes_per_cell, const int32_t
       return fail("invalid arguments", DFA_ERR_ARG);
     if (num_nodes > INT32_MAX) return fail("sizes exceed int32", DFA_ERR_ARG);
     if (num_nodes == 0) return DFA_OK;
+    for (int64_t i = 0; i < num_cells * nodes_per_cell; ++i)
+      if (cell_nodes[i] < 0 || cell_nodes[i] >= num_nodes) return fail("cell node out of range", DFA_ERR_ARG);
     if (rcm_on_device() && num_cells > 0 && rcm_nodes_device(num_cells, nodes_per_cell, cell_nodes, num_nodes,
                                                               old_to_new))
       return DFA_OK;
