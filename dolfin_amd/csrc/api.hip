@@ -649,13 +649,16 @@ static void ensure_colors(dfa_assembler h, hipStream_t s) {
   for (int64_t r = 0; r < h->ndofs; ++r)
     if (rp[r + 1] - rp[r] > 256)
       throw std::runtime_error("coloured scatter: a CSR row holds more than 256 entries");
-  std::vector<int32_t> color(static_cast<size_t>(h->nc));
-  const int32_t ncol = dfa::greedy_cell_coloring(h->nc, nvc, cd.data(), h->ndofs, 1, color.data());
-  std::vector<int64_t> ptr(static_cast<size_t>(ncol) + 1, 0);
-  for (int32_t c : color) ++ptr[c + 1];
-  for (int32_t k = 0; k < ncol; ++k) ptr[k + 1] += ptr[k];
-  std::vector<int32_t> perm(static_cast<size_t>(h->nc));
-  {
+  // colours on the device (Jones-Plassmann), else DOLFIN's sequential greedy on the host
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> perm;
+  if (env_flag("DFA_HOST_COLOR") || !dfa::color_cells_device(h->nc, nvc, h->d_cdofs, h->ndofs, perm, ptr, s)) {
+    std::vector<int32_t> color(static_cast<size_t>(h->nc));
+    const int32_t ncol = dfa::greedy_cell_coloring(h->nc, nvc, cd.data(), h->ndofs, 1, color.data());
+    ptr.assign(static_cast<size_t>(ncol) + 1, 0);
+    for (int32_t c : color) ++ptr[c + 1];
+    for (int32_t k = 0; k < ncol; ++k) ptr[k + 1] += ptr[k];
+    perm.assign(static_cast<size_t>(h->nc), 0);
     std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
     for (int64_t c = 0; c < h->nc; ++c) perm[fill[color[c]]++] = static_cast<int32_t>(c);
   }

@@ -60,6 +60,11 @@ struct DeviceArena {
 // build does not apply and the host's rcm_ordering is to be used.
 bool rcm_ordering_device(int64_t n, const int32_t* d_ap, const int32_t* d_adj, int32_t max_degree,
                          int32_t* old_to_new, hipStream_t s);
+// Colouring of the cells for the graph-coloured scatter (cells sharing a dof differ), on the
+// device (color_device.hip): the cells in colour order and the colour offsets; false when it
+// does not apply (the host greedy colouring is then used).
+bool color_cells_device(int64_t nc, int32_t nvc, const int32_t* d_cdofs, int64_t ndofs,
+                        std::vector<int32_t>& perm, std::vector<int64_t>& color_ptr, hipStream_t s);
 // The node graph (adjacency without self loops) of a device CSR structure.
 void node_graph_device(int64_t n, const int32_t* rp, const int32_t* col, int32_t** ap, int32_t** adj,
                        int32_t& max_degree, hipStream_t s);
