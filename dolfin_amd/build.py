@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["api.hip", "p1_kernels.hip", "p2_kernels.hip", "adjoint_kernels.hip",
-           "batch_kernels.hip", "facet_kernels.hip", "la_kernels.hip", "csr_build.hip", "plan_device.hip", "rcm_device.hip", "color_device.hip",
+           "batch_kernels.hip", "facet_kernels.hip", "la_kernels.hip", "csr_build.hip", "plan_device.hip", "rcm_device.hip", "color_device.hip", "plan_vp2_device.hip",
            "structure.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

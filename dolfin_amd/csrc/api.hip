@@ -155,6 +155,7 @@ int dfa_set_plan_defaults(int32_t p1_block_rows, int32_t p1_threads, int32_t p2_
   return DFA_OK;
 }
 
+extern "C++" {  // internal helpers keep C++ linkage inside the C block
 namespace {
 // Phase times of dfa_assembler_create on stderr when DFA_TIMINGS=1 (the role of DOLFIN's
 // "Build sparsity" / "Init tensor" timers, dolfin/common/Timer.h).
@@ -284,32 +285,39 @@ void upload_host_p1_plan(dfa_assembler_s* h, const HostP1Plan& hp, hipStream_t s
   hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");  // host vectors die after this
 }
 
+// A device array against its host build ("" = equal, else the first differing element).
+template <class T, class V>
+std::string same_array(const char* name, const T* d, const V& hv) {
+  const size_t n = hv.size();
+  if (n && !d) return std::string(name) + ": missing on the device";
+  std::vector<std::remove_cv_t<T>> dv(n);
+  if (n) hip_check(hipMemcpy(dv.data(), d, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy");
+  if (n && std::memcmp(dv.data(), hv.data(), n * sizeof(T)) != 0) {
+    size_t i = 0;
+    while (std::memcmp(&dv[i], &hv[i], sizeof(T)) == 0) ++i;
+    return std::string(name) + ": first difference at element " + std::to_string(i) + " of " +
+           std::to_string(n);
+  }
+  return "";
+}
+
+// Outcome of two plan builds when at least one failed ("" = both failed alike).
+std::string compare_plan_failures(bool host_ok, const std::string& herr, bool dev_failed,
+                                  const std::string& derr) {
+  if (host_ok != !dev_failed)
+    return std::string("device plan ") + (dev_failed ? "failed (" + derr + ")" : "built") +
+           ", host plan " + (host_ok ? "built" : "failed (" + herr + ")");
+  return herr == derr ? "" : "different errors: device '" + derr + "', host '" + herr + "'";
+}
+
 // Element-by-element comparison of the device-built plan with the host build ("" = equal).
 std::string compare_p1_plans(dfa_assembler_s* h, const HostP1Plan& hp, bool host_ok,
                              const std::string& herr, bool dev_failed, const std::string& derr,
                              hipStream_t s) {
-  if (!host_ok || dev_failed) {
-    if (host_ok != !dev_failed)
-      return std::string("device plan ") + (dev_failed ? "failed (" + derr + ")" : "built") +
-             ", host plan " + (host_ok ? "built" : "failed (" + herr + ")");
-    return herr == derr ? "" : "different errors: device '" + derr + "', host '" + herr + "'";
-  }
+  if (!host_ok || dev_failed) return compare_plan_failures(host_ok, herr, dev_failed, derr);
   const auto& P = hp.P;
   const auto& t = h->btopo;
-  auto same = [&](const char* name, const auto* d, const auto& hv) -> std::string {
-    using T = std::remove_cv_t<std::remove_pointer_t<decltype(d)>>;
-    const size_t n = hv.size();
-    if (n && !d) return std::string(name) + ": missing on the device";
-    std::vector<T> dv(n);
-    if (n) hip_check(hipMemcpy(dv.data(), d, n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy");
-    if (n && std::memcmp(dv.data(), hv.data(), n * sizeof(T)) != 0) {
-      size_t i = 0;
-      while (std::memcmp(&dv[i], &hv[i], sizeof(T)) == 0) ++i;
-      return std::string(name) + ": first difference at element " + std::to_string(i) + " of " +
-             std::to_string(n);
-    }
-    return "";
-  };
+  auto same = [](const char* name, const auto* d, const auto& hv) { return same_array(name, d, hv); };
   (void)s;
   std::string r;
   auto chk = [&](std::string x) {
@@ -364,7 +372,42 @@ std::string compare_p1_plans(dfa_assembler_s* h, const HostP1Plan& hp, bool host
   }
   return r;
 }
+
+// The same for the vector-P2 plan and rank table.
+std::string compare_vp2_plans(dfa_assembler_s* h, const dfa::VP2BlockPlan& P, const std::vector<uint8_t>& rank,
+                              bool host_ok, const std::string& herr, bool dev_failed, const std::string& derr) {
+  std::string r = same_array("rank", h->d_vp2_rank, rank);
+  if (!r.empty()) return r;
+  if (!host_ok || dev_failed) return compare_plan_failures(host_ok, herr, dev_failed, derr);
+  const auto& t = h->vtopo;
+  auto chk = [&](std::string x) {
+    if (r.empty() && !x.empty()) r = x;
+  };
+  auto scalar = [&](const char* name, int64_t a, int64_t b) {
+    if (a != b) chk(std::string(name) + ": device " + std::to_string(a) + ", host " + std::to_string(b));
+  };
+  scalar("num_blocks", t.num_blocks, P.num_blocks);
+  scalar("max_verts", t.max_verts, P.max_verts);
+  scalar("max_owned", t.max_owned, P.max_owned);
+  scalar("max_vals", t.max_vals, P.max_vals);
+  scalar("component_major", t.component_major, P.component_major);
+  scalar("appearances", h->appearances, static_cast<int64_t>(P.task_cell.size()));
+  if (!r.empty()) return r;
+  chk(same_array("vert_ptr", t.vert_ptr, P.vert_ptr));
+  chk(same_array("vert_id", t.vert_id, P.vert_id));
+  chk(same_array("vert_kdof", t.vert_kdof, P.vert_kdof));
+  chk(same_array("own_ptr", t.own_ptr, P.own_ptr));
+  chk(same_array("own_rows", t.own_rows, P.own_rows));
+  chk(same_array("own_info", t.own_info, P.own_info));
+  chk(same_array("task_ptr", t.task_ptr, P.task_ptr));
+  chk(same_array("task", t.task, P.task));
+  chk(same_array("task_cell", t.task_cell, P.task_cell));
+  chk(same_array("task_nvert", t.task_nvert, P.task_nvert));
+  chk(same_array("blk_vals", t.blk_vals, P.blk_vals));
+  return r;
+}
 }  // namespace
+}  // extern "C++"
 
 int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t nc,
                          const int32_t* h_cells, int32_t element, int32_t nd,
@@ -524,44 +567,116 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
           h->plan_error = "no coordinates given at creation";
         }
       } else {
+        // rank table and block plan on the device when the CSR was expanded from the node
+        // graph, else (or with DFA_HOST_PLAN=1) on the host; DFA_PLAN_VERIFY=1 builds both
         std::vector<uint8_t> rank;
-        if (!csr.node_layout) ensure_host_col();
-        h->vp2_component_major = dfa::build_vp2_ranks(nc, h_cell_dofs, ndofs, csr, rank);
-        timer.mark("vector-P2 ranks");
-        h->d_vp2_rank = h->base.upload(rank, s);
-        if (h_x) {
-          try {
-            dfa::VP2BlockPlan P;
-            dfa::build_vp2_block_plan(nc, h_cells, h_cell_dofs, h_coeff_dofs, h_x, csr, rank,
-                                      h->vp2_component_major, default_p2_block_vals(), P);
-            timer.mark("vector-P2 block plan");
-            auto& t = h->vtopo;
-            t.num_blocks = P.num_blocks;
-            t.max_verts = P.max_verts;
-            t.max_owned = P.max_owned;
-            t.max_vals = P.max_vals;
-            t.component_major = P.component_major;
-            t.vert_ptr = h->plan.upload(P.vert_ptr, s);
-            t.vert_id = h->plan.upload(P.vert_id, s);
-            t.vert_kdof = h->plan.upload(P.vert_kdof, s);
-            t.own_ptr = h->plan.upload(P.own_ptr, s);
-            t.own_rows = h->plan.upload(P.own_rows, s);
-            t.own_info = h->plan.upload(P.own_info, s);
-            t.task_ptr = h->plan.upload(P.task_ptr, s);
-            t.task = h->plan.upload(P.task, s);
-            t.task_nvert = h->plan.upload(P.task_nvert, s);
-            t.task_cell = h->plan.upload(P.task_cell, s);
-            t.blk_vals = h->plan.upload(P.blk_vals, s);
-            t.row_ptr = h->d_row_ptr;
-            t.cell_dofs = h->d_cdofs;
-            h->appearances = static_cast<int64_t>(P.task_cell.size());
-            h->has_vp2_plan = true;
+        bool ranks_on_device = false;
+        if (csr.node_layout && !env_flag("DFA_HOST_PLAN")) {
+          void* dr = nullptr;
+          hip_check(hipMalloc(&dr, std::max<size_t>(static_cast<size_t>(nc) * 100, 16)), "hipMalloc");
+          h->d_vp2_rank = h->base.adopt(static_cast<uint8_t*>(dr), nc * 100);
+          ranks_on_device = dfa::vp2_ranks_device(nc, h->d_cdofs, h->d_row_ptr, h->d_col, csr.node_layout == 2,
+                                                  h->d_vp2_rank, s);
+          if (ranks_on_device) {
+            h->vp2_component_major = csr.node_layout == 1;
+            timer.mark("vector-P2 ranks (device)");
+          }
+        }
+        auto host_ranks = [&]() {
+          if (!csr.node_layout) ensure_host_col();
+          const bool cm = dfa::build_vp2_ranks(nc, h_cell_dofs, ndofs, csr, rank);
+          timer.mark("vector-P2 ranks");
+          return cm;
+        };
+        if (!ranks_on_device) {
+          h->vp2_component_major = host_ranks();
+          if (h->d_vp2_rank) {
+            hip_check(hipMemcpyAsync(h->d_vp2_rank, rank.data(), rank.size(), hipMemcpyHostToDevice, s),
+                      "hipMemcpyAsync");
             hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-          } catch (const dfa::HipError&) {
-            throw;
-          } catch (const std::exception& e) {
-            h->plan.release();
-            h->plan_error = e.what();
+          } else {
+            h->d_vp2_rank = h->base.upload(rank, s);
+          }
+        }
+        if (h_x) {
+          int rc = dfa::kDevicePlanUnsupported;
+          std::string derr;
+          if (ranks_on_device) {
+            dfa::DeviceVP2Input in;
+            in.nv = nv;
+            in.nc = nc;
+            in.ndofs = ndofs;
+            in.node_layout = csr.node_layout;
+            in.d_cells = h->d_cells;
+            in.d_cdofs = h->d_cdofs;
+            in.d_kdofs = h->d_kdofs;
+            in.h_x = h_x;
+            in.d_row_ptr = h->d_row_ptr;
+            in.d_rank = h->d_vp2_rank;
+            dfa::VP2BlockTopo t;
+            int64_t app = 0;
+            rc = dfa::build_vp2_plan_device(in, default_p2_block_vals(), h->plan, t, app, derr, s);
+            if (rc == dfa::kDevicePlanOk) {
+              t.row_ptr = h->d_row_ptr;
+              t.cell_dofs = h->d_cdofs;
+              h->vtopo = t;
+              h->appearances = app;
+              h->has_vp2_plan = true;
+              h->plan_on_device = true;
+              timer.mark("vector-P2 block plan (device)");
+            } else {
+              h->plan.release();
+              if (rc == dfa::kDevicePlanError) h->plan_error = derr;
+            }
+          }
+          if (rc == dfa::kDevicePlanUnsupported || verify_plan) {
+            if (rank.empty()) {
+              const bool cm = host_ranks();
+              if (cm != h->vp2_component_major) throw std::runtime_error("vector P2 layout differs");
+            }
+            dfa::VP2BlockPlan P;
+            bool ok = true;
+            std::string herr;
+            try {
+              dfa::build_vp2_block_plan(nc, h_cells, h_cell_dofs, h_coeff_dofs, h_x, csr, rank,
+                                        h->vp2_component_major, default_p2_block_vals(), P);
+            } catch (const dfa::HipError&) {
+              throw;
+            } catch (const std::exception& e) {
+              ok = false;
+              herr = e.what();
+            }
+            timer.mark("vector-P2 block plan");
+            if (rc != dfa::kDevicePlanUnsupported) {
+              h->plan_verify_error = compare_vp2_plans(h, P, rank, ok, herr, rc == dfa::kDevicePlanError, derr);
+              if (!h->plan_verify_error.empty())
+                std::fprintf(stderr, "[dfa] plan verification: %s\n", h->plan_verify_error.c_str());
+            } else if (!ok) {
+              h->plan_error = herr;
+            } else {
+              auto& t = h->vtopo;
+              t.num_blocks = P.num_blocks;
+              t.max_verts = P.max_verts;
+              t.max_owned = P.max_owned;
+              t.max_vals = P.max_vals;
+              t.component_major = P.component_major;
+              t.vert_ptr = h->plan.upload(P.vert_ptr, s);
+              t.vert_id = h->plan.upload(P.vert_id, s);
+              t.vert_kdof = h->plan.upload(P.vert_kdof, s);
+              t.own_ptr = h->plan.upload(P.own_ptr, s);
+              t.own_rows = h->plan.upload(P.own_rows, s);
+              t.own_info = h->plan.upload(P.own_info, s);
+              t.task_ptr = h->plan.upload(P.task_ptr, s);
+              t.task = h->plan.upload(P.task, s);
+              t.task_nvert = h->plan.upload(P.task_nvert, s);
+              t.task_cell = h->plan.upload(P.task_cell, s);
+              t.blk_vals = h->plan.upload(P.blk_vals, s);
+              t.row_ptr = h->d_row_ptr;
+              t.cell_dofs = h->d_cdofs;
+              h->appearances = static_cast<int64_t>(P.task_cell.size());
+              h->has_vp2_plan = true;
+              hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");  // host vectors die after this
+            }
           }
         }
       }
@@ -974,7 +1089,7 @@ int dfa_assemble_system_x0_noalloc(dfa_assembler h, int32_t a_form, int32_t L_fo
 int dfa_plan_builder(dfa_assembler h, int32_t* on_device) {
   return guarded([&]() -> int {
     if (!h || !on_device) return fail("NULL argument", DFA_ERR_ARG);
-    *on_device = h->has_plan && h->plan_on_device ? 1 : 0;
+    *on_device = (h->has_plan || h->has_vp2_plan) && h->plan_on_device ? 1 : 0;
     if (!h->plan_verify_error.empty())
       return fail("device plan differs from the host plan: " + h->plan_verify_error, DFA_ERR_STATE);
     return DFA_OK;

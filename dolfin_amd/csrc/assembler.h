@@ -100,6 +100,25 @@ int build_p1_plan_device(const DevicePlanInput& in, int32_t target_rows, bool wa
                          int32_t block_threads, int32_t chain_threads, DeviceArena& plan,
                          P1BlockTopo& t, DevicePlanStats& stats, std::string& err, hipStream_t s);
 
+// Vector P2 on the device (plan_vp2_device.hip): the rank table of build_vp2_ranks (false when
+// a column is missing or a row has more than 256 nodes) and the arrays of build_vp2_block_plan,
+// for CSR structures expanded from the node graph (node_layout 1 = component-major,
+// 2 = node-major).
+bool vp2_ranks_device(int64_t nc, const int32_t* d_cdofs, const int32_t* d_row_ptr, const int32_t* d_col,
+                      bool node_major, uint8_t* d_rank, hipStream_t s);
+struct DeviceVP2Input {
+  int64_t nv = 0, nc = 0, ndofs = 0;
+  int node_layout = 0;
+  const int32_t* d_cells = nullptr;
+  const int32_t* d_cdofs = nullptr;
+  const int32_t* d_kdofs = nullptr;
+  const double* h_x = nullptr;  // host coordinates (uploaded for the partition)
+  const int32_t* d_row_ptr = nullptr;
+  const uint8_t* d_rank = nullptr;
+};
+int build_vp2_plan_device(const DeviceVP2Input& in, int32_t max_block_vals, DeviceArena& plan, VP2BlockTopo& t,
+                          int64_t& appearances, std::string& err, hipStream_t s);
+
 }  // namespace dfa
 
 struct dfa_assembler_s {

@@ -84,8 +84,9 @@ class Engine:
         return buf.value.decode()
 
     def plan_builder(self):
-        """"device" when the P1 block plan was built by the device planner, "host" otherwise;
-        raises when a DFA_PLAN_VERIFY=1 creation found the two builds different."""
+        """"device" when the block plan (P1, or vector P2) was built by the device planner,
+        "host" otherwise; raises when a DFA_PLAN_VERIFY=1 creation found the two builds
+        different."""
         v = ctypes.c_int32()
         _lib.check(_lib.load().dfa_plan_builder(self._h, ctypes.byref(v)))
         return "device" if v.value else "host"

@@ -179,8 +179,9 @@ int dfa_dirichlet_x0(int64_t n, const uint8_t* d_bc_mask, const double* d_bc_val
 int dfa_pass_kernel(dfa_assembler h, int32_t a_form, int32_t L_form, int32_t has_bc,
                     int32_t scatter, char* name, int32_t len);
 
-/* Introspection (no counterpart in the reference): where the handle's P1 block plan was
-   built, *on_device = 1 (the device planner) or 0 (the host builder, or no plan). With
+/* Introspection (no counterpart in the reference): where the handle's block plan (P1, or
+   vector P2 with its rank table) was built, *on_device = 1 (the device planner) or 0 (the
+   host builder, or no plan). With
    DFA_PLAN_VERIFY=1 in the environment at creation the plan is built both ways and compared
    element by element; a difference makes this call fail (DFA_ERR_STATE) with the first
    differing array named in dfa_last_error(). */
