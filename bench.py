@@ -342,6 +342,8 @@ def main():
             sec["p1_spmv"] = {"ms": mv_ms, "hbm_achieved_GBs": mv_bytes / (mv_ms * 1e-3) / 1e9,
                               "hbm_frac": mv_bytes / (mv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                               "algorithmic_bytes": mv_bytes,
+                              "kernel": "spmv_stream_kernel, 16-bit column offsets per row block",
+                              "format_bytes": 10 * eng.nnz + 4 * (V.dim() + 1) + 16 * V.dim(),
                               "traffic": load_traffic(f"p1_spmv_{args.n}"),
                               "workload": f"y = A x on the assembled P1 {args.n}^3 matrix"}
             log("spmv timed")
@@ -453,6 +455,9 @@ def main():
                                "hbm_achieved_GBs": mv2_bytes / (mv2_ms * 1e-3) / 1e9,
                                "hbm_frac": mv2_bytes / (mv2_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                "algorithmic_bytes": mv2_bytes,
+                               "kernel": "spmv_node_kernel, one node column per 3 x 3 block",
+                               "format_bytes": 8 * e2.nnz + 4 * (e2.nnz // 9) + 4 * (V2.dim() // 3 + 1)
+                                               + 16 * V2.dim(),
                                "traffic": load_traffic(f"vp2_spmv_{args.p2_n}"),
                                "workload": f"y = A x on the assembled vector-P2 {args.p2_n}^3 matrix"}
         if args.batch > 0 and want("batch"):

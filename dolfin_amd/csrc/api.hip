@@ -467,17 +467,27 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
       dfa::HostCSR csr;
       int32_t* drp = nullptr;
       int32_t* dcol = nullptr;
+      int32_t* dnrp = nullptr;
+      int32_t* dncol = nullptr;
+      const bool vp2 = element == DFA_ELEMENT_VECTOR_P2;
       bool col_on_host = true;
       // vector P2 needs the columns on the host only when its CSR was not expanded from the
       // node graph (the ranks are then read from the node CSR)
       const bool want_col = element == DFA_ELEMENT_P1 ? (!device_plan || verify_plan) : false;
       if (!env_flag("DFA_HOST_CSR") &&
           dfa::build_csr_device(nc, nd, h->d_cdofs, ndofs, keep_diagonal != 0, csr, &drp, &dcol, s,
-                                want_col)) {
+                                want_col, vp2 ? &dnrp : nullptr, vp2 ? &dncol : nullptr)) {
         h->nnz = csr.row_ptr.back();
         h->d_row_ptr = h->base.adopt(drp, static_cast<int64_t>(csr.row_ptr.size() * sizeof(int32_t)));
         h->d_col = h->base.adopt(dcol, h->nnz * static_cast<int64_t>(sizeof(int32_t)));
         col_on_host = want_col;
+        if (csr.node_layout) {  // vector P2 expanded from the node graph, kept on the device
+          const int64_t nn = ndofs / 3;
+          h->node_layout = csr.node_layout;
+          h->node_nnz = h->nnz / 9;
+          h->d_node_rp = h->base.adopt(dnrp, (nn + 1) * static_cast<int64_t>(sizeof(int32_t)));
+          h->d_node_col = h->base.adopt(dncol, h->node_nnz * static_cast<int64_t>(sizeof(int32_t)));
+        }
       } else {
         dfa::build_csr(nc, nd, h_cell_dofs, ndofs, keep_diagonal != 0, csr);
         h->nnz = static_cast<int64_t>(csr.col.size());
@@ -504,6 +514,31 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
         }
         h->spmv_nblocks = static_cast<int64_t>(sb.size()) - 1;
         h->d_spmv_blocks = h->base.upload(pairs, s);
+        if (h->node_layout) {
+          const std::vector<int32_t> nc2 =
+              dfa::build_node_chunks(csr.row_ptr, h->node_layout, dfa::kNodeBudget, dfa::kNodeMaxNodes);
+          if (!nc2.empty()) {
+            h->node_nchunks = static_cast<int64_t>(nc2.size() / 2) - 1;
+            h->d_node_chunks = h->base.upload(nc2, s);
+          }
+        }
+        if (!h->d_node_chunks && h->spmv_nblocks > 0) {  // 16-bit column offsets per row block
+          void* p0 = nullptr;
+          void* p1 = nullptr;
+          hip_check(hipMalloc(&p0, static_cast<size_t>(h->spmv_nblocks) * sizeof(int32_t)), "hipMalloc");
+          h->d_blk_col0 = h->base.adopt(static_cast<int32_t*>(p0), h->spmv_nblocks * 4);
+          hip_check(hipMalloc(&p1, std::max<size_t>(static_cast<size_t>(h->nnz) * sizeof(uint16_t), 16)),
+                    "hipMalloc");
+          h->d_col16 = h->base.adopt(static_cast<uint16_t*>(p1), h->nnz * 2);
+          dfa::CsrView A;
+          A.nrows = ndofs;
+          A.nnz = h->nnz;
+          A.col = h->d_col;
+          A.blocks = h->d_spmv_blocks;
+          A.nblocks = h->spmv_nblocks;
+          hip_check(dfa::launch_narrow_columns(A, h->d_blk_col0, h->d_col16, s), "narrow_columns_kernel");
+        }
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");  // host vectors die here
       }
       if (element == DFA_ELEMENT_P1) {
         if (h_x) {
@@ -584,6 +619,17 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
         }
         auto host_ranks = [&]() {
           if (!csr.node_layout) ensure_host_col();
+          if (csr.node_layout && csr.node_row_ptr.empty()) {  // the node graph, copied on demand
+            const int64_t nn = ndofs / 3;
+            csr.node_row_ptr.resize(static_cast<size_t>(nn) + 1);
+            csr.node_col.resize(static_cast<size_t>(h->node_nnz));
+            hip_check(hipMemcpyAsync(csr.node_row_ptr.data(), h->d_node_rp, (nn + 1) * sizeof(int32_t),
+                                     hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+            if (h->node_nnz)
+              hip_check(hipMemcpyAsync(csr.node_col.data(), h->d_node_col, h->node_nnz * sizeof(int32_t),
+                                       hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+            hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+          }
           const bool cm = dfa::build_vp2_ranks(nc, h_cell_dofs, ndofs, csr, rank);
           timer.mark("vector-P2 ranks");
           return cm;
@@ -1333,6 +1379,18 @@ int dfa_csr_spmv(dfa_assembler h, const double* d_values, double alpha, const do
     A.values = d_values;
     A.blocks = h->d_spmv_blocks;
     A.nblocks = h->spmv_nblocks;
+    if (!std::getenv("DFA_SPMV_CSR")) {
+      A.blk_col0 = h->d_blk_col0;
+      A.col16 = h->d_col16;
+    }
+    if (h->d_node_chunks && !std::getenv("DFA_SPMV_CSR")) {
+      A.node_layout = h->node_layout;
+      A.node_nnz = h->node_nnz;
+      A.node_rp = h->d_node_rp;
+      A.node_col = h->d_node_col;
+      A.node_chunks = h->d_node_chunks;
+      A.node_nchunks = h->node_nchunks;
+    }
     hip_check(dfa::launch_spmv(A, alpha, d_x, beta, d_y, static_cast<hipStream_t>(stream)),
               "spmv_kernel");
     return DFA_OK;

@@ -52,10 +52,6 @@ struct DeviceArena {
   }
 };
 
-// The CSR structure of build_csr, built on the device from device cell_dofs (csr_build.hip):
-// the arrays stay on the device (returned through d_row_ptr / d_col; the caller frees them)
-// and are also copied into `out`. Returns false, with nothing allocated, when device memory
-// is short.
 // Reverse Cuthill-McKee numbering of a device graph (rcm_device.hip); false when the device
 // build does not apply and the host's rcm_ordering is to be used.
 bool rcm_ordering_device(int64_t n, const int32_t* d_ap, const int32_t* d_adj, int32_t max_degree,
@@ -69,9 +65,16 @@ bool color_cells_device(int64_t nc, int32_t nvc, const int32_t* d_cdofs, int64_t
 void node_graph_device(int64_t n, const int32_t* rp, const int32_t* col, int32_t** ap, int32_t** adj,
                        int32_t& max_degree, hipStream_t s);
 
+// The CSR structure of build_csr, built on the device from device cell_dofs (csr_build.hip):
+// the arrays stay on the device (returned through d_row_ptr / d_col; the caller frees them)
+// and are also copied into `out`. Returns false, with nothing allocated, when device memory
+// is short. For a vector-P2 CSR expanded from the node graph (out.node_layout != 0), the node graph is
+// returned through d_node_rp / d_node_col when both are given (the caller frees them; the host
+// copy in `out` is then left empty), else copied into out.node_row_ptr / node_col.
 bool build_csr_device(int64_t num_cells, int32_t nd, const int32_t* d_cell_dofs,
                       int64_t num_rows, bool keep_diagonal, HostCSR& out, int32_t** d_row_ptr,
-                      int32_t** d_col, hipStream_t stream, bool download_col = true);
+                      int32_t** d_col, hipStream_t stream, bool download_col = true,
+                      int32_t** d_node_rp = nullptr, int32_t** d_node_col = nullptr);
 
 // Device build of the P1 block plan and edge chains (plan_device.hip): the same arrays as
 // build_p1_block_plan + build_p1_chain_plan + the slot arrays of the edge-chain kernels,
@@ -145,6 +148,18 @@ struct dfa_assembler_s {
   // vector P2: per cell, rank of node b among the neighbours of node a (100 bytes) and the
   // CSR strides (node stride s1, component stride s2 per row)
   uint8_t* d_vp2_rank = nullptr;
+  // vector P2 expanded from the node graph: the node CSR (nodes = component-0 dofs, or dof / 3
+  // when node-major), layout 1 = component-major, 2 = node-major; 0 = no node structure
+  int32_t node_layout = 0;
+  int64_t node_nnz = 0;
+  int32_t* d_node_rp = nullptr;
+  int32_t* d_node_col = nullptr;
+  int32_t* d_node_chunks = nullptr;  // node-blocked SpMV chunks (none: the streamed CSR SpMV)
+  int64_t node_nchunks = 0;
+  // streamed CSR SpMV: per row block the smallest column (-1: 4-byte columns) and the 16-bit
+  // column offsets
+  int32_t* d_blk_col0 = nullptr;
+  uint16_t* d_col16 = nullptr;
   int32_t vp2_s1 = 0;
   bool vp2_component_major = true;
   bool has_vp2_plan = false;

@@ -132,6 +132,12 @@ void cuthill_mckee_from(int64_t n, const int32_t* adj_ptr, const int32_t* adj, i
 // own. Returns the first row of each block followed by the row count.
 std::vector<int32_t> build_spmv_blocks(const std::vector<int32_t>& row_ptr, int32_t budget,
                                        int32_t max_rows);
+// Node chunks of the node-blocked SpMV (vector P2 expanded from the node graph, node_layout 1
+// or 2): (first node, first node entry) pairs of consecutive nodes with at most `budget` node
+// entries and `max_nodes` nodes each, closed by (nodes, node entries); empty when one node
+// alone has more than `budget` entries.
+std::vector<int32_t> build_node_chunks(const std::vector<int32_t>& row_ptr, int node_layout, int32_t budget,
+                                       int32_t max_nodes);
 
 // Throws std::runtime_error on inconsistent input or exceeded limits.
 void build_p1_block_plan(int32_t gdim, int64_t num_vertices, int64_t num_cells,

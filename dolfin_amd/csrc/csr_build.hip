@@ -215,7 +215,8 @@ void download_csr(int64_t ndofs, int64_t nnz, int32_t* rp, int32_t* col, HostCSR
 // The node-blocked vector-P2 build. Returns 1 when done, 0 when the dofmap is not
 // node-blocked in either layout (the general build applies), -1 when device memory is short.
 int build_vp2_csr_device(int64_t nc, const int32_t* d_cell_dofs, int64_t ndofs, HostCSR& out,
-                         int32_t** d_row_ptr, int32_t** d_col, hipStream_t s, bool download_col) {
+                         int32_t** d_row_ptr, int32_t** d_col, hipStream_t s, bool download_col,
+                         int32_t** d_node_rp, int32_t** d_node_col) {
   const int64_t nn = ndofs / 3;
   DeviceBuffer flag, nid, ka, kb, tmp, cnt, nrp, ncol;
   if (hipMalloc(&flag.p, sizeof(int32_t)) != hipSuccess) {
@@ -278,8 +279,17 @@ int build_vp2_csr_device(int64_t nc, const int32_t* d_cell_dofs, int64_t ndofs, 
     hip_check(e, "vp2_rows_kernel / vp2_cols_kernel");
   }
   download_csr(ndofs, nnz, rp, col, out, download_col, s);
-  // the node graph, for host code that works per node (a ninth of the column indices)
   out.node_layout = node_major ? 2 : 1;
+  if (d_node_rp && d_node_col) {  // the caller keeps the node graph on the device
+    *d_node_rp = static_cast<int32_t*>(nrp.p);
+    *d_node_col = static_cast<int32_t*>(ncol.p);
+    nrp.p = nullptr;
+    ncol.p = nullptr;
+    *d_row_ptr = rp;
+    *d_col = col;
+    return 1;
+  }
+  // the node graph, for host code that works per node (a ninth of the column indices)
   out.node_row_ptr.resize(static_cast<size_t>(nn) + 1);
   out.node_col.resize(static_cast<size_t>(nnz_n));
   hipError_t e2 = hipMemcpyAsync(out.node_row_ptr.data(), node_rp, out.node_row_ptr.size() * sizeof(int32_t),
@@ -302,12 +312,13 @@ int build_vp2_csr_device(int64_t nc, const int32_t* d_cell_dofs, int64_t ndofs, 
 
 bool build_csr_device(int64_t nc, int32_t nd, const int32_t* d_cell_dofs, int64_t ndofs,
                       bool keep_diagonal, HostCSR& out, int32_t** d_row_ptr, int32_t** d_col,
-                      hipStream_t s, bool download_col) {
+                      hipStream_t s, bool download_col, int32_t** d_node_rp, int32_t** d_node_col) {
   if (ndofs <= 0 || ndofs > INT32_MAX) return false;
   // vector P2 with a node-blocked dofmap (keep_diagonal, which adds the diagonal of dofs in
   // no cell, takes the general build)
   if (nd == 30 && nc > 0 && ndofs % 3 == 0 && !keep_diagonal && !std::getenv("DFA_CSR_GENERAL")) {
-    const int r = build_vp2_csr_device(nc, d_cell_dofs, ndofs, out, d_row_ptr, d_col, s, download_col);
+    const int r = build_vp2_csr_device(nc, d_cell_dofs, ndofs, out, d_row_ptr, d_col, s, download_col,
+                                       d_node_rp, d_node_col);
     if (r > 0) return true;
     if (r < 0) return false;
   }

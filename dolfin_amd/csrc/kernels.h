@@ -140,10 +140,28 @@ struct CsrView {
   const double* values = nullptr;
   const int32_t* blocks = nullptr;  // nblocks + 1 pairs (first row, row_ptr of that row)
   int64_t nblocks = 0;
+  // narrow columns: per row block the smallest column (or -1: the block reads col) and the
+  // columns as 16-bit offsets from it
+  const int32_t* blk_col0 = nullptr;
+  const uint16_t* col16 = nullptr;
+  // vector P2 expanded from the node graph (node_layout 1 = component-major, 2 = node-major):
+  // every node entry (n, m) is a 3 x 3 block of the dof rows; 0 = a general CSR
+  int32_t node_layout = 0;
+  int64_t node_nnz = 0;
+  const int32_t* node_rp = nullptr;      // nrows / 3 + 1
+  const int32_t* node_col = nullptr;     // node_nnz
+  const int32_t* node_chunks = nullptr;  // node_nchunks + 1 pairs (first node, first node entry)
+  int64_t node_nchunks = 0;
 };
+// Node-blocked SpMV: a workgroup takes a chunk of consecutive nodes with at most kNodeBudget
+// node entries and kNodeMaxNodes nodes (from build_node_chunks).
+constexpr int kNodeBudget = 1024;
+constexpr int kNodeMaxNodes = 128;
 hipError_t launch_spmv(const CsrView& A, double alpha, const double* x, double beta, double* y,
                        hipStream_t s);
 hipError_t launch_diagonal(const CsrView& A, double* d, hipStream_t s);
+// The narrow-column arrays of the streamed SpMV (blk_col0 [nblocks], col16 [nnz]).
+hipError_t launch_narrow_columns(const CsrView& A, int32_t* blk_col0, uint16_t* col16, hipStream_t s);
 // Host-side block insertion (GenericMatrix::add_local): slot[i*n+j] = position of
 // (rows[i], cols[j]) in the CSR (-1 if the pattern lacks it), then values[slot] += block.
 hipError_t launch_block_slots(int32_t m, const int32_t* rows, int32_t n, const int32_t* cols,

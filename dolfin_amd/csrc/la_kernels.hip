@@ -13,6 +13,8 @@
 // the row pointers and y.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "kernels.h"
 
 namespace dfa {
@@ -23,8 +25,9 @@ constexpr int kNpt = kSpmvBudget / kLaThreads;  // products per thread
 
 __global__ __launch_bounds__(kLaThreads) void spmv_stream_kernel(
     int nblocks, const int32_t* __restrict__ blocks, const int32_t* __restrict__ rp,
-    const int32_t* __restrict__ col, const double* __restrict__ vals, double alpha,
-    const double* __restrict__ x, double beta, double* __restrict__ y) {
+    const int32_t* __restrict__ col, const int32_t* __restrict__ blk_col0, const uint16_t* __restrict__ col16,
+    const double* __restrict__ vals, double alpha, const double* __restrict__ x, double beta,
+    double* __restrict__ y) {
   __shared__ double prod[kSpmvBudget];
   __shared__ int32_t srp[kSpmvMaxRows + 1];
   __shared__ double wsum[kLaThreads / 64];
@@ -55,17 +58,42 @@ __global__ __launch_bounds__(kLaThreads) void spmv_stream_kernel(
   for (int i = tid; i <= nr; i += kLaThreads) srp[i] = rp[r0 + i] - base;
   int c[kNpt];
   double v[kNpt];
+  // Every load is unconditional (indices past the block clamped to its last nonzero, whose
+  // products are never stored), so that all 2 x kNpt loads of a thread are in flight at once;
+  // values and columns are read once: streaming loads keep L2 for the x gathers.
+  const int last = end - 1;
+  const int c0 = col16 ? blk_col0[blk] : -1;
+  if (end == base) {  // rows without entries: nothing to read
 #pragma unroll
-  for (int j = 0; j < kNpt; ++j) {
-    const int k = base + j * kLaThreads + tid;
-    // read once: streaming loads keep L2 for the x gathers
-    c[j] = k < end ? __builtin_nontemporal_load(col + k) : 0;
-    v[j] = k < end ? __builtin_nontemporal_load(vals + k) : 0.0;
+    for (int j = 0; j < kNpt; ++j) {
+      c[j] = 0;
+      v[j] = 0.0;
+    }
+  } else if (c0 >= 0) {  // 2-byte column offsets
+    uint32_t o[kNpt];
+#pragma unroll
+    for (int j = 0; j < kNpt; ++j) {
+      const int k = min(base + j * kLaThreads + tid, last);
+      o[j] = __builtin_nontemporal_load(col16 + k);
+      v[j] = __builtin_nontemporal_load(vals + k);
+    }
+#pragma unroll
+    for (int j = 0; j < kNpt; ++j) c[j] = c0 + static_cast<int>(o[j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kNpt; ++j) {
+      const int k = min(base + j * kLaThreads + tid, last);
+      c[j] = __builtin_nontemporal_load(col + k);
+      v[j] = __builtin_nontemporal_load(vals + k);
+    }
   }
+  double xv[kNpt];  // gathered unconditionally too (clamped entries name a valid column)
+#pragma unroll
+  for (int j = 0; j < kNpt; ++j) xv[j] = x[c[j]];
 #pragma unroll
   for (int j = 0; j < kNpt; ++j) {
     const int k = j * kLaThreads + tid;
-    if (base + k < end) prod[k] = v[j] * x[c[j]];
+    if (base + k < end) prod[k] = v[j] * xv[j];
   }
   __syncthreads();
   int lg = 0;  // lanes per row: 2^lg, the largest power of two with nr * 2^lg <= threads, <= 64
@@ -79,6 +107,143 @@ __global__ __launch_bounds__(kLaThreads) void spmv_stream_kernel(
     for (int o = L >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (lr < nr && ln == 0) y[r0 + lr] = beta == 0.0 ? alpha * s : fma(beta, y[r0 + lr], alpha * s);
   }
+}
+
+// Node-blocked SpMV for vector-P2 structures expanded from the node graph: with nodes n, m and
+// components i, j, the dof rows (i, n) hold the 3 x 3 blocks (i, j) of every node entry
+// (n, m), at row_ptr(i, n) + j d + q (component-major, row (i, n) = i nn + n, column
+// j nn + m) or row_ptr(3 n + i) + 3 q + j (node-major, column 3 m + j), for the q-th of the d
+// node entries of node n; the row starts follow from the node row pointers. Each node column
+// is read once for its nine values (the dof-level column array, 4 bytes per value, is not
+// read) and x once per node entry and component. A workgroup takes a chunk of consecutive
+// nodes (at most kNodeBudget node entries): the node columns and the chunk's node pointers
+// are loaded together, every entry finds its node by a binary search in LDS, gathers x and
+// its nine values, stages the three row products in LDS, and each of the chunk's 3 x nodes
+// rows is summed there by a power-of-two group of lanes. Workgroups are mapped so that each
+// XCD sweeps a contiguous range of chunks.
+constexpr int kNodeThreads = 512;
+constexpr int kNodeEpt = kNodeBudget / kNodeThreads;  // node entries per thread
+template <bool NODE_MAJOR>
+__global__ __launch_bounds__(kNodeThreads) void spmv_node_kernel(
+    int nchunks, const int32_t* __restrict__ chunks, int64_t nn, int64_t nnz_n, const int32_t* __restrict__ nrp,
+    const int32_t* __restrict__ ncol, const double* __restrict__ vals, double alpha,
+    const double* __restrict__ x, double beta, double* __restrict__ y) {
+  __shared__ double prod[3 * kNodeBudget];
+  __shared__ int32_t snp[kNodeMaxNodes + 1];
+  const int q8 = nchunks >> 3, rem = nchunks & 7;
+  const int xcd = blockIdx.x & 7;
+  const int ch = xcd * q8 + min(xcd, rem) + static_cast<int>(blockIdx.x >> 3);
+  const int tid = threadIdx.x;
+  const int2 c0 = reinterpret_cast<const int2*>(chunks)[ch];
+  const int2 c1 = reinterpret_cast<const int2*>(chunks)[ch + 1];
+  const int n0 = c0.x, nnodes = c1.x - c0.x, E0 = c0.y, E = c1.y - c0.y;
+  // value and x indices fit 32 bits: nnz and the dof count are at most INT32_MAX
+  const uint32_t nn32 = static_cast<uint32_t>(nn), nnz32 = static_cast<uint32_t>(nnz_n);
+  if (tid <= nnodes) snp[tid] = nrp[n0 + tid] - E0;
+  // unconditional loads (entries past the chunk clamped to its last, never stored), so that a
+  // thread's loads are all in flight at once
+  uint32_t m[kNodeEpt];
+#pragma unroll
+  for (int h = 0; h < kNodeEpt; ++h) {
+    const int e = min(tid + h * kNodeThreads, E - 1);
+    m[h] = static_cast<uint32_t>(__builtin_nontemporal_load(ncol + E0 + e));
+  }
+  double xm[kNodeEpt][3];
+#pragma unroll
+  for (int h = 0; h < kNodeEpt; ++h)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) xm[h][j] = NODE_MAJOR ? x[3u * m[h] + j] : x[j * nn32 + m[h]];
+  __syncthreads();
+  double v[kNodeEpt][9];
+#pragma unroll
+  for (int h = 0; h < kNodeEpt; ++h) {
+    const int e = min(tid + h * kNodeThreads, E - 1);
+    int a = 0, b = nnodes - 1;  // the node ln with snp[ln] <= e < snp[ln + 1]
+    while (a < b) {
+      const int mid = (a + b + 1) >> 1;
+      if (snp[mid] <= e) a = mid;
+      else b = mid - 1;
+    }
+    const uint32_t s0 = static_cast<uint32_t>(snp[a]);
+    const uint32_t d = static_cast<uint32_t>(snp[a + 1]) - s0, q = static_cast<uint32_t>(e) - s0;
+    const uint32_t ent = static_cast<uint32_t>(E0) + s0;  // first node entry of node n0 + a
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const uint32_t base = NODE_MAJOR ? 9u * ent + 3u * i * d : 3u * i * nnz32 + 3u * ent;
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        v[h][3 * i + j] = __builtin_nontemporal_load(vals + (NODE_MAJOR ? base + 3u * q + j : base + j * d + q));
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < kNodeEpt; ++h) {
+    const int e = tid + h * kNodeThreads;
+    if (e < E)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        prod[i * kNodeBudget + e] =
+            fma(v[h][3 * i + 2], xm[h][2], fma(v[h][3 * i + 1], xm[h][1], v[h][3 * i] * xm[h][0]));
+  }
+  __syncthreads();
+  // rows (i, node ln), r = i * nnodes + ln; lanes per row: the largest power of two with
+  // rows * 2^lg <= threads, at most 32
+  const int nrows = 3 * nnodes;
+  int lg = 0;
+  while (lg < 5 && (nrows << (lg + 1)) <= kNodeThreads) ++lg;
+  const int L = 1 << lg, groups = kNodeThreads >> lg;
+  const int ln0 = tid & (L - 1);
+  for (int r = tid >> lg; r - (tid >> lg) < nrows; r += groups) {  // uniform trip count
+    double sum = 0.0;
+    int i = 0, ln = 0;
+    if (r < nrows) {
+      i = r / nnodes;
+      ln = r - i * nnodes;
+      for (int e = snp[ln] + ln0; e < snp[ln + 1]; e += L) sum += prod[i * kNodeBudget + e];
+    }
+    for (int o = L >> 1; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+    if (r < nrows && ln0 == 0) {
+      const uint32_t n = static_cast<uint32_t>(n0 + ln);
+      const uint32_t row = NODE_MAJOR ? 3u * n + i : i * nn32 + n;
+      y[row] = beta == 0.0 ? alpha * sum : fma(beta, y[row], alpha * sum);
+    }
+  }
+}
+
+// Narrow columns of each streamed row block: the block's smallest column and the columns as
+// offsets from it when they span at most 65536 (else -1: the block reads the 4-byte columns).
+// A single row longer than a block keeps the 4-byte columns.
+__global__ __launch_bounds__(kLaThreads) void narrow_columns_kernel(const int32_t* __restrict__ blocks,
+                                                                   const int32_t* __restrict__ col,
+                                                                   int32_t* __restrict__ blk_col0,
+                                                                   uint16_t* __restrict__ col16) {
+  __shared__ int wmin[kLaThreads / 64], wmax[kLaThreads / 64];
+  const int blk = blockIdx.x, tid = threadIdx.x;
+  const int base = blocks[2 * blk + 1], end = blocks[2 * blk + 3];
+  int lo = INT32_MAX, hi = INT32_MIN;
+  for (int k = base + tid; k < end; k += kLaThreads) {
+    lo = min(lo, col[k]);
+    hi = max(hi, col[k]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o));
+    hi = max(hi, __shfl_xor(hi, o));
+  }
+  if ((tid & 63) == 0) {
+    wmin[tid >> 6] = lo;
+    wmax[tid >> 6] = hi;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < kLaThreads / 64; ++w) {
+    lo = min(lo, wmin[w]);
+    hi = max(hi, wmax[w]);
+  }
+  const bool narrow = end - base <= kSpmvBudget && end > base &&
+                      static_cast<int64_t>(hi) - static_cast<int64_t>(lo) <= 65535;
+  if (tid == 0) blk_col0[blk] = narrow ? lo : -1;
+  if (narrow)
+    for (int k = base + tid; k < end; k += kLaThreads) col16[k] = static_cast<uint16_t>(col[k] - lo);
 }
 
 __global__ __launch_bounds__(kLaThreads) void diagonal_kernel(int64_t nrows, const int32_t* __restrict__ rp,
@@ -178,10 +343,33 @@ hipError_t launch_dirichlet_x0(int64_t n, const uint8_t* mask, const double* g, 
 hipError_t launch_spmv(const CsrView& A, double alpha, const double* x, double beta, double* y,
                        hipStream_t s) {
   if (A.nrows == 0) return hipSuccess;
+  if (A.node_layout) {
+    if (!A.node_rp || !A.node_col || !A.node_chunks || A.node_nchunks <= 0 || A.node_nchunks > INT32_MAX ||
+        A.nrows % 3 || A.node_nnz * 9 != A.nnz)
+      return hipErrorInvalidValue;
+    const int nch = static_cast<int>(A.node_nchunks);
+    if (A.node_layout == 2)
+      hipLaunchKernelGGL(spmv_node_kernel<true>, dim3(static_cast<unsigned>(nch)), dim3(kNodeThreads), 0, s, nch,
+                         A.node_chunks, A.nrows / 3, A.node_nnz, A.node_rp, A.node_col, A.values, alpha, x, beta,
+                         y);
+    else
+      hipLaunchKernelGGL(spmv_node_kernel<false>, dim3(static_cast<unsigned>(nch)), dim3(kNodeThreads), 0, s, nch,
+                         A.node_chunks, A.nrows / 3, A.node_nnz, A.node_rp, A.node_col, A.values, alpha, x, beta,
+                         y);
+    return hipGetLastError();
+  }
   if (!A.blocks || A.nblocks <= 0 || A.nblocks > INT32_MAX) return hipErrorInvalidValue;
   hipLaunchKernelGGL(spmv_stream_kernel, dim3(static_cast<unsigned>(A.nblocks)), dim3(kLaThreads), 0,
-                     s, static_cast<int>(A.nblocks), A.blocks, A.row_ptr, A.col, A.values, alpha, x,
-                     beta, y);
+                     s, static_cast<int>(A.nblocks), A.blocks, A.row_ptr, A.col,
+                     A.col16 ? A.blk_col0 : nullptr, A.col16, A.values, alpha, x, beta, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_narrow_columns(const CsrView& A, int32_t* blk_col0, uint16_t* col16, hipStream_t s) {
+  if (A.nblocks <= 0) return hipSuccess;
+  if (!A.blocks || !A.col || A.nblocks > INT32_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(narrow_columns_kernel, dim3(static_cast<unsigned>(A.nblocks)), dim3(kLaThreads), 0, s,
+                     A.blocks, A.col, blk_col0, col16);
   return hipGetLastError();
 }
 

@@ -621,6 +621,30 @@ std::vector<int32_t> build_spmv_blocks(const std::vector<int32_t>& row_ptr, int3
   return blocks;
 }
 
+std::vector<int32_t> build_node_chunks(const std::vector<int32_t>& row_ptr, int node_layout, int32_t budget,
+                                       int32_t max_nodes) {
+  const int64_t nn = (static_cast<int64_t>(row_ptr.size()) - 1) / 3;
+  // node entries before node n: component-0 row n holds 3 per entry (component-major), the
+  // three rows of node n start at 9 per entry (node-major)
+  auto nrp = [&](int64_t n) -> int64_t {
+    if (n == nn) return static_cast<int64_t>(row_ptr.back()) / 9;
+    return node_layout == 2 ? row_ptr[3 * n] / 9 : row_ptr[n] / 3;
+  };
+  std::vector<int32_t> chunks;
+  int64_t n = 0;
+  while (n < nn) {
+    if (nrp(n + 1) - nrp(n) > budget) return {};  // a node longer than a chunk: no node path
+    chunks.push_back(static_cast<int32_t>(n));
+    chunks.push_back(static_cast<int32_t>(nrp(n)));
+    int64_t e = n + 1;
+    while (e < nn && e - n < max_nodes && nrp(e + 1) - nrp(n) <= budget) ++e;
+    n = e;
+  }
+  chunks.push_back(static_cast<int32_t>(nn));
+  chunks.push_back(static_cast<int32_t>(nrp(nn)));
+  return chunks;
+}
+
 void build_exterior_facets(int32_t gdim, int64_t nc, const int32_t* cells,
                            std::vector<int32_t>& facet_cell, std::vector<int8_t>& facet_local) {
   const int nv = gdim + 1;

@@ -1,7 +1,10 @@
 """Device linear algebra on assembled matrices: CSR SpMV y = alpha A x + beta y and the diagonal
 against scipy on the same CSR arrays (tolerance: relative L2 1e-14 for SpMV, bit-exact for the
 diagonal), Jacobi CG against a direct solve (relative 1e-9) and the P1 nodal-exactness known
-answer of -laplace(u) = -6, u = 1 + x^2 + 2 y^2."""
+answer of -laplace(u) = -6, u = 1 + x^2 + 2 y^2. Vector-P2 matrices whose structure was
+expanded from the node graph take the node-blocked SpMV kernel (both dof layouts); the
+general streamed kernel runs for P1, for vector P2 with keep_diagonal (general structure
+build) and under DFA_SPMV_CSR=1."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -24,22 +27,46 @@ def _scipy(A):
     return sp.csr_matrix((v, col, rp), shape=(n, n))
 
 
-def _p1_matrix(n):
+def _p1_matrix(n, perm=None):
     mesh, V, kappa, f = workloads.p1_poisson_3d(n, seed=5)
+    if perm is not None:  # renumbered dofs: "all" (random) or "swap" (two far-apart vertices)
+        nv = mesh.num_vertices()
+        p = np.random.default_rng(n).permutation(nv) if perm == "all" else np.arange(nv)
+        if perm == "swap":
+            p[[0, nv - 1]] = p[[nv - 1, 0]]
+        V = dfa.FunctionSpace(mesh, "P", 1, node_permutation=p)
     return dfa.assemble(dfa.poisson_form(V, workloads.to_function(V, kappa)))
 
 
-def _vp2_matrix(n):
-    mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(n, seed=6)
+def _vp2_matrix(n, node_major=False, keep_diagonal=False):
+    perm = None
+    if node_major:
+        m = dfa.UnitCubeMesh(n, n, n)
+        perm = np.random.default_rng(n).permutation(m.num_vertices() + m.cell_edges()[1])
+    mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(n, seed=6, node_permutation=perm)
     Q = V.coefficient_space()
-    return dfa.assemble(dfa.elasticity_form(V, workloads.to_function(Q, mu),
-                                            workloads.to_function(Q, lmbda)))
+    a = dfa.elasticity_form(V, workloads.to_function(Q, mu), workloads.to_function(Q, lmbda))
+    if keep_diagonal:
+        asm = dfa.Assembler()
+        asm.keep_diagonal = True
+        A = dfa.Matrix()
+        asm.assemble(A, a)
+        return A
+    return dfa.assemble(a)
 
 
-@pytest.mark.parametrize("kind,n", [("p1", 7), ("p1", 24), ("vp2", 3), ("vp2", 6)])
+# P1 row blocks read 16-bit column offsets when their columns span at most 65536 ("p1"), the
+# 4-byte columns otherwise ("p1_all": randomly renumbered dofs, every block wide; "p1_swap":
+# two far-apart vertices swapped, a few wide blocks among narrow ones)
+@pytest.mark.parametrize("kind,n", [("p1", 7), ("p1", 24), ("p1_all", 40), ("p1_swap", 40), ("vp2", 3),
+                                    ("vp2", 6), ("vp2_node_major", 4), ("vp2_node_major", 7),
+                                    ("vp2_general", 4)])
 def test_spmv_and_diagonal_vs_scipy(kind, n):
     print(f"case spmv {kind} n={n}", flush=True)
-    A = _p1_matrix(n) if kind == "p1" else _vp2_matrix(n)
+    if kind.startswith("p1"):
+        A = _p1_matrix(n, perm=kind[3:] or None)
+    else:
+        A = _vp2_matrix(n, node_major=kind == "vp2_node_major", keep_diagonal=kind == "vp2_general")
     S = _scipy(A)
     rng = np.random.default_rng(n)
     x = rng.uniform(-1, 1, S.shape[0])
@@ -55,6 +82,23 @@ def test_spmv_and_diagonal_vs_scipy(kind, n):
     d = A.diagonal()
     torch.cuda.synchronize()
     assert np.array_equal(d.cpu().numpy(), S.diagonal())
+
+
+@pytest.mark.parametrize("kind", ["vp2", "vp2_node_major", "p1", "p1_swap"])
+def test_spmv_equals_plain_streamed_csr(monkeypatch, kind):
+    # the node-blocked kernel (vector P2) and the 16-bit column offsets (P1) against the
+    # streamed kernel on the 4-byte columns (DFA_SPMV_CSR=1): equal up to the summation order
+    print(f"case spmv vs plain csr {kind}", flush=True)
+    if kind.startswith("p1"):
+        A = _p1_matrix(40, perm=kind[3:] or None)
+    else:
+        A = _vp2_matrix(5, node_major=kind == "vp2_node_major")
+    x = torch.from_numpy(np.random.default_rng(11).uniform(-1, 1, A.structure()[0].numel() - 1)).cuda()
+    y_node = A.mult(x)
+    monkeypatch.setenv("DFA_SPMV_CSR", "1")
+    y_csr = A.mult(x)
+    torch.cuda.synchronize()
+    assert (torch.linalg.norm(y_node - y_csr) / torch.linalg.norm(y_csr)).item() < 1e-14
 
 
 def test_spmv_long_row_fan_mesh():
