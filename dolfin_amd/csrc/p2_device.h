@@ -359,19 +359,55 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
   const int tid = threadIdx.x;
 
   const int v0 = t.vert_ptr[blk], nv = t.vert_ptr[blk + 1] - v0;
-  for (int i = tid; i < nv; i += kBT) {
+  const int o0 = t.own_ptr[blk], no = t.own_ptr[blk + 1] - o0;
+  const int t0 = t.task_ptr[blk], nt = t.task_ptr[blk + 1] - t0;
+  const int nvx = t.task_nvert[blk], ned = nt - nvx;
+  const int pv = (nvx + 63) & ~63;
+  // Work items are laid out in wavefront-aligned segments (below); item kk -> task k.
+  auto item = [&](int kk, int& k, bool& vert) {
+    vert = kk < pv;
+    k = vert ? kk : nvx + (kk - pv);
+    return vert ? kk < nvx : kk < pv + ned;
+  };
+  const bool need_c = WANT_A || BC;  // the body force alone needs no coefficients
+  // The prologue's loads in two rounds, each issued before its first wait: the first slot of
+  // every per-thread list with its index clamped into the block (a block has at least one
+  // vertex and owned node; clamped entries are read, never stored), then the vertex gathers
+  // and the CSR row starts. (Guarded loads, one per loop iteration, made every load wait for
+  // the previous one.)
+  const int iv = min(tid, nv - 1), io = min(tid, no - 1), ir = min(tid, 3 * no - 1);
+  const int64_t vid = t.vert_id[v0 + iv];
+  const int32_t vkd = t.vert_kdof[v0 + iv];
+  const uint32_t oinfo = t.own_info[o0 + io];
+  const int32_t orow = t.own_rows[3 * o0 + ir];
+  {
+    const double2 r0 = make_double2(p.x[vid * 3], p.x[vid * 3 + 1]);
+    const double2 r1 = make_double2(p.x[vid * 3 + 2], need_c ? p.mu[vkd] : 0.0);
+    const double2 r2 = make_double2(need_c ? p.lmbda[vkd] : 0.0, 0.0);
+    const int32_t obase = WANT_A ? t.row_ptr[orow] : 0;
+    if (tid < nv) {
+      double* rec = svert + tid * 6;
+      *reinterpret_cast<double2*>(rec) = r0;
+      *reinterpret_cast<double2*>(rec + 2) = r1;
+      *reinterpret_cast<double2*>(rec + 4) = r2;
+    }
+    if (tid < no) sinfo[tid] = oinfo;
+    // owned rows and their CSR starts, fetched now so that the write-out issues no loads
+    if (tid < 3 * no) {
+      srows[tid] = orow;
+      if (WANT_A) sbase[tid] = obase;
+    }
+  }
+  for (int i = tid + kBT; i < nv; i += kBT) {  // blocks of more than kBT vertices
     const int64_t v = t.vert_id[v0 + i];
     const int32_t kd = t.vert_kdof[v0 + i];
     double* rec = svert + i * 6;
     *reinterpret_cast<double2*>(rec) = make_double2(p.x[v * 3], p.x[v * 3 + 1]);
-    const bool need_c = WANT_A || BC;  // the body force alone needs no coefficients
     *reinterpret_cast<double2*>(rec + 2) = make_double2(p.x[v * 3 + 2], need_c ? p.mu[kd] : 0.0);
     *reinterpret_cast<double2*>(rec + 4) = make_double2(need_c ? p.lmbda[kd] : 0.0, 0.0);
   }
-  const int o0 = t.own_ptr[blk], no = t.own_ptr[blk + 1] - o0;
-  for (int i = tid; i < no; i += kBT) sinfo[i] = t.own_info[o0 + i];
-  // owned rows and their CSR starts, fetched now so that the write-out issues no loads
-  for (int i = tid; i < 3 * no; i += kBT) {
+  for (int i = tid + kBT; i < no; i += kBT) sinfo[i] = t.own_info[o0 + i];
+  for (int i = tid + kBT; i < 3 * no; i += kBT) {
     const int32_t row = t.own_rows[3 * o0 + i];
     srows[i] = row;
     if (WANT_A) sbase[i] = t.row_ptr[row];
@@ -390,20 +426,10 @@ __device__ __forceinline__ void vp2_block_body(const VP2BlockTopo& t, const VP2P
   // and one of its owned nodes) is one thread over all ten node pairs b. (Splitting edge items
   // over two threads, five pairs each, made a block take five wavefront passes for four
   // wavefronts: 7 % slower on the 64^3 workload.)
-  const int t0 = t.task_ptr[blk], nt = t.task_ptr[blk + 1] - t0;
-  const int nvx = t.task_nvert[blk], ned = nt - nvx;
-  const int pv = (nvx + 63) & ~63;
   for (int kk = tid; kk < pv + ned; kk += kBT) {
     int k;
     bool vert;
-    if (kk < pv) {
-      if (kk >= nvx) continue;
-      k = kk;
-      vert = true;
-    } else {
-      k = nvx + (kk - pv);
-      vert = false;
-    }
+    if (!item(kk, k, vert)) continue;
     // task words are read once: streaming loads keep L2 for the vertex gathers
     const uint64_t w0 = __builtin_nontemporal_load(t.task + 3 * (t0 + k));
     const uint64_t w1 = __builtin_nontemporal_load(t.task + 3 * (t0 + k) + 1);
