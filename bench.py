@@ -346,6 +346,28 @@ def main():
                               "format_bytes": 10 * eng.nnz + 4 * (V.dim() + 1) + 16 * V.dim(),
                               "traffic": load_traffic(f"p1_spmv_{args.n}"),
                               "workload": f"y = A x on the assembled P1 {args.n}^3 matrix"}
+            # Jacobi-preconditioned CG on the same matrix (dfa_cg_solve: the SpMV and three fused
+            # vector kernels per iteration): 50 iterations per solve, tolerance 0 so that all run
+            bcg = torch.rand(V.dim(), dtype=torch.float64, device="cuda")
+            xcg = torch.zeros_like(bcg)
+            cg_err = []
+
+            def cg50():
+                xcg.zero_()
+                try:
+                    dfa.solve(A, xcg, bcg, "cg", "jacobi", rtol=0.0, atol=0.0, maxiter=50, check_every=50)
+                except RuntimeError as e:
+                    if "did not converge" not in str(e):
+                        cg_err.append(str(e))
+            w_cg, ev_cg, _ = timed_steps(cg50, 4, 1, settle_s=0.0)
+            cg_it = float(np.mean(ev_cg)) / 50
+            cg_bytes = mv_bytes + 8 * 13 * V.dim()  # + p.Ap (2n), x, r update (7n), p update (4n)
+            sec["p1_cg"] = {"ms_per_iteration": cg_it, "spmv_ms": mv_ms,
+                            "hbm_frac": cg_bytes / (cg_it * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                            "algorithmic_bytes_per_iteration": cg_bytes,
+                            "errors": cg_err,
+                            "workload": f"Jacobi-preconditioned CG iterations on the P1 {args.n}^3 matrix "
+                                        "(50 per solve, 4 solves)"}
             log("spmv timed")
         if want("strategies"):
             # the other two scatter strategies on the same A+b workload (the choice between

@@ -36,7 +36,7 @@ EXPORTS = ["dfa_last_error", "dfa_version", "dfa_assembler_create", "dfa_assembl
            "dfa_csr_dims", "dfa_csr_structure", "dfa_assemble_matrix", "dfa_assemble_vector",
            "dfa_assemble_system", "dfa_adjoint", "dfa_plan_info", "dfa_set_plan_defaults",
            "dfa_batch_create", "dfa_batch_launch", "dfa_batch_info", "dfa_batch_destroy",
-           "dfa_exterior_facets", "dfa_assemble_facets", "dfa_csr_spmv", "dfa_csr_diagonal",
+           "dfa_exterior_facets", "dfa_assemble_facets", "dfa_csr_spmv", "dfa_csr_diagonal", "dfa_cg_solve",
            "dfa_node_reordering", "dfa_graph_reordering", "dfa_mesh_coloring", "dfa_color_info",
            "dfa_device_info", "dfa_assemble_system_x0", "dfa_dirichlet_x0", "dfa_pass_kernel", "dfa_plan_builder",
            "dfa_csr_add_local", "dfa_vector_add_local", "dfa_assemble_system_x0_noalloc"]
@@ -89,6 +89,8 @@ def load():
     L.dfa_exterior_facets.argtypes = [_vp, ctypes.POINTER(_i64), _vp, _vp]
     L.dfa_csr_spmv.argtypes = [_vp, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp]
     L.dfa_csr_diagonal.argtypes = [_vp, _vp, _vp, _vp]
+    L.dfa_cg_solve.argtypes = [_vp, _vp, _vp, _vp, _i32, ctypes.c_double, ctypes.c_double, _i64, _i32,
+                               ctypes.POINTER(_i64), _vp]
     L.dfa_node_reordering.argtypes = [_i64, _i32, _vp, _i64, _vp]
     L.dfa_graph_reordering.argtypes = [_i64, _vp, _vp, _vp]
     L.dfa_mesh_coloring.argtypes = [_i64, _i32, _vp, _i64, _i32, _vp, ctypes.POINTER(_i32)]

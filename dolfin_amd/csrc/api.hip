@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 
 #include <algorithm>
@@ -299,6 +300,33 @@ std::string same_array(const char* name, const T* d, const V& hv) {
            std::to_string(n);
   }
   return "";
+}
+
+// The SpMV view of a handle's structure with the values d_values: the node-blocked kernel for
+// vector P2 expanded from the node graph, 16-bit column offsets otherwise (DFA_SPMV_CSR=1: the
+// plain 4-byte columns).
+dfa::CsrView spmv_view(dfa_assembler_s* h, const double* d_values) {
+  dfa::CsrView A;
+  A.nrows = h->ndofs;
+  A.nnz = h->nnz;
+  A.row_ptr = h->d_row_ptr;
+  A.col = h->d_col;
+  A.values = d_values;
+  A.blocks = h->d_spmv_blocks;
+  A.nblocks = h->spmv_nblocks;
+  if (!std::getenv("DFA_SPMV_CSR")) {
+    A.blk_col0 = h->d_blk_col0;
+    A.col16 = h->d_col16;
+    if (h->d_node_chunks) {
+      A.node_layout = h->node_layout;
+      A.node_nnz = h->node_nnz;
+      A.node_rp = h->d_node_rp;
+      A.node_col = h->d_node_col;
+      A.node_chunks = h->d_node_chunks;
+      A.node_nchunks = h->node_nchunks;
+    }
+  }
+  return A;
 }
 
 // Outcome of two plan builds when at least one failed ("" = both failed alike).
@@ -746,6 +774,7 @@ int dfa_assembler_destroy(dfa_assembler h) {
   h->colors.release();
   h->facets.release();
   h->x0buf.release();
+  h->cg.release();
   delete h;
   return DFA_OK;
 }
@@ -1371,29 +1400,80 @@ int dfa_csr_spmv(dfa_assembler h, const double* d_values, double alpha, const do
                  double beta, double* d_y, void* stream) {
   return guarded([&]() -> int {
     if (!h || !d_values || !d_x || !d_y) return fail("NULL argument", DFA_ERR_ARG);
-    dfa::CsrView A;
-    A.nrows = h->ndofs;
-    A.nnz = h->nnz;
-    A.row_ptr = h->d_row_ptr;
-    A.col = h->d_col;
-    A.values = d_values;
-    A.blocks = h->d_spmv_blocks;
-    A.nblocks = h->spmv_nblocks;
-    if (!std::getenv("DFA_SPMV_CSR")) {
-      A.blk_col0 = h->d_blk_col0;
-      A.col16 = h->d_col16;
-    }
-    if (h->d_node_chunks && !std::getenv("DFA_SPMV_CSR")) {
-      A.node_layout = h->node_layout;
-      A.node_nnz = h->node_nnz;
-      A.node_rp = h->d_node_rp;
-      A.node_col = h->d_node_col;
-      A.node_chunks = h->d_node_chunks;
-      A.node_nchunks = h->node_nchunks;
-    }
-    hip_check(dfa::launch_spmv(A, alpha, d_x, beta, d_y, static_cast<hipStream_t>(stream)),
+    hip_check(dfa::launch_spmv(spmv_view(h, d_values), alpha, d_x, beta, d_y, static_cast<hipStream_t>(stream)),
               "spmv_kernel");
     return DFA_OK;
+  });
+}
+
+int dfa_cg_solve(dfa_assembler h, const double* d_values, double* d_x, const double* d_b, int32_t jacobi,
+                 double rtol, double atol, int64_t maxiter, int32_t check_every, int64_t* iterations,
+                 void* stream) {
+  return guarded([&]() -> int {
+    if (!h || !d_values || !d_x || !d_b || !iterations) return fail("NULL argument", DFA_ERR_ARG);
+    if (check_every < 1) return fail("check_every must be at least 1", DFA_ERR_ARG);
+    if (!(rtol >= 0.0) || !(atol >= 0.0)) return fail("tolerances must be non-negative", DFA_ERR_ARG);
+    *iterations = 0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int64_t n = h->ndofs;
+    if (maxiter < 0) maxiter = 10 * n;
+    if (!h->d_cg_r) {  // work vectors, kept with the handle
+      auto vec = [&](int64_t m) {
+        void* d = nullptr;
+        hip_check(hipMalloc(&d, std::max<size_t>(static_cast<size_t>(m) * sizeof(double), 16)), "hipMalloc");
+        return h->cg.adopt(static_cast<double*>(d), m * 8);
+      };
+      h->d_cg_r = vec(n);
+      h->d_cg_p = vec(n);
+      h->d_cg_ap = vec(n);
+      h->d_cg_dinv = vec(n);
+      h->d_cg_part = vec(3 * dfa::kCgMaxGroups);
+      void* d = nullptr;
+      hip_check(hipMalloc(&d, sizeof(dfa::CgState)), "hipMalloc");
+      h->d_cg_state = h->cg.adopt(static_cast<dfa::CgState*>(d), sizeof(dfa::CgState));
+    }
+    const dfa::CsrView A = spmv_view(h, d_values);
+    double* r = h->d_cg_r;
+    const double* dinv = jacobi ? h->d_cg_dinv : nullptr;
+    // r = b - A x; the preconditioner diag(A)^-1 (1 where the diagonal is 0)
+    hip_check(hipMemcpyAsync(r, d_b, n * sizeof(double), hipMemcpyDeviceToDevice, s), "hipMemcpyAsync");
+    hip_check(dfa::launch_spmv(A, -1.0, d_x, 1.0, r, s), "spmv_kernel");
+    if (jacobi) {
+      hip_check(dfa::launch_diagonal(A, h->d_cg_dinv, s), "diagonal_kernel");
+      hip_check(dfa::launch_cg_jacobi(n, h->d_cg_dinv, s), "cg_jacobi_kernel");
+    }
+    dfa::CgState hs{};
+    hs.done = -1;
+    hip_check(hipMemcpyAsync(h->d_cg_state, &hs, sizeof hs, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    hip_check(dfa::launch_cg_init(n, d_b, r, dinv, h->d_cg_p, h->d_cg_part, h->d_cg_state, s), "cg_init_kernel");
+    hip_check(hipMemcpyAsync(&hs, h->d_cg_state, sizeof hs, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    const double tol = std::max(rtol * std::sqrt(hs.bb), atol);
+    const double tol2 = tol * tol;
+    if (hs.rr <= tol2) return DFA_OK;
+    hip_check(hipMemcpyAsync(&h->d_cg_state->tol2, &tol2, sizeof tol2, hipMemcpyHostToDevice, s),
+              "hipMemcpyAsync");
+    for (int64_t it = 1; it <= maxiter; ++it) {
+      hip_check(dfa::launch_cg_iteration(A, it, d_x, h->d_cg_p, r, h->d_cg_ap, dinv, h->d_cg_part,
+                                         h->d_cg_state, s),
+                "cg iteration");
+      if (it % check_every == 0 || it == maxiter) {
+        hip_check(hipMemcpyAsync(&hs, h->d_cg_state, sizeof hs, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        if (hs.done >= 0) {
+          *iterations = hs.done;
+          return DFA_OK;
+        }
+        if (hs.broke) {
+          *iterations = it;
+          return fail("conjugate gradients broke down (p.Ap = 0) at iteration <= " + std::to_string(it),
+                      DFA_ERR_STATE);
+        }
+      }
+    }
+    *iterations = maxiter;
+    return fail("conjugate gradients did not converge in " + std::to_string(maxiter) + " iterations",
+                DFA_ERR_STATE);
   });
 }
 

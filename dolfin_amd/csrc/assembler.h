@@ -182,4 +182,12 @@ struct dfa_assembler_s {
   // Dirichlet values shifted by x0 (dfa_assemble_system_x0), allocated on first use
   double* d_bc_x0 = nullptr;
   dfa::DeviceArena x0buf;
+  // conjugate-gradient work vectors and scalars (dfa_cg_solve), allocated on first use
+  double* d_cg_r = nullptr;
+  double* d_cg_p = nullptr;
+  double* d_cg_ap = nullptr;
+  double* d_cg_dinv = nullptr;
+  double* d_cg_part = nullptr;
+  dfa::CgState* d_cg_state = nullptr;
+  dfa::DeviceArena cg;
 };

@@ -160,6 +160,20 @@ constexpr int kNodeMaxNodes = 128;
 hipError_t launch_spmv(const CsrView& A, double alpha, const double* x, double beta, double* y,
                        hipStream_t s);
 hipError_t launch_diagonal(const CsrView& A, double* d, hipStream_t s);
+// Conjugate gradients on the device: scalars of one solve (device memory); the partials
+// buffer holds 3 * kCgMaxGroups doubles.
+struct CgState {
+  double rz, rr, bb, alpha, beta, tol2;
+  int64_t done;   // iteration at which |r|^2 <= tol2 (-1: not yet)
+  int32_t broke;  // p.Ap = 0 with r != 0
+  int32_t pad;
+};
+constexpr int kCgMaxGroups = 2048;
+hipError_t launch_cg_init(int64_t n, const double* b, const double* r, const double* dinv, double* p,
+                          double* part, CgState* st, hipStream_t s);
+hipError_t launch_cg_iteration(const CsrView& A, int64_t it, double* x, double* p, double* r, double* ap,
+                               const double* dinv, double* part, CgState* st, hipStream_t s);
+hipError_t launch_cg_jacobi(int64_t n, double* d, hipStream_t s);
 // The narrow-column arrays of the streamed SpMV (blk_col0 [nblocks], col16 [nnz]).
 hipError_t launch_narrow_columns(const CsrView& A, int32_t* blk_col0, uint16_t* col16, hipStream_t s);
 // Host-side block insertion (GenericMatrix::add_local): slot[i*n+j] = position of

@@ -267,6 +267,218 @@ __global__ __launch_bounds__(kLaThreads) void diagonal_kernel(int64_t nrows, con
   d[r] = v;
 }
 
+// ---- conjugate gradients -------------------------------------------------------------------
+// Per iteration: Ap = A p (SpMV above), then vector kernels that each write one partial sum
+// per workgroup, each followed by a one-workgroup kernel that sums the partials in index
+// order (deterministic; the kernel boundary makes them visible, so no device-scope fence is
+// needed, which on several XCDs would write back L2) and updates the scalars in CgState. A
+// converged or broken-down state makes every later kernel return at once, so the iterate is
+// frozen until the host reads the state.
+constexpr int kCgThreads = 256;
+
+__device__ __forceinline__ double cg_block_sum(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kCgThreads / 64; ++w) t += red[w];
+  __syncthreads();
+  return t;  // every thread
+}
+
+// the workgroup's NV sums into part[q * gridDim.x + blockIdx.x]
+template <int NV>
+__device__ __forceinline__ void cg_partials(const double (&v)[NV], double* __restrict__ part) {
+  __shared__ double red[kCgThreads / 64];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const double t = cg_block_sum(v[q], red);
+    if (threadIdx.x == 0) part[static_cast<int64_t>(q) * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// the sums over g workgroups' partials, in workgroup order (one workgroup; every thread)
+template <int NV>
+__device__ __forceinline__ void cg_totals(const double* __restrict__ part, int g, double (&tot)[NV]) {
+  __shared__ double red[kCgThreads / 64];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double a = 0.0;
+    for (int w = threadIdx.x; w < g; w += kCgThreads) a += part[static_cast<int64_t>(q) * g + w];
+    tot[q] = cg_block_sum(a, red);
+  }
+}
+
+__global__ __launch_bounds__(kCgThreads) void cg_init_finish(const double* __restrict__ part, int g, CgState* st) {
+  double tot[3];
+  cg_totals<3>(part, g, tot);
+  if (threadIdx.x == 0) {
+    st->rz = tot[0];
+    st->rr = tot[1];
+    st->bb = tot[2];
+  }
+}
+
+// alpha = rz / pAp (0 and breakdown when pAp = 0)
+__global__ __launch_bounds__(kCgThreads) void cg_alpha_finish(const double* __restrict__ part, int g, CgState* st) {
+  if (st->done >= 0 || st->broke) return;
+  double tot[1];
+  cg_totals<1>(part, g, tot);
+  if (threadIdx.x == 0) {
+    if (tot[0] == 0.0) st->broke = 1;
+    st->alpha = tot[0] != 0.0 ? st->rz / tot[0] : 0.0;
+  }
+}
+
+// beta = rz' / rz; converged at `it` when rr <= tol^2
+__global__ __launch_bounds__(kCgThreads) void cg_beta_finish(const double* __restrict__ part, int g, int64_t it,
+                                                             CgState* st) {
+  if (st->done >= 0 || st->broke) return;
+  double tot[2];
+  cg_totals<2>(part, g, tot);
+  if (threadIdx.x == 0) {
+    st->beta = st->rz != 0.0 ? tot[0] / st->rz : 0.0;
+    st->rz = tot[0];
+    st->rr = tot[1];
+    if (tot[1] <= st->tol2) st->done = it;
+  }
+}
+
+// The vector kernels take four elements per thread at a time (stride = the grid): all loads
+// of a batch are issued before the arithmetic and the stores, then a remainder loop. J: the
+// Jacobi preconditioner (z = dinv r), else z = r.
+#define CG_LOOP(i, s, n)                                                              \
+  const int64_t s = static_cast<int64_t>(gridDim.x) * kCgThreads;                     \
+  int64_t i = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x;            \
+  for (; i + 3 * s < n; i += 4 * s)
+
+// r = b - A x is in r: z = D r, p = z; partials of rz = r.z, rr = r.r, bb = b.b
+template <bool J>
+__global__ __launch_bounds__(kCgThreads) void cg_init_kernel(int64_t n, const double* __restrict__ b,
+                                                             const double* __restrict__ r,
+                                                             const double* __restrict__ dinv, double* __restrict__ p,
+                                                             double* __restrict__ part) {
+  double v[3] = {0.0, 0.0, 0.0};
+  CG_LOOP(i, s, n) {
+    double rv[4], bv[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      rv[u] = r[i + u * s];
+      bv[u] = b[i + u * s];
+      dv[u] = J ? dinv[i + u * s] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double zi = J ? dv[u] * rv[u] : rv[u];
+      p[i + u * s] = zi;
+      v[0] = fma(rv[u], zi, v[0]);
+      v[1] = fma(rv[u], rv[u], v[1]);
+      v[2] = fma(bv[u], bv[u], v[2]);
+    }
+  }
+  for (; i < n; i += s) {
+    const double ri = r[i], zi = J ? dinv[i] * ri : ri;
+    p[i] = zi;
+    v[0] = fma(ri, zi, v[0]);
+    v[1] = fma(ri, ri, v[1]);
+    v[2] = fma(b[i], b[i], v[2]);
+  }
+  cg_partials<3>(v, part);
+}
+
+// partials of p.Ap
+__global__ __launch_bounds__(kCgThreads) void cg_dot_kernel(int64_t n, const double* __restrict__ p,
+                                                            const double* __restrict__ ap, double* __restrict__ part,
+                                                            const CgState* st) {
+  if (st->done >= 0 || st->broke) return;
+  double v[1] = {0.0};
+  CG_LOOP(i, s, n) {
+    double pv[4], av[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      pv[u] = p[i + u * s];
+      av[u] = ap[i + u * s];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[0] = fma(pv[u], av[u], v[0]);
+  }
+  for (; i < n; i += s) v[0] = fma(p[i], ap[i], v[0]);
+  cg_partials<1>(v, part);
+}
+
+// x += alpha p, r -= alpha Ap; partials of rz' = r.(D r) and rr = r.r
+template <bool J>
+__global__ __launch_bounds__(kCgThreads) void cg_update_kernel(int64_t n, double* __restrict__ x,
+                                                               const double* __restrict__ p,
+                                                               double* __restrict__ r,
+                                                               const double* __restrict__ ap,
+                                                               const double* __restrict__ dinv,
+                                                               double* __restrict__ part, const CgState* st) {
+  if (st->done >= 0 || st->broke) return;
+  const double alpha = st->alpha;
+  double v[2] = {0.0, 0.0};
+  CG_LOOP(i, s, n) {
+    double xv[4], pv[4], rv[4], av[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xv[u] = x[i + u * s];
+      pv[u] = p[i + u * s];
+      rv[u] = r[i + u * s];
+      av[u] = ap[i + u * s];
+      dv[u] = J ? dinv[i + u * s] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[i + u * s] = fma(alpha, pv[u], xv[u]);
+      const double ri = fma(-alpha, av[u], rv[u]);
+      r[i + u * s] = ri;
+      const double zi = J ? dv[u] * ri : ri;
+      v[0] = fma(ri, zi, v[0]);
+      v[1] = fma(ri, ri, v[1]);
+    }
+  }
+  for (; i < n; i += s) {
+    x[i] = fma(alpha, p[i], x[i]);
+    const double ri = fma(-alpha, ap[i], r[i]);
+    r[i] = ri;
+    const double zi = J ? dinv[i] * ri : ri;
+    v[0] = fma(ri, zi, v[0]);
+    v[1] = fma(ri, ri, v[1]);
+  }
+  cg_partials<2>(v, part);
+}
+
+// p = D r + beta p
+template <bool J>
+__global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(int64_t n, const double* __restrict__ r,
+                                                            const double* __restrict__ dinv, double* __restrict__ p,
+                                                            const CgState* st) {
+  if (st->done >= 0 || st->broke) return;
+  const double beta = st->beta;
+  CG_LOOP(i, s, n) {
+    double rv[4], pv[4], dv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      rv[u] = r[i + u * s];
+      pv[u] = p[i + u * s];
+      dv[u] = J ? dinv[i + u * s] : 1.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) p[i + u * s] = fma(beta, pv[u], J ? dv[u] * rv[u] : rv[u]);
+  }
+  for (; i < n; i += s) p[i] = fma(beta, p[i], J ? dinv[i] * r[i] : r[i]);
+}
+#undef CG_LOOP
+
+// dinv = 1 / d where d != 0, else 1 (the Jacobi preconditioner)
+__global__ __launch_bounds__(kCgThreads) void cg_jacobi_kernel(int64_t n, double* __restrict__ d) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kCgThreads + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kCgThreads)
+    d[i] = d[i] != 0.0 ? 1.0 / d[i] : 1.0;
+}
+
 // Newton-increment Dirichlet values: out = x0 - g on Dirichlet dofs, 0 elsewhere.
 __global__ __launch_bounds__(kLaThreads) void dirichlet_x0_kernel(int64_t n,
                                                                   const uint8_t* __restrict__ mask,
@@ -328,6 +540,47 @@ hipError_t launch_block_add(int64_t nk, const int32_t* slot, const double* block
   if (nk <= 0) return hipSuccess;
   hipLaunchKernelGGL(block_add_kernel, dim3(static_cast<unsigned>((nk + kLaThreads - 1) / kLaThreads)),
                      dim3(kLaThreads), 0, s, nk, slot, block, values);
+  return hipGetLastError();
+}
+
+int cg_grid(int64_t n) {
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 4 * kCgThreads - 1) / (4 * kCgThreads),
+                                                                 kCgMaxGroups)));
+}
+
+hipError_t launch_cg_init(int64_t n, const double* b, const double* r, const double* dinv, double* p,
+                          double* part, CgState* st, hipStream_t s) {
+  const int g = cg_grid(n);
+  if (dinv)
+    hipLaunchKernelGGL(cg_init_kernel<true>, dim3(g), dim3(kCgThreads), 0, s, n, b, r, dinv, p, part);
+  else
+    hipLaunchKernelGGL(cg_init_kernel<false>, dim3(g), dim3(kCgThreads), 0, s, n, b, r, dinv, p, part);
+  hipLaunchKernelGGL(cg_init_finish, dim3(1), dim3(kCgThreads), 0, s, part, g, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_iteration(const CsrView& A, int64_t it, double* x, double* p, double* r, double* ap,
+                               const double* dinv, double* part, CgState* st, hipStream_t s) {
+  const int64_t n = A.nrows;
+  hipError_t e = launch_spmv(A, 1.0, p, 0.0, ap, s);
+  if (e != hipSuccess) return e;
+  const int g = cg_grid(n);
+  hipLaunchKernelGGL(cg_dot_kernel, dim3(g), dim3(kCgThreads), 0, s, n, p, ap, part, st);
+  hipLaunchKernelGGL(cg_alpha_finish, dim3(1), dim3(kCgThreads), 0, s, part, g, st);
+  if (dinv)
+    hipLaunchKernelGGL(cg_update_kernel<true>, dim3(g), dim3(kCgThreads), 0, s, n, x, p, r, ap, dinv, part, st);
+  else
+    hipLaunchKernelGGL(cg_update_kernel<false>, dim3(g), dim3(kCgThreads), 0, s, n, x, p, r, ap, dinv, part, st);
+  hipLaunchKernelGGL(cg_beta_finish, dim3(1), dim3(kCgThreads), 0, s, part, g, it, st);
+  if (dinv)
+    hipLaunchKernelGGL(cg_dir_kernel<true>, dim3(g), dim3(kCgThreads), 0, s, n, r, dinv, p, st);
+  else
+    hipLaunchKernelGGL(cg_dir_kernel<false>, dim3(g), dim3(kCgThreads), 0, s, n, r, dinv, p, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_cg_jacobi(int64_t n, double* d, hipStream_t s) {
+  hipLaunchKernelGGL(cg_jacobi_kernel, dim3(cg_grid(n)), dim3(kCgThreads), 0, s, n, d);
   return hipGetLastError();
 }
 

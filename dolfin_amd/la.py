@@ -8,6 +8,8 @@ EigenMatrix.cpp:230-255) and the diagonal. solve() is dolfin.solve(A, x, b, "cg"
 conjugate gradients as EigenKrylovSolver runs them, with every vector in HBM.
 """
 
+import ctypes
+
 import numpy as np
 import torch
 
@@ -157,9 +159,11 @@ def solve(A, x, b, method="cg", preconditioner="jacobi", rtol=1e-10, atol=0.0, m
     Returns that iteration count (dolfin::solve returns the same count); raises when the method
     does not converge or breaks down (p.Ap = 0 with r != 0).
 
-    The residual is tested on the device every iteration; the host reads the result every
-    check_every iterations, and the iterate is frozen from the converged iteration on, so the
-    answer and the count do not depend on check_every."""
+    The iteration runs in the library (dfa_cg_solve: the SpMV and three fused vector kernels
+    per iteration, deterministic reductions); the residual is tested on the device every
+    iteration, the host reads the result every check_every iterations, and the iterate is
+    frozen from the converged iteration on, so the answer and the count do not depend on
+    check_every."""
     if method != "cg":
         raise ValueError("only the conjugate gradient method is provided")
     if preconditioner not in ("jacobi", "none", "default"):
@@ -167,47 +171,14 @@ def solve(A, x, b, method="cg", preconditioner="jacobi", rtol=1e-10, atol=0.0, m
     xv = x.values if isinstance(x, Vector) else x
     bv = b.values if isinstance(b, Vector) else b
     n = A.size()
-    if maxiter is None:
-        maxiter = 10 * n
-    if preconditioner == "none":
-        dinv = None
-    else:
-        d = A.diagonal()
-        dinv = torch.where(d != 0, 1.0 / d, torch.ones_like(d))
-    r = torch.empty_like(bv)
-    r.copy_(bv)
-    A.mult(xv, r, alpha=-1.0, beta=1.0)          # r = b - A x
-    tol = max(rtol * float(torch.linalg.vector_norm(bv)), atol)
-    tol2 = tol * tol
-    if float(torch.dot(r, r)) <= tol2:
-        return 0
-    z = r * dinv if dinv is not None else r.clone()
-    p = z.clone()
-    rz = torch.dot(r, z)
-    Ap = torch.empty_like(bv)
-    zero = torch.zeros((), dtype=bv.dtype, device=bv.device)
-    done = torch.full((), -1, dtype=torch.int64, device=bv.device)    # converged iteration
-    broke = torch.zeros((), dtype=torch.bool, device=bv.device)
-    for it in range(1, maxiter + 1):
-        active = done < 0
-        A.mult(p, Ap)
-        pAp = torch.dot(p, Ap)
-        ok = active & (pAp != 0)
-        broke |= active & (pAp == 0)
-        alpha = torch.where(ok, rz / torch.where(ok, pAp, 1.0), zero)
-        xv.add_(alpha * p)
-        r.sub_(alpha * Ap)
-        z = r * dinv if dinv is not None else r
-        rz_new = torch.dot(r, z)
-        beta = torch.where(rz != 0, rz_new / torch.where(rz != 0, rz, 1.0), zero)
-        p.mul_(beta).add_(z)
-        rz = rz_new
-        rr = torch.dot(r, r)
-        done = torch.where(active & ~broke & (rr <= tol2), torch.full_like(done, it), done)
-        if it % check_every == 0 or it == maxiter:
-            dn = int(done)
-            if dn >= 0:
-                return dn
-            if bool(broke):
-                raise RuntimeError(f"conjugate gradients broke down (p.Ap = 0) at iteration <= {it}")
-    raise RuntimeError(f"conjugate gradients did not converge in {maxiter} iterations")
+    for t, name in ((xv, "x"), (bv, "b")):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == n):
+            raise ValueError(f"{name} must be a contiguous float64 device vector of size {n}")
+    its = ctypes.c_int64(0)
+    rc = _lib.load().dfa_cg_solve(A._engine.handle, _lib.ptr(A.values), _lib.ptr(xv), _lib.ptr(bv),
+                                  int(preconditioner != "none"), float(rtol), float(atol),
+                                  -1 if maxiter is None else int(maxiter), int(check_every),
+                                  ctypes.byref(its), _lib.stream())
+    if rc != 0:
+        raise RuntimeError(_lib.load().dfa_last_error().decode())
+    return int(its.value)

@@ -308,6 +308,24 @@ int dfa_vector_add_local(double* d_b, int64_t size, int32_t m, const int32_t* h_
    preconditioner of EigenKrylovSolver (dolfin/la/EigenKrylovSolver.cpp:52-56). */
 int dfa_csr_diagonal(dfa_assembler h, const double* d_values, double* d_diag, void* stream);
 
+/*
+ * Conjugate gradients on the device for the matrix d_values on the handle's structure: the
+ * "cg" method of EigenKrylovSolver with the "jacobi" preconditioner (jacobi != 0: diag(A)^-1,
+ * 1 where the diagonal is 0) or none (dolfin/la/EigenKrylovSolver.cpp:43-56, reached through
+ * dolfin::solve(A, x, b, "cg", "jacobi")). d_x (device) holds the initial guess and receives
+ * the solution; d_b is not modified. Stops at the first iteration k with
+ * |r_k| <= max(rtol |b|, atol) and sets *iterations = k (0 when the initial residual already
+ * satisfies it). Fails with DFA_ERR_STATE on breakdown (p.Ap = 0 with r != 0) or when maxiter
+ * iterations (maxiter < 0: 10 x rows) do not converge, *iterations then holding the iterations
+ * done. The convergence state lives on the device and is read every check_every iterations;
+ * the iterate is frozen from the converged iteration on, so neither the answer nor the count
+ * depends on check_every. Reductions are summed in a fixed order (deterministic). The work
+ * vectors (4 x rows doubles) are allocated on the handle's first solve and kept. Synchronous.
+ */
+int dfa_cg_solve(dfa_assembler h, const double* d_values, double* d_x, const double* d_b, int32_t jacobi,
+                 double rtol, double atol, int64_t maxiter, int32_t check_every, int64_t* iterations,
+                 void* stream);
+
 /* Device properties the roofline figures rest on (no counterpart in the reference):
    info[0] compute units, [1] peak engine clock (kHz), [2] peak memory clock (kHz),
    [3] memory bus width (bits), [4] L2 size (bytes), [5] device memory (bytes). */

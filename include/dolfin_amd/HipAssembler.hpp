@@ -179,6 +179,15 @@ class DeviceCSRMatrix {
   void diagonal(double* d, hipStream_t s = nullptr) const {
     check(dfa_csr_diagonal(space_->handle(), values_.get(), d, s));
   }
+  // Conjugate gradients A x = b on the device (x: initial guess in, solution out; both
+  // size(0) doubles in device memory), dolfin::solve(A, x, b, "cg", "jacobi" | "none"); returns
+  // the converged iteration, throws on breakdown or when maxiter iterations do not converge.
+  int64_t solve_cg(double* x, const double* b, bool jacobi = true, double rtol = 1e-10, double atol = 0.0,
+                   int64_t maxiter = -1, hipStream_t s = nullptr) const {
+    int64_t its = 0;
+    check(dfa_cg_solve(space_->handle(), values_.get(), x, b, jacobi ? 1 : 0, rtol, atol, maxiter, 10, &its, s));
+    return its;
+  }
   // CSR export (EigenMatrix::data): row_ptr, col, values on the host.
   void data(std::vector<int32_t>& row_ptr, std::vector<int32_t>& col, std::vector<double>& vals) const {
     DeviceArray<int32_t> rp(static_cast<size_t>(space_->num_rows() + 1)), cc(static_cast<size_t>(space_->nnz()));

@@ -194,6 +194,28 @@ int main() {
     for (int i = 1; i < n; ++i) dok &= close(hd[i], 2.0 * n);
     std::printf("UnitIntervalMesh(48) P1: max|A 1| = %.3e, diagonal %s\n", ymax, dok ? "ok" : "wrong");
     ok &= ymax < 1e-10 && dok;
+    // Jacobi CG on the consistent system A x = A u (A is singular: the constants are its null
+    // space, so the check is the residual, not x itself)
+    std::vector<double> hu(n + 1);
+    for (int i = 0; i <= n; ++i) hu[i] = std::sin(3.0 * i / n);
+    DeviceArray<double> du;
+    du.upload(hu.data(), hu.size());
+    DeviceVector rhs, sol, chk;
+    rhs.init(n + 1);
+    sol.init(n + 1);
+    chk.init(n + 1);
+    A.mult(du.get(), rhs.values());
+    const int64_t its = A.solve_cg(sol.values(), rhs.values(), true, 1e-12);
+    A.mult(sol.values(), chk.values());
+    const std::vector<double> hr = rhs.get_local(), hc = chk.get_local();
+    double rmax = 0.0, bmax = 0.0;
+    for (int i = 0; i <= n; ++i) {
+      rmax = std::fmax(rmax, std::fabs(hc[i] - hr[i]));
+      bmax = std::fmax(bmax, std::fabs(hr[i]));
+    }
+    std::printf("UnitIntervalMesh(48) P1: CG %lld iterations, max|A x - b| / max|b| = %.3e\n",
+                static_cast<long long>(its), rmax / bmax);
+    ok &= its > 0 && its <= n + 1 && rmax / bmax < 1e-9;
   }
   std::printf(ok ? "adapter checks passed\n" : "adapter checks FAILED\n");
   return ok ? 0 : 1;

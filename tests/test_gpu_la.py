@@ -195,3 +195,66 @@ def test_cg_diagonal_system_stops_exactly():
         sols.append(x.get_local())
     assert len(its) == 1
     assert all(np.array_equal(sols[0], s) for s in sols[1:])
+
+
+def _cg_reference(S, b, jacobi, rtol):
+    """Preconditioned conjugate gradients in numpy, the algorithm the device solver runs."""
+    d = S.diagonal()
+    dinv = np.where(d != 0, 1.0 / np.where(d != 0, d, 1.0), 1.0) if jacobi else np.ones_like(b)
+    x = np.zeros_like(b)
+    r = b.copy()
+    tol2 = (rtol * np.linalg.norm(b)) ** 2
+    z = dinv * r
+    p = z.copy()
+    rz = r @ z
+    for it in range(1, 10 * len(b) + 1):
+        ap = S @ p
+        alpha = rz / (p @ ap)
+        x += alpha * p
+        r -= alpha * ap
+        z = dinv * r
+        rz_new = r @ z
+        if r @ r <= tol2:
+            return x, it
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    raise AssertionError("reference CG did not converge")
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "none"])
+def test_cg_matches_reference_iterations(precond):
+    # the device solver against the same iteration in numpy: the converged iteration agrees up
+    # to the summation order of the reductions (at most one step), and so does the solution
+    print(f"case cg reference {precond}", flush=True)
+    mesh, V, kappa, f = workloads.p1_poisson_3d(9, seed=21)
+    bc = dfa.DirichletBC(V, 0.25)
+    A, b = dfa.assemble_system(dfa.poisson_form(V, workloads.to_function(V, kappa)),
+                               dfa.load_form(V, workloads.to_function(V, f)), bc)
+    x = dfa.Vector(V.dim())
+    it = dfa.solve(A, x, b, "cg", precond, rtol=1e-11)
+    xr, itr = _cg_reference(_scipy(A), b.get_local(), precond == "jacobi", 1e-11)
+    assert abs(it - itr) <= 1
+    assert _rel(x.get_local(), xr) < 1e-9
+
+
+def test_cg_vector_p2_elasticity():
+    # elasticity with the boundary clamped: the node-blocked SpMV inside the solver
+    print("case cg vector P2", flush=True)
+    mesh, V, mu, lmbda, f3 = workloads.vp2_elasticity_3d(3, seed=4)
+    Q = V.coefficient_space()
+    a = dfa.elasticity_form(V, workloads.to_function(Q, mu), workloads.to_function(Q, lmbda))
+    A, b = dfa.assemble_system(a, dfa.body_force_form(V, f3), dfa.DirichletBC(V, 0.0))
+    x = dfa.Vector(V.dim())
+    it = dfa.solve(A, x, b, "cg", "jacobi", rtol=1e-12)
+    ref = spla.spsolve(_scipy(A).tocsc(), b.get_local())
+    assert it > 0
+    assert _rel(x.get_local(), ref) < 1e-9
+
+
+def test_cg_not_converged_raises():
+    mesh, V, kappa, f = workloads.p1_poisson_3d(6, seed=22)
+    A, b = dfa.assemble_system(dfa.poisson_form(V, workloads.to_function(V, kappa)),
+                               dfa.load_form(V, workloads.to_function(V, f)), dfa.DirichletBC(V, 0.0))
+    x = dfa.Vector(V.dim())
+    with pytest.raises(RuntimeError, match="did not converge in 3 iterations"):
+        dfa.solve(A, x, b, "cg", "jacobi", rtol=1e-14, maxiter=3)
