@@ -325,6 +325,23 @@ def main():
                                 "workload": f"P1 kappa-Poisson stiffness matrix alone, {args.n}^3"}
             del Am
             log("matrix-only pass timed")
+            # SystemAssembler with a Dirichlet condition on the whole boundary (apply_bc per cell,
+            # SystemAssembler.cpp:1036-1112): the same forms, the same bytes plus the mask
+            bc = dfa.DirichletBC(V, 0.5)
+            asm_bc = dfa.SystemAssembler(a, L, bc, scatter=scatter)
+            Abc, bbc = dfa.Matrix(), dfa.Vector()
+            asm_bc.assemble(Abc, bbc)
+            w_bc, ev_bc, _ = timed_steps(lambda: asm_bc.assemble(Abc, bbc), args.steps, args.warmup,
+                                         settle_s=args.settle)
+            bc_ms = float(np.mean(ev_bc))
+            sec["p1_system_bc"] = {"ms": bc_ms, "cells_per_s": nc / (bc_ms * 1e-3),
+                                   "hbm_frac": alg_bytes / (bc_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                   "kernel": eng.pass_kernel(dfa._lib.FORM_P1_POISSON, dfa._lib.FORM_P1_LOAD,
+                                                             has_bc=True, scatter=scatter),
+                                   "workload": f"P1 kappa-Poisson + load with a Dirichlet condition on the "
+                                               f"boundary ({len(bc.dofs)} dofs), {args.n}^3"}
+            del Abc, bbc
+            log("Dirichlet system timed")
         if want("adjoint"):
             def adj():
                 adjoint._run(V, dfa._lib.FORM_P1_POISSON, xx, kk, None, ff, None, uu)

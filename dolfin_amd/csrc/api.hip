@@ -568,6 +568,22 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
         }
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");  // host vectors die here
       }
+      if (element == DFA_ELEMENT_P1) {  // Dirichlet conditions after assembly (edge chains)
+        void* d = nullptr;
+        hip_check(hipMalloc(&d, std::max<size_t>(static_cast<size_t>(ndofs) * sizeof(int32_t), 16)), "hipMalloc");
+        h->d_dof_cells = h->base.adopt(static_cast<int32_t*>(d), ndofs * 4);
+        hip_check(hipMemsetAsync(d, 0, static_cast<size_t>(ndofs) * sizeof(int32_t), s), "hipMemsetAsync");
+        hip_check(dfa::launch_dof_cell_count(nc * nd, h->d_cdofs, h->d_dof_cells, s), "dof_cell_count_kernel");
+        void* t = nullptr;
+        hip_check(hipMalloc(&t, std::max<size_t>(static_cast<size_t>(h->nnz) * sizeof(int32_t), 16)), "hipMalloc");
+        h->d_tmap = h->base.adopt(static_cast<int32_t*>(t), h->nnz * 4);
+        dfa::CsrView A;
+        A.nrows = ndofs;
+        A.nnz = h->nnz;
+        A.row_ptr = h->d_row_ptr;
+        A.col = h->d_col;
+        hip_check(dfa::launch_transpose_map(A, h->d_tmap, s), "transpose_map_kernel");
+      }
       if (element == DFA_ELEMENT_P1) {
         if (h_x) {
           const char* ce = std::getenv("DFA_P1_CHAINS");
@@ -888,6 +904,19 @@ static void ensure_colors(dfa_assembler h, hipStream_t s) {
 // One fused pass over the cells (Assembler::assemble_cells / SystemAssembler::cell_wise_assembly).
 // With name_only non-NULL nothing is launched: the name of the kernel the pass would run is
 // written there instead (pointer arguments are then only tested for NULL).
+// The Dirichlet rows of an assembled system (dirichlet_rows_kernel; the cells-per-dof counts
+// are built at creation, so this allocates nothing).
+static void apply_dirichlet_rows(dfa_assembler h, const uint8_t* bcm, const double* bcv, double* values,
+                                 double* b, hipStream_t s) {
+  dfa::CsrView A;
+  A.nrows = h->ndofs;
+  A.nnz = h->nnz;
+  A.row_ptr = h->d_row_ptr;
+  A.col = h->d_col;
+  hip_check(dfa::launch_dirichlet_rows(A, h->d_tmap, bcm, bcv, h->d_dof_cells, values, b, s),
+            "dirichlet_rows_kernel");
+}
+
 static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const double* x,
                     const double* a_c0, const double* a_c1, const double* L_c0, const double* f3,
                     const uint8_t* bcm, const double* bcv, double* values, double* b,
@@ -921,10 +950,30 @@ static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const doubl
     p.values = values;
     p.b = b;
     const int sc = resolve_scatter(h, scatter);
+    // Dirichlet conditions after assembly: the edge-chain kernel assembles the system without
+    // them, then dirichlet_rows_kernel applies what the per-cell elimination sums up to (not
+    // for add_values, whose earlier values must stay untouched; DFA_BC_INLINE=1 keeps the
+    // per-cell elimination inside the blocked kernel)
+    dfa::P1Pass q = p;
+    q.bc = false;
+    q.bc_mask = nullptr;
+    q.bc_value = nullptr;
+    const bool bc_after = bc && !add && values && sc == DFA_SCATTER_BLOCKED && h->has_plan && h->d_tmap &&
+                          dfa::p1_chains_apply(h->btopo, q) && !env_flag("DFA_BC_INLINE");
     if (name_only) {
       if (sc == DFA_SCATTER_ATOMIC) *name_only = "p1_atomic_kernel";
       else if (sc == DFA_SCATTER_COLORED) *name_only = "p1_colored_kernel";
+      else if (bc_after) *name_only = dfa::p1_blocked_kernel_name(h->btopo, q) + " + dirichlet_rows_kernel";
       else *name_only = dfa::p1_blocked_kernel_name(h->btopo, p);
+      return DFA_OK;
+    }
+    if (bc_after) {
+      if (!h->rows_covered) {
+        hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
+        if (b) hip_check(dfa::launch_fill_zero(b, h->ndofs, s), "fill_zero");
+      }
+      hip_check(dfa::launch_p1_blocked(h->btopo, q, s), "p1_chain_kernel");
+      apply_dirichlet_rows(h, bcm, bcv, values, b, s);
       return DFA_OK;
     }
     if (sc == DFA_SCATTER_ATOMIC) {

@@ -97,6 +97,8 @@ hipError_t launch_p1_color_gather(int64_t num_cells, int32_t nvc, const int32_t*
 hipError_t launch_p1_blocked(const P1BlockTopo& t, const P1Pass& p, hipStream_t s);
 size_t p1_blocked_lds_bytes(const P1BlockTopo& t, const P1Pass& p);
 // Name and template arguments of the kernel launch_p1_blocked runs for (t, p).
+// True when launch_p1_blocked runs the edge-chain kernel for this pass.
+bool p1_chains_apply(const P1BlockTopo& t, const P1Pass& p);
 std::string p1_blocked_kernel_name(const P1BlockTopo& t, const P1Pass& p);
 hipError_t launch_fill_zero(double* p, int64_t n, hipStream_t s);
 
@@ -160,6 +162,13 @@ constexpr int kNodeMaxNodes = 128;
 hipError_t launch_spmv(const CsrView& A, double alpha, const double* x, double beta, double* y,
                        hipStream_t s);
 hipError_t launch_diagonal(const CsrView& A, double* d, hipStream_t s);
+// Dirichlet conditions applied to an assembled square system as SystemAssembler's per-cell
+// elimination sums up (cnt: cells per dof, from launch_dof_cell_count); b may be NULL.
+hipError_t launch_dof_cell_count(int64_t nk, const int32_t* cell_dofs, int32_t* cnt, hipStream_t s);
+hipError_t launch_dirichlet_rows(const CsrView& A, const int32_t* tmap, const uint8_t* mask, const double* g,
+                                 const int32_t* cnt, double* values, double* b, hipStream_t s);
+// tmap[k] = the slot of A_ji for the entry k = A_ij (-1 when the pattern has none).
+hipError_t launch_transpose_map(const CsrView& A, int32_t* tmap, hipStream_t s);
 // Conjugate gradients on the device: scalars of one solve (device memory); the partials
 // buffer holds 3 * kCgMaxGroups doubles.
 struct CgState {

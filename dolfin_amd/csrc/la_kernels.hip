@@ -479,6 +479,87 @@ __global__ __launch_bounds__(kCgThreads) void cg_jacobi_kernel(int64_t n, double
     d[i] = d[i] != 0.0 ? 1.0 / d[i] : 1.0;
 }
 
+// Number of cells holding each dof (the diagonal SystemAssembler leaves on a Dirichlet row:
+// 1 per cell).
+__global__ __launch_bounds__(kLaThreads) void dof_cell_count_kernel(int64_t nk, const int32_t* __restrict__ cd,
+                                                                   int32_t* __restrict__ cnt) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kLaThreads + threadIdx.x;
+  if (k < nk) atomicAdd(&cnt[cd[k]], 1);
+}
+
+// SystemAssembler's per-cell Dirichlet elimination (SystemAssembler::apply_bc,
+// SystemAssembler.cpp:1051-1075) summed over the cells, applied to the assembled square system
+// (A, b) without it: a Dirichlet row i becomes cnt(i) on the diagonal and 0 elsewhere with
+// b_i = cnt(i) g_i (every cell holding i sets its element row to e_i and b_e,i = g_i); for a
+// free row j, the entry A_ji of a Dirichlet column moves to the right-hand side,
+// b_j -= A_ji g_i, and becomes 0 (the element columns are zeroed after the lifting). The
+// free rows' diagonals are unchanged. A fixed grid of 16-lane groups; group q owns the rows
+// q, q + G, q + 2G, ... (G groups), so the contiguous Dirichlet rows of a mesh face spread over
+// many groups. A group reads the masks of 16 of its rows at once (one per lane), then works
+// through its Dirichlet rows among them with one entry per lane; tmap[k] is the slot of A_ji
+// for the entry k = A_ij (the sparsity is symmetric). Free rows take several liftings:
+// atomic additions.
+__global__ __launch_bounds__(kLaThreads) void dirichlet_rows_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                                                   const int32_t* __restrict__ col,
+                                                                   const int32_t* __restrict__ tmap,
+                                                                   const uint8_t* __restrict__ mask,
+                                                                   const double* __restrict__ g,
+                                                                   const int32_t* __restrict__ cnt,
+                                                                   double* __restrict__ values,
+                                                                   double* __restrict__ b) {
+  const int ln = static_cast<int>(threadIdx.x & 15);
+  const int shift = static_cast<int>(threadIdx.x & 48);  // the group's 16 bits in the wavefront ballot
+  const int64_t G = static_cast<int64_t>(gridDim.x) * (kLaThreads / 16);
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kLaThreads / 16) + (threadIdx.x >> 4);
+  const int64_t rounds = (n + 16 * G - 1) / (16 * G);  // the same for every group
+  for (int64_t t = 0; t < rounds; ++t) {
+    const int64_t mine = q + (16 * t + ln) * G;
+    const bool on = mine < n && mask[mine];
+    uint32_t bits = static_cast<uint32_t>((__ballot(on) >> shift) & 0xffffu);
+    while (bits) {
+      const int64_t i = q + (16 * t + __ffs(bits) - 1) * G;
+      bits &= bits - 1;
+      const double gi = g[i];
+      const int32_t r1 = rp[i + 1];
+      for (int32_t k = rp[i] + ln; k < r1; k += 16) {
+        const int32_t j = col[k];
+        const int32_t tk = tmap[k];
+        if (j == i) {
+          values[k] = static_cast<double>(cnt[i]);
+          continue;
+        }
+        values[k] = 0.0;
+        if (mask[j] || tk < 0) continue;
+        const double w = values[tk];
+        values[tk] = 0.0;
+        if (b) unsafeAtomicAdd(b + j, -w * gi);
+      }
+      if (b && ln == 0) b[i] = static_cast<double>(cnt[i]) * gi;
+    }
+  }
+}
+
+// tmap[k] = slot of A_ji for the entry k = A_ij (-1 when the pattern has no A_ji): 16 lanes
+// per row, a binary search in row j per entry.
+__global__ __launch_bounds__(kLaThreads) void transpose_map_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                                                  const int32_t* __restrict__ col,
+                                                                  int32_t* __restrict__ tmap) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * (kLaThreads / 16) + (threadIdx.x >> 4);
+  if (i >= n) return;
+  const int ln = static_cast<int>(threadIdx.x & 15);
+  for (int32_t k = rp[i] + ln; k < rp[i + 1]; k += 16) {
+    const int32_t j = col[k];
+    int32_t lo = rp[j], hi = rp[j + 1];
+    const int32_t end = hi;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi) >> 1;
+      if (col[mid] < i) lo = mid + 1;
+      else hi = mid;
+    }
+    tmap[k] = lo < end && col[lo] == static_cast<int32_t>(i) ? lo : -1;
+  }
+}
+
 // Newton-increment Dirichlet values: out = x0 - g on Dirichlet dofs, 0 elsewhere.
 __global__ __launch_bounds__(kLaThreads) void dirichlet_x0_kernel(int64_t n,
                                                                   const uint8_t* __restrict__ mask,
@@ -581,6 +662,32 @@ hipError_t launch_cg_iteration(const CsrView& A, int64_t it, double* x, double* 
 
 hipError_t launch_cg_jacobi(int64_t n, double* d, hipStream_t s) {
   hipLaunchKernelGGL(cg_jacobi_kernel, dim3(cg_grid(n)), dim3(kCgThreads), 0, s, n, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_dof_cell_count(int64_t nk, const int32_t* cell_dofs, int32_t* cnt, hipStream_t s) {
+  if (nk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dof_cell_count_kernel, dim3(static_cast<unsigned>((nk + kLaThreads - 1) / kLaThreads)),
+                     dim3(kLaThreads), 0, s, nk, cell_dofs, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirichlet_rows(const CsrView& A, const int32_t* tmap, const uint8_t* mask, const double* g,
+                                 const int32_t* cnt, double* values, double* b, hipStream_t s) {
+  if (A.nrows == 0) return hipSuccess;
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((A.nrows + 16 * (kLaThreads / 16) - 1) /
+                                                                  (16 * (kLaThreads / 16)), 4096));
+  hipLaunchKernelGGL(dirichlet_rows_kernel, dim3(static_cast<unsigned>(grid)), dim3(kLaThreads), 0, s, A.nrows,
+                     A.row_ptr, A.col, tmap, mask, g, cnt, values, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_transpose_map(const CsrView& A, int32_t* tmap, hipStream_t s) {
+  if (A.nrows == 0) return hipSuccess;
+  const int64_t grid = (A.nrows + kLaThreads / 16 - 1) / (kLaThreads / 16);
+  if (grid > INT32_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(transpose_map_kernel, dim3(static_cast<unsigned>(grid)), dim3(kLaThreads), 0, s, A.nrows,
+                     A.row_ptr, A.col, tmap);
   return hipGetLastError();
 }
 

@@ -641,6 +641,8 @@ hipError_t launch_p1_color_gather(int64_t nc, int32_t nvc, const int32_t* perm,
   return hipGetLastError();
 }
 
+bool p1_chains_apply(const P1BlockTopo& t, const P1Pass& p) { return chains_fit(t, p); }
+
 std::string p1_blocked_kernel_name(const P1BlockTopo& t, const P1Pass& p) {
   // the choices of launch_blocked_t, launch_chain_t and launch_blocked_tu
   char buf[192];

@@ -276,7 +276,8 @@ def test_pass_kernel_names():
     eng = dfa.engine_for(V)
     P, Lf, M = _lib.FORM_P1_POISSON, _lib.FORM_P1_LOAD, _lib.FORM_P1_MASS
     assert eng.pass_kernel(P, -1).startswith("p1_chain_kernel<WANT_B=0")
-    assert eng.pass_kernel(P, Lf, has_bc=True).startswith("p1_blocked_kernel<D=3,AFORM=0,WANT_B=1,BC=1")
+    # Dirichlet conditions follow the edge-chain assembly (dirichlet_rows_kernel)
+    assert eng.pass_kernel(P, Lf, has_bc=True) == "p1_chain_kernel<WANT_B=1,T=512,RR=1> + dirichlet_rows_kernel"
     assert eng.pass_kernel(M, -1).startswith("p1_blocked_kernel<D=3,AFORM=2,WANT_B=0,BC=0")
     assert eng.pass_kernel(P, Lf, scatter=dfa.SCATTER_ATOMIC) == "p1_atomic_kernel"
     assert eng.pass_kernel(P, Lf, scatter=dfa.SCATTER_COLORED) == "p1_colored_kernel"
@@ -296,3 +297,59 @@ def test_boundary_dofs_vector_p2():
         cd = V.cell_dofs()
         expect = np.unique(cd[np.concatenate([on] * 3, axis=1)])
         np.testing.assert_array_equal(dfa.fem.boundary_facet_nodes(V), expect)
+
+
+@pytest.mark.parametrize("permuted", [False, True])
+@pytest.mark.parametrize("n", [3, 9])
+def test_p1_dirichlet_after_chains(oracle, monkeypatch, n, permuted):
+    # SystemAssembler with Dirichlet conditions on the boundary and on scattered interior dofs:
+    # the edge-chain kernel assembles without them and dirichlet_rows_kernel applies what the
+    # per-cell elimination (SystemAssembler.cpp:1051-1075) sums up to. Checked against the
+    # oracle's per-cell loop and against the per-cell elimination inside the blocked kernel
+    # (DFA_BC_INLINE=1), for assemble(A, b), assemble(A) and the incremental assemble(A, b, x0).
+    print(f"case dirichlet after chains n={n} permuted={permuted}", flush=True)
+    mesh, V, kappa, f = _p1_case(n, permuted, seed=60 + n)
+    rng = np.random.default_rng(61 + n)
+    dofs = np.union1d(dfa.DirichletBC(V, 0.0).dofs, rng.choice(V.dim(), V.dim() // 7, replace=False))
+    g = rng.uniform(-1, 1, V.dim())
+    bc = dfa.DirichletBC(V, g[dofs], dofs)
+    eng = dfa.engine_for(V)
+    assert eng.pass_kernel(_lib.FORM_P1_POISSON, _lib.FORM_P1_LOAD, has_bc=True).endswith(
+        "+ dirichlet_rows_kernel")
+    a = dfa.poisson_form(V, workloads.to_function(V, kappa))
+    L = dfa.load_form(V, workloads.to_function(V, f))
+    asm = dfa.SystemAssembler(a, L, bc)
+    A, b = dfa.Matrix(), dfa.Vector()
+    asm.assemble(A, b)
+    Aonly = dfa.Matrix()
+    asm.assemble(Aonly)
+    x0 = torch.from_numpy(rng.uniform(-1, 1, V.dim())).cuda()
+    Ax, bx = dfa.Matrix(), dfa.Vector()
+    asm.assemble(Ax, bx, x0)
+    torch.cuda.synchronize()
+    rp, col, vals = A.data()
+    mask = np.zeros(V.dim(), np.uint8)
+    mask[dofs] = 1
+    gv = np.zeros(V.dim())
+    gv[dofs] = g[dofs]
+    ra, rb = oracle.assemble_system(oracle.P1_POISSON, oracle.P1_LOAD, mesh.coordinates(), mesh.cells(),
+                                    V.cell_dofs(), rp, col, mask, gv, coeff_dofs=V.cell_dofs(),
+                                    a_c0=kappa, L_c0=f)
+    assert rel(vals, ra) < TOL
+    assert rel(b.get_local(), rb) < TOL
+    assert rel(Aonly.data()[2], ra) < TOL
+    gx = np.zeros(V.dim())
+    gx[dofs] = x0.cpu().numpy()[dofs] - g[dofs]
+    _, rbx = oracle.assemble_system(oracle.P1_POISSON, oracle.P1_LOAD, mesh.coordinates(), mesh.cells(),
+                                    V.cell_dofs(), rp, col, mask, gx, coeff_dofs=V.cell_dofs(),
+                                    a_c0=kappa, L_c0=f)
+    assert rel(Ax.data()[2], ra) < TOL
+    assert rel(bx.get_local(), rbx) < TOL
+    monkeypatch.setenv("DFA_BC_INLINE", "1")
+    assert eng.pass_kernel(_lib.FORM_P1_POISSON, _lib.FORM_P1_LOAD, has_bc=True).startswith(
+        "p1_blocked_kernel<D=3,AFORM=0,WANT_B=1,BC=1")
+    A2, b2 = dfa.Matrix(), dfa.Vector()
+    asm.assemble(A2, b2)
+    torch.cuda.synchronize()
+    assert rel(A2.data()[2], vals) < TOL
+    assert rel(b2.get_local(), b.get_local()) < TOL
