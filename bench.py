@@ -474,6 +474,22 @@ def main():
                 "algorithmic_bytes": alg2, "traffic": load_traffic(f"vp2_{sc2}_{args.p2_n}"),
                 "traffic_source": traffic_source(f"vp2_{sc2}_{args.p2_n}"),
                 "first_assembly_s": t2_first}
+            # SystemAssembler with a Dirichlet condition on the whole boundary: the blocked pass
+            # without it, then the node-graph passes that apply it to the assembled rows
+            bc2 = dfa.DirichletBC(V2, 0.0)
+            asm2_bc = dfa.SystemAssembler(a2, L2, bc2)
+            asm2_bc.assemble(A2, b2)
+            w_bc2, ev_bc2, _ = timed_steps(lambda: asm2_bc.assemble(A2, b2), max(5, args.steps // 5), 2,
+                                           settle_s=args.settle)
+            bc2_ms = float(np.mean(ev_bc2))
+            sec["vp2_system_bc"] = {"ms": bc2_ms, "cells_per_s": nc2 / (bc2_ms * 1e-3),
+                                    "ratio_to_no_bc": bc2_ms / ms2,
+                                    "kernel": e2.pass_kernel(dfa._lib.FORM_VP2_ELASTICITY,
+                                                             dfa._lib.FORM_VP2_BODY_FORCE, has_bc=True),
+                                    "workload": f"vector P2 elasticity + body force with a Dirichlet condition "
+                                                f"on the boundary ({len(bc2.dofs)} dofs), {args.p2_n}^3"}
+            del asm2_bc
+            log("vector P2 Dirichlet system timed")
             x2 = m2.coordinates_device()
             mu_t, la_t, u_t = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (mu, lmbda, u2))
             f3a = np.ascontiguousarray(f3, dtype=np.float64)

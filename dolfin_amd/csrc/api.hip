@@ -568,12 +568,19 @@ int dfa_assembler_create(dfa_assembler* out, int32_t gdim, int64_t nv, int64_t n
         }
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");  // host vectors die here
       }
-      if (element == DFA_ELEMENT_P1) {  // Dirichlet conditions after assembly (edge chains)
+      {  // Dirichlet conditions after assembly: cells per dof, and for P1 the transpose map
         void* d = nullptr;
         hip_check(hipMalloc(&d, std::max<size_t>(static_cast<size_t>(ndofs) * sizeof(int32_t), 16)), "hipMalloc");
         h->d_dof_cells = h->base.adopt(static_cast<int32_t*>(d), ndofs * 4);
         hip_check(hipMemsetAsync(d, 0, static_cast<size_t>(ndofs) * sizeof(int32_t), s), "hipMemsetAsync");
         hip_check(dfa::launch_dof_cell_count(nc * nd, h->d_cdofs, h->d_dof_cells, s), "dof_cell_count_kernel");
+        if (h->node_layout) {  // vector P2 on the node graph: Dirichlet bits and touched flags per node
+          void* nb = nullptr;
+          hip_check(hipMalloc(&nb, std::max<size_t>(static_cast<size_t>(2 * (ndofs / 3)), 16)), "hipMalloc");
+          h->d_node_bits = h->base.adopt(static_cast<uint8_t*>(nb), 2 * (ndofs / 3));
+        }
+      }
+      if (element == DFA_ELEMENT_P1) {
         void* t = nullptr;
         hip_check(hipMalloc(&t, std::max<size_t>(static_cast<size_t>(h->nnz) * sizeof(int32_t), 16)), "hipMalloc");
         h->d_tmap = h->base.adopt(static_cast<int32_t*>(t), h->nnz * 4);
@@ -901,11 +908,13 @@ static void ensure_colors(dfa_assembler h, hipStream_t s) {
   h->has_colors = true;
 }
 
-// One fused pass over the cells (Assembler::assemble_cells / SystemAssembler::cell_wise_assembly).
-// With name_only non-NULL nothing is launched: the name of the kernel the pass would run is
-// written there instead (pointer arguments are then only tested for NULL).
-// The Dirichlet rows of an assembled system (dirichlet_rows_kernel; the cells-per-dof counts
-// are built at creation, so this allocates nothing).
+// The Dirichlet rows of an assembled system: dirichlet_node_rows_kernel and its pull pass on a
+// vector-P2 structure expanded from the node graph, else dirichlet_rows_kernel (the cells-per-dof
+// counts and the per-node flags are allocated at creation, so this allocates nothing).
+static bool node_dirichlet(dfa_assembler h) {
+  return h->node_layout && h->d_node_bits && !env_flag("DFA_BC_ROWS");
+}
+
 static void apply_dirichlet_rows(dfa_assembler h, const uint8_t* bcm, const double* bcv, double* values,
                                  double* b, hipStream_t s) {
   dfa::CsrView A;
@@ -913,9 +922,22 @@ static void apply_dirichlet_rows(dfa_assembler h, const uint8_t* bcm, const doub
   A.nnz = h->nnz;
   A.row_ptr = h->d_row_ptr;
   A.col = h->d_col;
+  if (node_dirichlet(h)) {
+    A.node_layout = h->node_layout;
+    A.node_nnz = h->node_nnz;
+    A.node_rp = h->d_node_rp;
+    A.node_col = h->d_node_col;
+    hip_check(dfa::launch_dirichlet_nodes(A, bcm, bcv, h->d_dof_cells, h->d_node_bits, values, b, s),
+              "dirichlet_node_rows_kernel");
+    return;
+  }
   hip_check(dfa::launch_dirichlet_rows(A, h->d_tmap, bcm, bcv, h->d_dof_cells, values, b, s),
             "dirichlet_rows_kernel");
 }
+
+// One fused pass over the cells (Assembler::assemble_cells / SystemAssembler::cell_wise_assembly).
+// With name_only non-NULL nothing is launched: the name of the kernel the pass would run is
+// written there instead (pointer arguments are then only tested for NULL).
 
 static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const double* x,
                     const double* a_c0, const double* a_c1, const double* L_c0, const double* f3,
@@ -958,7 +980,8 @@ static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const doubl
     q.bc = false;
     q.bc_mask = nullptr;
     q.bc_value = nullptr;
-    const bool bc_after = bc && !add && values && sc == DFA_SCATTER_BLOCKED && h->has_plan && h->d_tmap &&
+    const bool bc_after = bc && !add && values && (!b || L_form >= 0) && sc == DFA_SCATTER_BLOCKED &&
+                          h->has_plan && h->d_tmap &&
                           dfa::p1_chains_apply(h->btopo, q) && !env_flag("DFA_BC_INLINE");
     if (name_only) {
       if (sc == DFA_SCATTER_ATOMIC) *name_only = "p1_atomic_kernel";
@@ -1034,8 +1057,29 @@ static int run_pass(dfa_assembler h, int32_t a_form, int32_t L_form, const doubl
   p.values = values;
   p.b = b;
   p.add = add;
+  // Dirichlet conditions after the blocked pass, on the assembled rows (what the per-cell
+  // elimination sums up to), as for P1
+  dfa::VP2Pass q = p;
+  q.bc = false;
+  q.bc_mask = nullptr;
+  q.bc_value = nullptr;
+  const bool bc_after = bc && !add && p.want_A && (!b || p.want_b) && sc == DFA_SCATTER_BLOCKED &&
+                        h->has_vp2_plan && h->d_dof_cells && !env_flag("DFA_BC_INLINE");
   if (name_only) {
-    *name_only = dfa::vp2_kernel_name(sc == DFA_SCATTER_BLOCKED, p);
+    *name_only = bc_after ? dfa::vp2_kernel_name(true, q) +
+                                (node_dirichlet(h) ? " + dirichlet_node_rows_kernel" : " + dirichlet_rows_kernel")
+                          : dfa::vp2_kernel_name(sc == DFA_SCATTER_BLOCKED, p);
+    return DFA_OK;
+  }
+  if (bc_after) {
+    if (!h->rows_covered) {
+      hip_check(dfa::launch_fill_zero(values, h->nnz, s), "fill_zero");
+      if (b) hip_check(dfa::launch_fill_zero(b, h->ndofs, s), "fill_zero");
+    }
+    if (dfa::vp2_blocked_lds_bytes(h->vtopo, q.want_b) > 160 * 1024)
+      throw std::runtime_error("vector P2 blocked plan needs more LDS than a CU has");
+    hip_check(dfa::launch_vp2_blocked(h->vtopo, q, s), "vp2_blocked_kernel");
+    apply_dirichlet_rows(h, bcm, bcv, values, b, s);
     return DFA_OK;
   }
   if (sc == DFA_SCATTER_BLOCKED) {

@@ -182,10 +182,11 @@ struct dfa_assembler_s {
   // Dirichlet values shifted by x0 (dfa_assemble_system_x0), allocated on first use
   double* d_bc_x0 = nullptr;
   dfa::DeviceArena x0buf;
-  // Dirichlet conditions after assembly (P1 with edge chains): cells per dof (the diagonal of
-  // a Dirichlet row) and the transpose map of the CSR (the slot of A_ji for each A_ij)
+  // Dirichlet conditions on the assembled rows: cells per dof (the diagonal of a Dirichlet row),
+  // for P1 the transpose map of the CSR (the slot of A_ji for each A_ij)
   int32_t* d_dof_cells = nullptr;
   int32_t* d_tmap = nullptr;
+  uint8_t* d_node_bits = nullptr;      // vector P2 on the node graph: Dirichlet bits, touched flags
   // conjugate-gradient work vectors and scalars (dfa_cg_solve), allocated on first use
   double* d_cg_r = nullptr;
   double* d_cg_p = nullptr;

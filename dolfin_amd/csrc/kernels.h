@@ -167,8 +167,14 @@ hipError_t launch_diagonal(const CsrView& A, double* d, hipStream_t s);
 hipError_t launch_dof_cell_count(int64_t nk, const int32_t* cell_dofs, int32_t* cnt, hipStream_t s);
 hipError_t launch_dirichlet_rows(const CsrView& A, const int32_t* tmap, const uint8_t* mask, const double* g,
                                  const int32_t* cnt, double* values, double* b, hipStream_t s);
-// tmap[k] = the slot of A_ji for the entry k = A_ij (-1 when the pattern has none).
+// tmap may be NULL (A_ji found by binary search). tmap[k] = the slot of A_ji for the entry
+// k = A_ij (-1 when the pattern has none).
 hipError_t launch_transpose_map(const CsrView& A, int32_t* tmap, hipStream_t s);
+// The same for a vector-P2 structure expanded from the node graph (A.node_layout != 0): the
+// Dirichlet node rows, then the free rows next to them pull their Dirichlet columns; scratch:
+// two bytes per node.
+hipError_t launch_dirichlet_nodes(const CsrView& A, const uint8_t* mask, const double* g, const int32_t* cnt,
+                                  uint8_t* scratch, double* values, double* b, hipStream_t s);
 // Conjugate gradients on the device: scalars of one solve (device memory); the partials
 // buffer holds 3 * kCgMaxGroups doubles.
 struct CgState {

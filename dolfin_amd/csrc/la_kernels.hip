@@ -493,12 +493,14 @@ __global__ __launch_bounds__(kLaThreads) void dof_cell_count_kernel(int64_t nk, 
 // b_i = cnt(i) g_i (every cell holding i sets its element row to e_i and b_e,i = g_i); for a
 // free row j, the entry A_ji of a Dirichlet column moves to the right-hand side,
 // b_j -= A_ji g_i, and becomes 0 (the element columns are zeroed after the lifting). The
-// free rows' diagonals are unchanged. A fixed grid of 16-lane groups; group q owns the rows
-// q, q + G, q + 2G, ... (G groups), so the contiguous Dirichlet rows of a mesh face spread over
-// many groups. A group reads the masks of 16 of its rows at once (one per lane), then works
-// through its Dirichlet rows among them with one entry per lane; tmap[k] is the slot of A_ji
-// for the entry k = A_ij (the sparsity is symmetric). Free rows take several liftings:
-// atomic additions.
+// free rows' diagonals are unchanged. A fixed grid of L-lane groups (L = 16 for short rows,
+// 64 for long ones); group q owns the rows q, q + G, q + 2G, ... (G groups), so the contiguous
+// Dirichlet rows of a mesh face spread over many groups. A group reads the masks of L of its
+// rows at once (one per lane), then works through its Dirichlet rows among them with one entry
+// per lane. A_ji (the sparsity is symmetric) comes from the transpose map tmap (the slot of
+// A_ji for each A_ij) when TM, else from a binary search in row j. Free rows take several
+// liftings: atomic additions.
+template <int L, bool TM>
 __global__ __launch_bounds__(kLaThreads) void dirichlet_rows_kernel(int64_t n, const int32_t* __restrict__ rp,
                                                                    const int32_t* __restrict__ col,
                                                                    const int32_t* __restrict__ tmap,
@@ -507,29 +509,43 @@ __global__ __launch_bounds__(kLaThreads) void dirichlet_rows_kernel(int64_t n, c
                                                                    const int32_t* __restrict__ cnt,
                                                                    double* __restrict__ values,
                                                                    double* __restrict__ b) {
-  const int ln = static_cast<int>(threadIdx.x & 15);
-  const int shift = static_cast<int>(threadIdx.x & 48);  // the group's 16 bits in the wavefront ballot
-  const int64_t G = static_cast<int64_t>(gridDim.x) * (kLaThreads / 16);
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kLaThreads / 16) + (threadIdx.x >> 4);
-  const int64_t rounds = (n + 16 * G - 1) / (16 * G);  // the same for every group
+  const int ln = static_cast<int>(threadIdx.x & (L - 1));
+  const int shift = static_cast<int>(threadIdx.x & (63 & ~(L - 1)));  // the group's bits in the ballot
+  const uint64_t gmask = L == 64 ? ~uint64_t{0} : ((uint64_t{1} << L) - 1);
+  const int64_t G = static_cast<int64_t>(gridDim.x) * (kLaThreads / L);
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * (kLaThreads / L) + (threadIdx.x / L);
+  const int64_t rounds = (n + L * G - 1) / (L * G);  // the same for every group
   for (int64_t t = 0; t < rounds; ++t) {
-    const int64_t mine = q + (16 * t + ln) * G;
+    const int64_t mine = q + (L * t + ln) * G;
     const bool on = mine < n && mask[mine];
-    uint32_t bits = static_cast<uint32_t>((__ballot(on) >> shift) & 0xffffu);
+    uint64_t bits = (static_cast<uint64_t>(__ballot(on)) >> shift) & gmask;
     while (bits) {
-      const int64_t i = q + (16 * t + __ffs(bits) - 1) * G;
+      const int64_t i = q + (L * t + __ffsll(static_cast<unsigned long long>(bits)) - 1) * G;
       bits &= bits - 1;
       const double gi = g[i];
       const int32_t r1 = rp[i + 1];
-      for (int32_t k = rp[i] + ln; k < r1; k += 16) {
+      for (int32_t k = rp[i] + ln; k < r1; k += L) {
         const int32_t j = col[k];
-        const int32_t tk = tmap[k];
         if (j == i) {
           values[k] = static_cast<double>(cnt[i]);
           continue;
         }
         values[k] = 0.0;
-        if (mask[j] || tk < 0) continue;
+        if (mask[j]) continue;
+        int32_t tk;
+        if constexpr (TM) {
+          tk = tmap[k];
+        } else {
+          int32_t lo = rp[j], hi = rp[j + 1];
+          const int32_t end = hi;
+          while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (col[mid] < i) lo = mid + 1;
+            else hi = mid;
+          }
+          tk = lo < end && col[lo] == static_cast<int32_t>(i) ? lo : -1;
+        }
+        if (tk < 0) continue;
         const double w = values[tk];
         values[tk] = 0.0;
         if (b) unsafeAtomicAdd(b + j, -w * gi);
@@ -537,6 +553,142 @@ __global__ __launch_bounds__(kLaThreads) void dirichlet_rows_kernel(int64_t n, c
       if (b && ln == 0) b[i] = static_cast<double>(cnt[i]) * gi;
     }
   }
+}
+
+// Dirichlet conditions on a vector-P2 structure expanded from the node graph (the layout of
+// spmv_node_kernel: dof rows (i, n), columns (j, m), value of node entry q of node n at
+// node_slot). Three passes that read only the node rows involved:
+//   node_bc_bits_kernel    bits[n] = the Dirichlet components of node n; touched[n] = 0;
+//   dirichlet_node_rows_kernel  each Dirichlet row (i, n) gets 0 in every entry and cnt on its
+//                          diagonal, b = cnt g; every node m in such a row that has a free row
+//                          is marked touched;
+//   dirichlet_node_pull_kernel  each free row (i, m) of a touched node reads and zeroes its
+//                          entries in Dirichlet columns and subtracts their sum of A g from b,
+//                          reduced over the lanes in a fixed order (no atomics, and no search
+//                          for the transposed entry since the pass runs over the free rows).
+// The second and third find their nodes by a ballot over the flag bytes of 64 nodes per wave,
+// the nodes of a wave strided over the whole range (Dirichlet nodes cluster on the boundary, so
+// this spreads them evenly over the waves), and work on several nodes at once, one per group of
+// lanes. Values are read and written with streaming hints so that the scattered lines do not
+// displace the assembly's inputs from the caches (measured: without them the next assembly
+// pass ran 20 % slower).
+template <bool NODE_MAJOR>
+__device__ __forceinline__ int64_t node_dof(int64_t nn, int64_t node, int c) {
+  return NODE_MAJOR ? 3 * node + c : c * nn + node;
+}
+template <bool NODE_MAJOR>
+__device__ __forceinline__ int64_t node_slot(int64_t nnz_n, int32_t n0, int32_t d, int i, int j, int32_t q) {
+  if constexpr (NODE_MAJOR) return 9 * static_cast<int64_t>(n0) + 3 * (i * static_cast<int64_t>(d) + q) + j;
+  else return 3 * (i * nnz_n + n0) + j * static_cast<int64_t>(d) + q;
+}
+
+template <bool NODE_MAJOR>
+__global__ __launch_bounds__(kLaThreads) void node_bc_bits_kernel(int64_t nn, const uint8_t* __restrict__ mask,
+                                                                 uint8_t* __restrict__ bits,
+                                                                 uint8_t* __restrict__ touched) {
+  const int64_t n = static_cast<int64_t>(blockIdx.x) * kLaThreads + threadIdx.x;
+  if (n >= nn) return;
+  int v = 0;
+  for (int c = 0; c < 3; ++c) v |= mask[node_dof<NODE_MAJOR>(nn, n, c)] ? 1 << c : 0;
+  bits[n] = static_cast<uint8_t>(v);
+  touched[n] = 0;
+}
+
+// The nodes whose flag byte is nonzero: wave w of the G in the grid tests nodes w + (64 t + lane) G
+// and takes the flagged ones 64 / L at a time, calling body(n, lane in the group, live) in every
+// group of L lanes together (live false in a group left without a node, n is then 0), so that
+// the body's shuffles stay within a group.
+template <int L, class Body>
+__device__ __forceinline__ void for_flagged_nodes(int64_t nn, const uint8_t* __restrict__ flag, Body&& body) {
+  const int ln = static_cast<int>(threadIdx.x & 63);
+  const int grp = ln / L;
+  const int64_t G = static_cast<int64_t>(gridDim.x) * (kLaThreads / 64);
+  const int64_t wv = static_cast<int64_t>(blockIdx.x) * (kLaThreads / 64) + (threadIdx.x >> 6);
+  const int64_t rounds = (nn + 64 * G - 1) / (64 * G);  // the same for every wave
+  for (int64_t t = 0; t < rounds; ++t) {
+    const int64_t mine = wv + (64 * t + ln) * G;
+    uint64_t on = static_cast<uint64_t>(__ballot(mine < nn && flag[mine] != 0));
+    while (on) {
+      int l = -1;
+#pragma unroll
+      for (int k = 0; k < 64 / L; ++k) {
+        if (!on) break;
+        const int f = __ffsll(static_cast<unsigned long long>(on)) - 1;
+        on &= on - 1;
+        if (k == grp) l = f;
+      }
+      body(l >= 0 ? wv + (64 * t + l) * G : 0, ln & (L - 1), l >= 0);
+    }
+  }
+}
+
+template <bool NODE_MAJOR>
+__global__ __launch_bounds__(kLaThreads) void dirichlet_node_rows_kernel(
+    int64_t nn, int64_t nnz_n, const int32_t* __restrict__ nrp, const int32_t* __restrict__ ncol,
+    const uint8_t* __restrict__ bits, const double* __restrict__ g, const int32_t* __restrict__ cnt,
+    double* __restrict__ values, double* __restrict__ b, uint8_t* __restrict__ touched) {
+  constexpr int L = 32;
+  for_flagged_nodes<L>(nn, bits, [&](int64_t n, int ln, bool live) {
+    if (!live) return;
+    const int am = bits[n];
+    const int32_t n0 = nrp[n], d = nrp[n + 1] - n0;
+    double dg[3];
+    for (int i = 0; i < 3; ++i) dg[i] = am >> i & 1 ? static_cast<double>(cnt[node_dof<NODE_MAJOR>(nn, n, i)]) : 0.0;
+    for (int32_t q = ln; q < d; q += L) {
+      const int32_t m = __builtin_nontemporal_load(ncol + n0 + q);
+      for (int i = 0; i < 3; ++i) {
+        if (!(am >> i & 1)) continue;
+        for (int j = 0; j < 3; ++j)
+          __builtin_nontemporal_store(m == n && j == i ? dg[i] : 0.0,
+                                      values + node_slot<NODE_MAJOR>(nnz_n, n0, d, i, j, q));
+      }
+      if (bits[m] != 7) touched[m] = 1;
+    }
+    if (b && ln == 0)
+      for (int i = 0; i < 3; ++i)
+        if (am >> i & 1) {
+          const int64_t r = node_dof<NODE_MAJOR>(nn, n, i);
+          b[r] = dg[i] * g[r];
+        }
+  });
+}
+
+template <bool NODE_MAJOR>
+__global__ __launch_bounds__(kLaThreads) void dirichlet_node_pull_kernel(
+    int64_t nn, int64_t nnz_n, const int32_t* __restrict__ nrp, const int32_t* __restrict__ ncol,
+    const uint8_t* __restrict__ bits, const uint8_t* __restrict__ touched, const double* __restrict__ g,
+    double* __restrict__ values, double* __restrict__ b) {
+  constexpr int L = 16;
+  for_flagged_nodes<L>(nn, touched, [&](int64_t n, int ln, bool live) {
+    const int free_rows = live ? ~bits[n] & 7 : 0;
+    const int32_t n0 = live ? nrp[n] : 0, d = live ? nrp[n + 1] - n0 : 0;
+    double bv[3] = {0.0, 0.0, 0.0};  // loaded before the entries, so that the update waits on nothing
+    if (b && ln == 0)
+      for (int i = 0; i < 3; ++i)
+        if (free_rows >> i & 1) bv[i] = b[node_dof<NODE_MAJOR>(nn, n, i)];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int32_t q = ln; q < d; q += L) {
+      const int32_t m = __builtin_nontemporal_load(ncol + n0 + q);
+      const int bm = bits[m];
+      if (!bm) continue;
+      for (int j = 0; j < 3; ++j) {
+        if (!(bm >> j & 1)) continue;
+        const double gj = g[node_dof<NODE_MAJOR>(nn, m, j)];
+        for (int i = 0; i < 3; ++i) {
+          if (!(free_rows >> i & 1)) continue;
+          double* v = values + node_slot<NODE_MAJOR>(nnz_n, n0, d, i, j, q);
+          s[i] = fma(__builtin_nontemporal_load(v), gj, s[i]);
+          __builtin_nontemporal_store(0.0, v);
+        }
+      }
+    }
+    if (!b) return;
+    for (int i = 0; i < 3; ++i)
+      for (int o = L >> 1; o > 0; o >>= 1) s[i] += __shfl_xor(s[i], o);
+    if (ln == 0)
+      for (int i = 0; i < 3; ++i)
+        if (free_rows >> i & 1) b[node_dof<NODE_MAJOR>(nn, n, i)] = bv[i] - s[i];
+  });
 }
 
 // tmap[k] = slot of A_ji for the entry k = A_ij (-1 when the pattern has no A_ji): 16 lanes
@@ -675,10 +827,51 @@ hipError_t launch_dof_cell_count(int64_t nk, const int32_t* cell_dofs, int32_t* 
 hipError_t launch_dirichlet_rows(const CsrView& A, const int32_t* tmap, const uint8_t* mask, const double* g,
                                  const int32_t* cnt, double* values, double* b, hipStream_t s) {
   if (A.nrows == 0) return hipSuccess;
-  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((A.nrows + 16 * (kLaThreads / 16) - 1) /
-                                                                  (16 * (kLaThreads / 16)), 4096));
-  hipLaunchKernelGGL(dirichlet_rows_kernel, dim3(static_cast<unsigned>(grid)), dim3(kLaThreads), 0, s, A.nrows,
-                     A.row_ptr, A.col, tmap, mask, g, cnt, values, b);
+  const bool long_rows = A.nnz > 32 * A.nrows;  // more than 32 entries per row on average
+  const int L = long_rows ? 64 : 16;
+  const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((A.nrows + L * (kLaThreads / L) - 1) /
+                                                                  (L * (kLaThreads / L)), 4096));
+  const dim3 gd(static_cast<unsigned>(grid)), bd(kLaThreads);
+  if (long_rows) {
+    if (tmap) hipLaunchKernelGGL((dirichlet_rows_kernel<64, true>), gd, bd, 0, s, A.nrows, A.row_ptr, A.col, tmap,
+                                 mask, g, cnt, values, b);
+    else hipLaunchKernelGGL((dirichlet_rows_kernel<64, false>), gd, bd, 0, s, A.nrows, A.row_ptr, A.col, tmap,
+                            mask, g, cnt, values, b);
+  } else {
+    if (tmap) hipLaunchKernelGGL((dirichlet_rows_kernel<16, true>), gd, bd, 0, s, A.nrows, A.row_ptr, A.col, tmap,
+                                 mask, g, cnt, values, b);
+    else hipLaunchKernelGGL((dirichlet_rows_kernel<16, false>), gd, bd, 0, s, A.nrows, A.row_ptr, A.col, tmap,
+                            mask, g, cnt, values, b);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_dirichlet_nodes(const CsrView& A, const uint8_t* mask, const double* g, const int32_t* cnt,
+                                  uint8_t* scratch, double* values, double* b, hipStream_t s) {
+  if (A.nrows == 0) return hipSuccess;
+  if (!A.node_layout || !A.node_rp || !A.node_col || !scratch || A.nrows % 3 || A.node_nnz * 9 != A.nnz)
+    return hipErrorInvalidValue;
+  const int64_t nn = A.nrows / 3;
+  uint8_t* bits = scratch;
+  uint8_t* touched = scratch + nn;
+  const int64_t g1 = (nn + kLaThreads - 1) / kLaThreads;
+  // about one wave per SIMD slot of the chip: each wave tests a strided sample of the nodes
+  const int64_t g2 = std::max<int64_t>(1, std::min<int64_t>(((nn + 63) / 64 + kLaThreads / 64 - 1) / (kLaThreads / 64), 1024));
+  if (g1 > INT32_MAX) return hipErrorInvalidValue;
+  const dim3 bd(kLaThreads), d1(static_cast<unsigned>(g1)), d2(static_cast<unsigned>(g2));
+  if (A.node_layout == 2) {
+    hipLaunchKernelGGL(node_bc_bits_kernel<true>, d1, bd, 0, s, nn, mask, bits, touched);
+    hipLaunchKernelGGL(dirichlet_node_rows_kernel<true>, d2, bd, 0, s, nn, A.node_nnz, A.node_rp, A.node_col, bits,
+                       g, cnt, values, b, touched);
+    hipLaunchKernelGGL(dirichlet_node_pull_kernel<true>, d2, bd, 0, s, nn, A.node_nnz, A.node_rp, A.node_col, bits,
+                       touched, g, values, b);
+  } else {
+    hipLaunchKernelGGL(node_bc_bits_kernel<false>, d1, bd, 0, s, nn, mask, bits, touched);
+    hipLaunchKernelGGL(dirichlet_node_rows_kernel<false>, d2, bd, 0, s, nn, A.node_nnz, A.node_rp, A.node_col, bits,
+                       g, cnt, values, b, touched);
+    hipLaunchKernelGGL(dirichlet_node_pull_kernel<false>, d2, bd, 0, s, nn, A.node_nnz, A.node_rp, A.node_col, bits,
+                       touched, g, values, b);
+  }
   return hipGetLastError();
 }
 

@@ -353,3 +353,66 @@ def test_p1_dirichlet_after_chains(oracle, monkeypatch, n, permuted):
     torch.cuda.synchronize()
     assert rel(A2.data()[2], vals) < TOL
     assert rel(b2.get_local(), b.get_local()) < TOL
+
+
+@pytest.mark.parametrize("n,permuted", [(3, False), (3, True), (6, False)])
+def test_vp2_dirichlet_after_assembly(oracle, monkeypatch, n, permuted):
+    # Vector P2 with Dirichlet conditions on the boundary and on scattered interior dofs: the
+    # blocked kernel assembles without them and dirichlet_node_rows_kernel (then the free rows pull their
+    # Dirichlet columns over the node graph) applies what the per-cell elimination
+    # (SystemAssembler.cpp:1051-1075) sums up to. Checked against the oracle's per-cell loop,
+    # the incremental form with x0, dirichlet_rows_kernel (DFA_BC_ROWS=1: pushed by the
+    # Dirichlet rows, binary search for the transposed entry) and the per-cell elimination
+    # inside the blocked kernel (DFA_BC_INLINE=1).
+    print(f"case vp2 dirichlet after assembly n={n} permuted={permuted}", flush=True)
+    mesh, V, mu, lmbda, f3 = _vp2_case(n, permuted, seed=70 + n)
+    rng = np.random.default_rng(71 + n)
+    dofs = np.union1d(dfa.DirichletBC(V, 0.0).dofs, rng.choice(V.dim(), V.dim() // 9, replace=False))
+    g = rng.uniform(-1, 1, V.dim())
+    bc = dfa.DirichletBC(V, g[dofs], dofs)
+    eng = dfa.engine_for(V)
+    kname = eng.pass_kernel(_lib.FORM_VP2_ELASTICITY, _lib.FORM_VP2_BODY_FORCE, has_bc=True)
+    assert kname == "vp2_blocked_kernel<WANT_A=1,WANT_B=1,BC=0> + dirichlet_node_rows_kernel", kname
+    Q = V.coefficient_space()
+    a = dfa.elasticity_form(V, workloads.to_function(Q, mu), workloads.to_function(Q, lmbda))
+    L = dfa.body_force_form(V, f3)
+    asm = dfa.SystemAssembler(a, L, bc)
+    A, b = dfa.Matrix(), dfa.Vector()
+    asm.assemble(A, b)
+    Aonly = dfa.Matrix()
+    asm.assemble(Aonly)
+    x0 = torch.from_numpy(rng.uniform(-1, 1, V.dim())).cuda()
+    Ax, bx = dfa.Matrix(), dfa.Vector()
+    asm.assemble(Ax, bx, x0)
+    torch.cuda.synchronize()
+    rp, col, vals = A.data()
+    mask = np.zeros(V.dim(), np.uint8)
+    mask[dofs] = 1
+    kd = Q.cell_dofs()
+
+    def ref(gvals):
+        return oracle.assemble_system(oracle.VP2_ELASTICITY, oracle.VP2_BODY_FORCE,
+                                      mesh.coordinates(), mesh.cells(), V.cell_dofs(), rp, col, mask,
+                                      gvals, coeff_dofs=kd, a_c0=mu, a_c1=lmbda, f3=f3)
+
+    gv = np.zeros(V.dim())
+    gv[dofs] = g[dofs]
+    ra, rb = ref(gv)
+    assert rel(vals, ra) < TOL
+    assert rel(b.get_local(), rb) < TOL
+    assert rel(Aonly.data()[2], ra) < TOL
+    gx = np.zeros(V.dim())
+    gx[dofs] = x0.cpu().numpy()[dofs] - g[dofs]
+    _, rbx = ref(gx)
+    assert rel(Ax.data()[2], ra) < TOL
+    assert rel(bx.get_local(), rbx) < TOL
+    for env, expect in [("DFA_BC_ROWS", "vp2_blocked_kernel<WANT_A=1,WANT_B=1,BC=0> + dirichlet_rows_kernel"),
+                        ("DFA_BC_INLINE", "vp2_blocked_kernel<WANT_A=1,WANT_B=1,BC=1>")]:
+        monkeypatch.setenv(env, "1")
+        assert eng.pass_kernel(_lib.FORM_VP2_ELASTICITY, _lib.FORM_VP2_BODY_FORCE, has_bc=True) == expect
+        A2, b2 = dfa.Matrix(), dfa.Vector()
+        asm.assemble(A2, b2)
+        torch.cuda.synchronize()
+        assert rel(A2.data()[2], vals) < TOL
+        assert rel(b2.get_local(), b.get_local()) < TOL
+        monkeypatch.delenv(env)
