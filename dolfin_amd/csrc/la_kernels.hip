@@ -658,7 +658,7 @@ __global__ __launch_bounds__(kLaThreads) void dirichlet_node_pull_kernel(
     int64_t nn, int64_t nnz_n, const int32_t* __restrict__ nrp, const int32_t* __restrict__ ncol,
     const uint8_t* __restrict__ bits, const uint8_t* __restrict__ touched, const double* __restrict__ g,
     double* __restrict__ values, double* __restrict__ b) {
-  constexpr int L = 16;
+  constexpr int L = 32;
   for_flagged_nodes<L>(nn, touched, [&](int64_t n, int ln, bool live) {
     const int free_rows = live ? ~bits[n] & 7 : 0;
     const int32_t n0 = live ? nrp[n] : 0, d = live ? nrp[n + 1] - n0 : 0;
