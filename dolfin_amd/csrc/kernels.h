@@ -178,12 +178,13 @@ hipError_t launch_dirichlet_nodes(const CsrView& A, const uint8_t* mask, const d
 // Conjugate gradients on the device: scalars of one solve (device memory); the partials
 // buffer holds 3 * kCgMaxGroups doubles.
 struct CgState {
-  double rz, rr, bb, alpha, beta, tol2;
+  double rzs[2];  // r.z at the start of iteration it in rzs[it & 1]
+  double rr, bb, alpha, beta, tol2;
   int64_t done;   // iteration at which |r|^2 <= tol2 (-1: not yet)
   int32_t broke;  // p.Ap = 0 with r != 0
   int32_t pad;
 };
-constexpr int kCgMaxGroups = 2048;
+constexpr int kCgMaxGroups = 1024;
 hipError_t launch_cg_init(int64_t n, const double* b, const double* r, const double* dinv, double* p,
                           double* part, CgState* st, hipStream_t s);
 hipError_t launch_cg_iteration(const CsrView& A, int64_t it, double* x, double* p, double* r, double* ap,

@@ -268,10 +268,14 @@ __global__ __launch_bounds__(kLaThreads) void diagonal_kernel(int64_t nrows, con
 }
 
 // ---- conjugate gradients -------------------------------------------------------------------
-// Per iteration: Ap = A p (SpMV above), then vector kernels that each write one partial sum
-// per workgroup, each followed by a one-workgroup kernel that sums the partials in index
-// order (deterministic; the kernel boundary makes them visible, so no device-scope fence is
-// needed, which on several XCDs would write back L2) and updates the scalars in CgState. A
+// Per iteration: Ap = A p (SpMV above), then three vector kernels that each write one partial
+// sum per workgroup. The kernel that needs a sum (alpha = r.z / p.Ap in the update, beta =
+// r.z' / r.z in the direction update) reduces the previous kernel's partials itself, in
+// every workgroup and in index order, so all workgroups get the same value (deterministic;
+// the kernel boundary makes the partials visible, so no device-scope fence is needed, which on
+// several XCDs would write back L2, and no one-workgroup kernel runs in between). Workgroup 0
+// records the scalars in CgState; r.z is double-buffered by iteration parity because the
+// direction kernel's workgroups read the old value while workgroup 0 stores the new one. A
 // converged or broken-down state makes every later kernel return at once, so the iterate is
 // frozen until the host reads the state.
 constexpr int kCgThreads = 256;
@@ -315,36 +319,15 @@ __global__ __launch_bounds__(kCgThreads) void cg_init_finish(const double* __res
   double tot[3];
   cg_totals<3>(part, g, tot);
   if (threadIdx.x == 0) {
-    st->rz = tot[0];
+    st->rzs[1] = tot[0];  // iteration 1
     st->rr = tot[1];
     st->bb = tot[2];
   }
 }
 
-// alpha = rz / pAp (0 and breakdown when pAp = 0)
-__global__ __launch_bounds__(kCgThreads) void cg_alpha_finish(const double* __restrict__ part, int g, CgState* st) {
-  if (st->done >= 0 || st->broke) return;
-  double tot[1];
-  cg_totals<1>(part, g, tot);
-  if (threadIdx.x == 0) {
-    if (tot[0] == 0.0) st->broke = 1;
-    st->alpha = tot[0] != 0.0 ? st->rz / tot[0] : 0.0;
-  }
-}
-
-// beta = rz' / rz; converged at `it` when rr <= tol^2
-__global__ __launch_bounds__(kCgThreads) void cg_beta_finish(const double* __restrict__ part, int g, int64_t it,
-                                                             CgState* st) {
-  if (st->done >= 0 || st->broke) return;
-  double tot[2];
-  cg_totals<2>(part, g, tot);
-  if (threadIdx.x == 0) {
-    st->beta = st->rz != 0.0 ? tot[0] / st->rz : 0.0;
-    st->rz = tot[0];
-    st->rr = tot[1];
-    if (tot[1] <= st->tol2) st->done = it;
-  }
-}
+// the p.Ap partials, apart from the update's r.z' and r.r partials that the direction kernel
+// reads while the next iteration's p.Ap is not yet written
+__device__ __forceinline__ double* cg_dot_part(double* part) { return part + 2 * kCgMaxGroups; }
 
 // The vector kernels take four elements per thread at a time (stride = the grid): all loads
 // of a batch are issued before the arithmetic and the stores, then a remainder loop. J: the
@@ -405,19 +388,27 @@ __global__ __launch_bounds__(kCgThreads) void cg_dot_kernel(int64_t n, const dou
     for (int u = 0; u < 4; ++u) v[0] = fma(pv[u], av[u], v[0]);
   }
   for (; i < n; i += s) v[0] = fma(p[i], ap[i], v[0]);
-  cg_partials<1>(v, part);
+  cg_partials<1>(v, cg_dot_part(part));
 }
 
-// x += alpha p, r -= alpha Ap; partials of rz' = r.(D r) and rr = r.r
+// alpha = r.z / p.Ap (breakdown when p.Ap = 0); x += alpha p, r -= alpha Ap; partials of
+// r.z' = r.(D r) and r.r
 template <bool J>
-__global__ __launch_bounds__(kCgThreads) void cg_update_kernel(int64_t n, double* __restrict__ x,
+__global__ __launch_bounds__(kCgThreads) void cg_update_kernel(int64_t n, int64_t it, double* __restrict__ x,
                                                                const double* __restrict__ p,
                                                                double* __restrict__ r,
                                                                const double* __restrict__ ap,
                                                                const double* __restrict__ dinv,
-                                                               double* __restrict__ part, const CgState* st) {
+                                                               double* __restrict__ part, CgState* st) {
   if (st->done >= 0 || st->broke) return;
-  const double alpha = st->alpha;
+  double pap[1];
+  cg_totals<1>(cg_dot_part(part), static_cast<int>(gridDim.x), pap);
+  if (pap[0] == 0.0) {  // the same in every workgroup
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->broke = 1;
+    return;
+  }
+  const double alpha = st->rzs[it & 1] / pap[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alpha;
   double v[2] = {0.0, 0.0};
   CG_LOOP(i, s, n) {
     double xv[4], pv[4], rv[4], av[4], dv[4];
@@ -450,13 +441,24 @@ __global__ __launch_bounds__(kCgThreads) void cg_update_kernel(int64_t n, double
   cg_partials<2>(v, part);
 }
 
-// p = D r + beta p
+// beta = r.z' / r.z, converged at `it` when r.r <= tol^2; else p = D r + beta p
 template <bool J>
-__global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(int64_t n, const double* __restrict__ r,
+__global__ __launch_bounds__(kCgThreads) void cg_dir_kernel(int64_t n, int64_t it, const double* __restrict__ r,
                                                             const double* __restrict__ dinv, double* __restrict__ p,
-                                                            const CgState* st) {
+                                                            const double* __restrict__ part, CgState* st) {
   if (st->done >= 0 || st->broke) return;
-  const double beta = st->beta;
+  double tot[2];
+  cg_totals<2>(part, static_cast<int>(gridDim.x), tot);
+  const double rz = st->rzs[it & 1];
+  const double beta = rz != 0.0 ? tot[0] / rz : 0.0;
+  const bool converged = tot[1] <= st->tol2;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rzs[(it + 1) & 1] = tot[0];
+    st->rr = tot[1];
+    st->beta = beta;
+    if (converged) st->done = it;
+  }
+  if (converged) return;  // the iterate is final; p is not needed
   CG_LOOP(i, s, n) {
     double rv[4], pv[4], dv[4];
 #pragma unroll
@@ -799,16 +801,14 @@ hipError_t launch_cg_iteration(const CsrView& A, int64_t it, double* x, double* 
   if (e != hipSuccess) return e;
   const int g = cg_grid(n);
   hipLaunchKernelGGL(cg_dot_kernel, dim3(g), dim3(kCgThreads), 0, s, n, p, ap, part, st);
-  hipLaunchKernelGGL(cg_alpha_finish, dim3(1), dim3(kCgThreads), 0, s, part, g, st);
   if (dinv)
-    hipLaunchKernelGGL(cg_update_kernel<true>, dim3(g), dim3(kCgThreads), 0, s, n, x, p, r, ap, dinv, part, st);
+    hipLaunchKernelGGL(cg_update_kernel<true>, dim3(g), dim3(kCgThreads), 0, s, n, it, x, p, r, ap, dinv, part, st);
   else
-    hipLaunchKernelGGL(cg_update_kernel<false>, dim3(g), dim3(kCgThreads), 0, s, n, x, p, r, ap, dinv, part, st);
-  hipLaunchKernelGGL(cg_beta_finish, dim3(1), dim3(kCgThreads), 0, s, part, g, it, st);
+    hipLaunchKernelGGL(cg_update_kernel<false>, dim3(g), dim3(kCgThreads), 0, s, n, it, x, p, r, ap, dinv, part, st);
   if (dinv)
-    hipLaunchKernelGGL(cg_dir_kernel<true>, dim3(g), dim3(kCgThreads), 0, s, n, r, dinv, p, st);
+    hipLaunchKernelGGL(cg_dir_kernel<true>, dim3(g), dim3(kCgThreads), 0, s, n, it, r, dinv, p, part, st);
   else
-    hipLaunchKernelGGL(cg_dir_kernel<false>, dim3(g), dim3(kCgThreads), 0, s, n, r, dinv, p, st);
+    hipLaunchKernelGGL(cg_dir_kernel<false>, dim3(g), dim3(kCgThreads), 0, s, n, it, r, dinv, p, part, st);
   return hipGetLastError();
 }
 
