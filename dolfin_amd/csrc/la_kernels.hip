@@ -829,8 +829,11 @@ hipError_t launch_dirichlet_rows(const CsrView& A, const int32_t* tmap, const ui
   if (A.nrows == 0) return hipSuccess;
   const bool long_rows = A.nnz > 32 * A.nrows;  // more than 32 entries per row on average
   const int L = long_rows ? 64 : 16;
+  // about one wave per SIMD slot of the chip (1024 workgroups of 8 waves), each group looping
+  // over its strided rows: 2048 and 4096 workgroups (rows in fewer rounds, but waves in
+  // several batches) and 512 measured slower
   const int64_t grid = std::max<int64_t>(1, std::min<int64_t>((A.nrows + L * (kLaThreads / L) - 1) /
-                                                                  (L * (kLaThreads / L)), 4096));
+                                                                  (L * (kLaThreads / L)), 1024));
   const dim3 gd(static_cast<unsigned>(grid)), bd(kLaThreads);
   if (long_rows) {
     if (tmap) hipLaunchKernelGGL((dirichlet_rows_kernel<64, true>), gd, bd, 0, s, A.nrows, A.row_ptr, A.col, tmap,
