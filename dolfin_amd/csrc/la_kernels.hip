@@ -548,9 +548,8 @@ __global__ __launch_bounds__(kLaThreads) void dirichlet_rows_kernel(int64_t n, c
           tk = lo < end && col[lo] == static_cast<int32_t>(i) ? lo : -1;
         }
         if (tk < 0) continue;
-        const double w = values[tk];
+        if (b && gi != 0.0) unsafeAtomicAdd(b + j, -values[tk] * gi);  // A_ji read only when it lifts
         values[tk] = 0.0;
-        if (b) unsafeAtomicAdd(b + j, -w * gi);
       }
       if (b && ln == 0) b[i] = static_cast<double>(cnt[i]) * gi;
     }
@@ -636,15 +635,22 @@ __global__ __launch_bounds__(kLaThreads) void dirichlet_node_rows_kernel(
     const int32_t n0 = nrp[n], d = nrp[n + 1] - n0;
     double dg[3];
     for (int i = 0; i < 3; ++i) dg[i] = am >> i & 1 ? static_cast<double>(cnt[node_dof<NODE_MAJOR>(nn, n, i)]) : 0.0;
+    // the node entries: neighbours with a free row marked, the diagonal node entry found
+    int32_t qs = -1;
     for (int32_t q = ln; q < d; q += L) {
       const int32_t m = __builtin_nontemporal_load(ncol + n0 + q);
-      for (int i = 0; i < 3; ++i) {
-        if (!(am >> i & 1)) continue;
-        for (int j = 0; j < 3; ++j)
-          __builtin_nontemporal_store(m == n && j == i ? dg[i] : 0.0,
-                                      values + node_slot<NODE_MAJOR>(nnz_n, n0, d, i, j, q));
-      }
+      if (m == n) qs = q;
       if (bits[m] != 7) touched[m] = 1;
+    }
+    for (int o = L >> 1; o > 0; o >>= 1) qs = max(qs, __shfl_xor(qs, o));
+    // a Dirichlet row (i, n) is 3 d contiguous values (its node entries' three columns each):
+    // stored with consecutive lanes on consecutive values (one instruction per L values
+    // instead of one per column component; measured: 120 -> 43 us without these stores)
+    for (int i = 0; i < 3; ++i) {
+      if (!(am >> i & 1)) continue;
+      const int64_t base = node_slot<NODE_MAJOR>(nnz_n, n0, d, i, 0, 0);
+      const int diag = NODE_MAJOR ? 3 * qs + i : i * d + qs;
+      for (int jl = ln; jl < 3 * d; jl += L) __builtin_nontemporal_store(jl == diag ? dg[i] : 0.0, values + base + jl);
     }
     if (b && ln == 0)
       for (int i = 0; i < 3; ++i)
@@ -676,10 +682,11 @@ __global__ __launch_bounds__(kLaThreads) void dirichlet_node_pull_kernel(
       for (int j = 0; j < 3; ++j) {
         if (!(bm >> j & 1)) continue;
         const double gj = g[node_dof<NODE_MAJOR>(nn, m, j)];
+        const bool lift = b && gj != 0.0;  // the value is read only when it lifts into b
         for (int i = 0; i < 3; ++i) {
           if (!(free_rows >> i & 1)) continue;
           double* v = values + node_slot<NODE_MAJOR>(nnz_n, n0, d, i, j, q);
-          s[i] = fma(__builtin_nontemporal_load(v), gj, s[i]);
+          if (lift) s[i] = fma(__builtin_nontemporal_load(v), gj, s[i]);
           __builtin_nontemporal_store(0.0, v);
         }
       }

@@ -416,3 +416,42 @@ def test_vp2_dirichlet_after_assembly(oracle, monkeypatch, n, permuted):
         assert rel(A2.data()[2], vals) < TOL
         assert rel(b2.get_local(), b.get_local()) < TOL
         monkeypatch.delenv(env)
+
+
+@pytest.mark.parametrize("element", ["p1", "vp2"])
+def test_homogeneous_dirichlet_after_assembly(oracle, element):
+    # g = 0 on the boundary: the post-passes zero the eliminated entries of the free rows
+    # without reading them (nothing is lifted into b); A and b against the oracle's per-cell
+    # elimination (SystemAssembler.cpp:1051-1075), for assemble(A, b) and assemble(A).
+    print(f"case homogeneous dirichlet after assembly {element}", flush=True)
+    if element == "p1":
+        mesh, V, kappa, f = _p1_case(7, False, seed=80)
+        a = dfa.poisson_form(V, workloads.to_function(V, kappa))
+        L = dfa.load_form(V, workloads.to_function(V, f))
+        kw = dict(coeff_dofs=V.cell_dofs(), a_c0=kappa, L_c0=f)
+        forms = (oracle.P1_POISSON, oracle.P1_LOAD)
+    else:
+        mesh, V, mu, lmbda, f3 = _vp2_case(4, False, seed=81)
+        Q = V.coefficient_space()
+        a = dfa.elasticity_form(V, workloads.to_function(Q, mu), workloads.to_function(Q, lmbda))
+        L = dfa.body_force_form(V, f3)
+        kw = dict(coeff_dofs=Q.cell_dofs(), a_c0=mu, a_c1=lmbda, f3=f3)
+        forms = (oracle.VP2_ELASTICITY, oracle.VP2_BODY_FORCE)
+    bc = dfa.DirichletBC(V, 0.0)
+    assert dfa.engine_for(V).pass_kernel(*(_lib.FORM_P1_POISSON, _lib.FORM_P1_LOAD) if element == "p1" else
+                                         (_lib.FORM_VP2_ELASTICITY, _lib.FORM_VP2_BODY_FORCE),
+                                         has_bc=True).split(" + ")[1].startswith("dirichlet_")
+    asm = dfa.SystemAssembler(a, L, bc)
+    A, b = dfa.Matrix(), dfa.Vector()
+    asm.assemble(A, b)
+    Aonly = dfa.Matrix()
+    asm.assemble(Aonly)
+    torch.cuda.synchronize()
+    rp, col, vals = A.data()
+    mask = np.zeros(V.dim(), np.uint8)
+    mask[bc.dofs] = 1
+    ra, rb = oracle.assemble_system(*forms, mesh.coordinates(), mesh.cells(), V.cell_dofs(), rp, col, mask,
+                                    np.zeros(V.dim()), **kw)
+    assert rel(vals, ra) < TOL
+    assert rel(b.get_local(), rb) < TOL
+    assert rel(Aonly.data()[2], ra) < TOL
