@@ -118,8 +118,14 @@ int dfa_assemble_vector(dfa_assembler h, int32_t form, const double* d_x, const 
 
 /*
  * SystemAssembler::assemble(A, b) for one square bilinear form a and linear form L, fused in
- * one pass over the cells with the per-cell symmetric Dirichlet modification of
- * SystemAssembler::apply_bc (dolfin/fem/SystemAssembler.cpp:303-492,1051-1075).
+ * one pass over the cells, with the result of the per-cell symmetric Dirichlet modification
+ * of SystemAssembler::apply_bc (dolfin/fem/SystemAssembler.cpp:303-492,1051-1075): when the
+ * matrix is assembled without DFA_ADD_VALUES on the blocked scatter, the pass runs without
+ * the conditions and a second step applies what the per-cell elimination sums up to to the
+ * assembled rows (a Dirichlet row i: cells holding i on the diagonal, 0 elsewhere,
+ * b_i = that count times g_i; a free row j: b_j -= A_ji g_i and A_ji = 0); otherwise (and
+ * with the environment variable DFA_BC_INLINE=1) every cell tensor is modified in the pass.
+ * The two agree up to rounding.
  *   d_bc_mask [num_dofs] uint8 (nonzero = Dirichlet dof), d_bc_value [num_dofs] (may be NULL
  *   when there are no boundary conditions). d_values or d_b may be NULL (assemble(A) or
  *   assemble(b) only). flags: DFA_ADD_VALUES adds to A and b instead of overwriting them
