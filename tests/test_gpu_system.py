@@ -455,3 +455,47 @@ def test_homogeneous_dirichlet_after_assembly(oracle, element):
     assert rel(vals, ra) < TOL
     assert rel(b.get_local(), rb) < TOL
     assert rel(Aonly.data()[2], ra) < TOL
+
+
+@pytest.mark.parametrize("element", ["p1", "vp2"])
+def test_keep_diagonal_dirichlet_after_assembly(oracle, element):
+    # SystemAssembler with keep_diagonal (AssemblerBase.h:46-67): vector P2 then takes the
+    # general CSR build (no node graph), so its conditions go through dirichlet_rows_kernel
+    # with a binary search for each transposed entry; P1 keeps the transpose map. Random
+    # values on the boundary and on scattered interior dofs, against the oracle.
+    print(f"case keep_diagonal dirichlet after assembly {element}", flush=True)
+    rng = np.random.default_rng(90)
+    if element == "p1":
+        mesh, V, kappa, f = _p1_case(6, True, seed=91)
+        a = dfa.poisson_form(V, workloads.to_function(V, kappa))
+        L = dfa.load_form(V, workloads.to_function(V, f))
+        kw = dict(coeff_dofs=V.cell_dofs(), a_c0=kappa, L_c0=f)
+        forms = (oracle.P1_POISSON, oracle.P1_LOAD)
+        kinds = (_lib.FORM_P1_POISSON, _lib.FORM_P1_LOAD)
+    else:
+        mesh, V, mu, lmbda, f3 = _vp2_case(3, False, seed=92)
+        Q = V.coefficient_space()
+        a = dfa.elasticity_form(V, workloads.to_function(Q, mu), workloads.to_function(Q, lmbda))
+        L = dfa.body_force_form(V, f3)
+        kw = dict(coeff_dofs=Q.cell_dofs(), a_c0=mu, a_c1=lmbda, f3=f3)
+        forms = (oracle.VP2_ELASTICITY, oracle.VP2_BODY_FORCE)
+        kinds = (_lib.FORM_VP2_ELASTICITY, _lib.FORM_VP2_BODY_FORCE)
+    dofs = np.union1d(dfa.DirichletBC(V, 0.0).dofs, rng.choice(V.dim(), V.dim() // 8, replace=False))
+    g = rng.uniform(-1, 1, V.dim())
+    bc = dfa.DirichletBC(V, g[dofs], dofs)
+    kname = dfa.engine_for(V, keep_diagonal=True).pass_kernel(*kinds, has_bc=True)
+    assert kname.endswith("+ dirichlet_rows_kernel"), kname
+    asm = dfa.SystemAssembler(a, L, bc)
+    asm.keep_diagonal = True
+    A, b = dfa.Matrix(), dfa.Vector()
+    asm.assemble(A, b)
+    torch.cuda.synchronize()
+    rp, col, vals = A.data()
+    mask = np.zeros(V.dim(), np.uint8)
+    mask[dofs] = 1
+    gv = np.zeros(V.dim())
+    gv[dofs] = g[dofs]
+    ra, rb = oracle.assemble_system(*forms, mesh.coordinates(), mesh.cells(), V.cell_dofs(), rp, col, mask, gv,
+                                    **kw)
+    assert rel(vals, ra) < TOL
+    assert rel(b.get_local(), rb) < TOL
