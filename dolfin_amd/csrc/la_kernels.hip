@@ -501,7 +501,7 @@ __global__ __launch_bounds__(kLaThreads) void dof_cell_count_kernel(int64_t nk, 
 // rows at once (one per lane), then works through its Dirichlet rows among them with one entry
 // per lane. A_ji (the sparsity is symmetric) comes from the transpose map tmap (the slot of
 // A_ji for each A_ij) when TM, else from a binary search in row j. Free rows take several
-// liftings: atomic additions.
+// liftings: atomic additions; A_ji is read only when g_i lifts it (g_i != 0 and b wanted).
 template <int L, bool TM>
 __global__ __launch_bounds__(kLaThreads) void dirichlet_rows_kernel(int64_t n, const int32_t* __restrict__ rp,
                                                                    const int32_t* __restrict__ col,
