@@ -515,6 +515,27 @@ def main():
                                                + 16 * V2.dim(),
                                "traffic": load_traffic(f"vp2_spmv_{args.p2_n}"),
                                "workload": f"y = A x on the assembled vector-P2 {args.p2_n}^3 matrix"}
+            # Jacobi-preconditioned CG on the elasticity matrix (pure Neumann, so the iterations
+            # only measure speed): 20 iterations per solve, tolerance 0 so that all run
+            bcg2 = torch.rand(V2.dim(), dtype=torch.float64, device="cuda")
+            xcg2 = torch.zeros_like(bcg2)
+            cg2_err = []
+
+            def cg20():
+                xcg2.zero_()
+                try:
+                    dfa.solve(A2, xcg2, bcg2, "cg", "jacobi", rtol=0.0, atol=0.0, maxiter=20, check_every=20)
+                except RuntimeError as e:
+                    if "did not converge" not in str(e):
+                        cg2_err.append(str(e))
+            w_cg2, ev_cg2, _ = timed_steps(cg20, 3, 1, settle_s=0.0)
+            cg2_it = float(np.mean(ev_cg2)) / 20
+            cg2_bytes = mv2_bytes + 8 * 13 * V2.dim()
+            sec["vp2_cg"] = {"ms_per_iteration": cg2_it, "spmv_ms": mv2_ms,
+                             "hbm_frac": cg2_bytes / (cg2_it * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             "algorithmic_bytes_per_iteration": cg2_bytes, "errors": cg2_err,
+                             "workload": f"Jacobi-preconditioned CG iterations on the vector-P2 {args.p2_n}^3 "
+                                         "matrix (20 per solve, 3 solves)"}
         if args.batch > 0 and want("batch"):
             sec["batch"] = bench_batch(args.batch, log, settle_s=args.settle)
         result["secondary"] = sec
