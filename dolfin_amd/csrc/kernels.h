@@ -184,7 +184,7 @@ struct CgState {
   int32_t broke;  // p.Ap = 0 with r != 0
   int32_t pad;
 };
-constexpr int kCgMaxGroups = 1024;
+constexpr int kCgMaxGroups = 512;
 hipError_t launch_cg_init(int64_t n, const double* b, const double* r, const double* dinv, double* p,
                           double* part, CgState* st, hipStream_t s);
 hipError_t launch_cg_iteration(const CsrView& A, int64_t it, double* x, double* p, double* r, double* ap,
